@@ -1,0 +1,905 @@
+// Device numeric factorization: the symbolic plan is compiled once into a
+// program of batched kernel launches (per elimination-tree level), replayed
+// for every numeric factorization of the same pattern.
+//
+// Per level (all supernodes of equal height together):
+//   1. assemble fronts: zero, scatter A, extend-add children updates
+//   2. uncompressed supernodes (batched over the level): blocked Cholesky of
+//      the (ncols+R) x ncols panel, then update -= L21 L21^T
+//   3. compressed separators: in-place HODLR factor of the diagonal in the
+//      reference's traversal order (FactorManager.cpp:8907-8992), explicit
+//      blocks factored densely then compressed (DiagonalBlock.cpp:393-453,
+//      1384-1615), recursive off-diagonal blocks compressed from
+//      S L_left^{-T} (FactorManager.cpp:8998-9036); then L21 = F21 L11^{-T},
+//      the randomized range finder with power iteration on L21
+//      (FactorManager.cpp:8193-8297, 8672-8779), V = L21 U
+//      (FactorManager.cpp:5691-5751), and update -= V (U^T U) V^T
+//      (Supernode.cpp:1266-1429).
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+
+#include "factor.h"
+#include "solve.h"
+
+namespace rsc {
+
+namespace {
+
+enum OpKind {
+  OP_ASM = 0,
+  OP_GEMM_NN,
+  OP_GEMM_TN,
+  OP_GEMM_NT,
+  OP_GEMM_TT,
+  OP_POTRF,
+  OP_TRSM,
+  OP_COPY,
+  OP_SHIFT
+};
+
+struct Step {
+  int op;
+  int count;
+  int64_t first;
+};
+
+struct CholMat {
+  char* A;
+  int n, m, ld, node;
+  int tolerant;
+};
+struct TrsmMat {
+  char* X;
+  int m, ldx;
+  const char* L;
+  int n, ldl;
+};
+struct SketchJob {
+  char* X;
+  int rows, cols, ldx;
+  const char* G;  // r x rows column-major (ld r)
+  int r;
+  char* U;  // cols x r, ld cols (also holds Y)
+  char* V;  // rows x r, ld rows
+  char* Z;  // rows x r
+  char* W;  // r x r
+  int node;
+};
+
+template <typename T>
+class Engine final : public FactorEngine {
+ public:
+  Engine(const Plan& plan, const Csc& A, hipStream_t st)
+      : plan_(plan), A_(A), st_(st) {
+    setup_static();
+  }
+  ~Engine() override { (void)hipStreamSynchronize(st_); }
+
+  void numeric(const double* Ax) override;
+  void apply(int k, const double* B, int ldb, double* X, int ldx,
+             hipStream_t s) override;
+  int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
+  int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
+  void release_workspace() override {
+    (void)hipStreamSynchronize(st_);
+    work_.reset();
+    program_ready_ = false;
+  }
+
+ private:
+  const Plan& plan_;
+  const Csc& A_;
+  hipStream_t st_;
+  DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
+      drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
+      dctiles_, dsjobs_, dstatus_;
+  DBuf dsnodes_, dhtree_, dhops_, drows_, dlevnodes_, dperm_, dfv_, dw_;
+  int64_t fv_stride_ = 0;
+  int fv_k_ = 0;
+  std::vector<std::vector<int>> levnode_off_;  // per level: offset in dlevnodes
+  std::vector<int> lev_off_;
+  bool program_ready_ = false;
+  // program
+  std::vector<Step> prog_;
+  std::vector<dev::GemmJob> gjobs_;
+  std::vector<dev::GemmTile> gtiles_;
+  std::vector<dev::PotrfJob> pjobs_;
+  std::vector<dev::TrsmJob> tjobs_;
+  std::vector<dev::RowTile> ttiles_;
+  std::vector<dev::CopyJob> cjobs_;
+  std::vector<dev::RowTile> ctiles_;
+  std::vector<dev::ShiftJob> sjobs_;
+  std::vector<dev::AsmTask> tasks_;
+  int64_t scratch_elems_ = 0;
+  std::vector<int64_t> node_scratch_;   // per supernode
+  std::vector<std::vector<int64_t>> hnode_scratch_;
+
+  T* F() const { return factor_.as<T>(); }
+  T* Wb() const { return work_.as<T>(); }
+  char* fp(int64_t off) const { return (char*)(F() + off); }
+  char* wp(int64_t off) const { return (char*)(Wb() + off); }
+  int64_t upd_base() const { return plan_.work_elems; }
+  int64_t scratch_base() const { return plan_.work_elems + plan_.update_elems; }
+  int64_t rng_base() const { return scratch_base() + scratch_elems_; }
+
+  void setup_static();
+  void build_program();
+  void layout_scratch();
+
+  // emission helpers
+  void emit_gemm(int op, const std::vector<dev::GemmJob>& jobs, bool lower);
+  void emit_potrf(const std::vector<dev::PotrfJob>& jobs);
+  void emit_trsm(const std::vector<dev::TrsmJob>& jobs);
+  void emit_copy(const std::vector<dev::CopyJob>& jobs);
+  void panel_chol(const std::vector<CholMat>& mats);
+  void trsm_right(const std::vector<TrsmMat>& mats);
+  void qr(const std::vector<struct QrMat>& jobs);
+  void sketch(const std::vector<SketchJob>& jobs);
+  void lowrank_update(char* C, int ldc, int rows, const char* V, const char* U,
+                      int cols, int r, char* K, char* Tm);
+  void hsolve_right(const Supernode& s, int q, char* X, int m, int ldx, char* Tm);
+  void emit_compressed(int j);
+};
+
+template <typename T>
+void Engine<T>::emit_gemm(int op, const std::vector<dev::GemmJob>& jobs,
+                          bool lower) {
+  Step st{op, 0, (int64_t)gtiles_.size()};
+  for (const auto& J : jobs) {
+    if (J.m <= 0 || J.n <= 0) continue;
+    int jid = (int)gjobs_.size();
+    gjobs_.push_back(J);
+    int tm = (J.m + dev::TILE - 1) / dev::TILE, tn = (J.n + dev::TILE - 1) / dev::TILE;
+    for (int a = 0; a < tm; ++a)
+      for (int b = 0; b < tn; ++b) {
+        if (lower && a < b) continue;
+        gtiles_.push_back({jid, a, b});
+      }
+  }
+  st.count = (int)(gtiles_.size() - st.first);
+  if (st.count > 0) prog_.push_back(st);
+}
+
+template <typename T>
+void Engine<T>::emit_potrf(const std::vector<dev::PotrfJob>& jobs) {
+  if (jobs.empty()) return;
+  Step st{OP_POTRF, (int)jobs.size(), (int64_t)pjobs_.size()};
+  pjobs_.insert(pjobs_.end(), jobs.begin(), jobs.end());
+  prog_.push_back(st);
+}
+
+template <typename T>
+void Engine<T>::emit_trsm(const std::vector<dev::TrsmJob>& jobs) {
+  Step st{OP_TRSM, 0, (int64_t)ttiles_.size()};
+  for (const auto& J : jobs) {
+    if (J.m <= 0) continue;
+    int jid = (int)tjobs_.size();
+    tjobs_.push_back(J);
+    for (int r = 0; r < J.m; r += dev::TILE) ttiles_.push_back({jid, r});
+  }
+  st.count = (int)(ttiles_.size() - st.first);
+  if (st.count > 0) prog_.push_back(st);
+}
+
+template <typename T>
+void Engine<T>::emit_copy(const std::vector<dev::CopyJob>& jobs) {
+  Step st{OP_COPY, 0, (int64_t)ctiles_.size()};
+  for (const auto& J : jobs) {
+    if (J.m <= 0 || J.n <= 0) continue;
+    int jid = (int)cjobs_.size();
+    cjobs_.push_back(J);
+    for (int c = 0; c < J.n; c += dev::TILE) ctiles_.push_back({jid, c});
+  }
+  st.count = (int)(ctiles_.size() - st.first);
+  if (st.count > 0) prog_.push_back(st);
+}
+
+// Blocked right-looking Cholesky of the leading n columns of an m x n panel
+// (m >= n): the trailing rows below n receive the triangular solve.
+template <typename T>
+void Engine<T>::panel_chol(const std::vector<CholMat>& mats) {
+  const int NB = dev::TILE;
+  int maxn = 0;
+  for (auto& M : mats) maxn = std::max(maxn, M.n);
+  for (int s0 = 0; s0 < maxn; s0 += NB) {
+    std::vector<dev::PotrfJob> pj;
+    std::vector<dev::TrsmJob> tj;
+    std::vector<dev::GemmJob> gj;
+    for (auto& M : mats) {
+      if (M.n <= s0) continue;
+      const int nb = std::min(NB, M.n - s0);
+      T* A = (T*)M.A;
+      T* D = A + s0 + (int64_t)s0 * M.ld;
+      pj.push_back({D, nb, M.ld, M.node, M.tolerant});
+      const int below = M.m - s0 - nb;
+      if (below > 0) tj.push_back({D, D + nb, below, nb, M.ld, M.ld});
+      const int s1 = s0 + nb;
+      if (M.n > s1) {
+        dev::GemmJob g;
+        g.A = A + s1 + (int64_t)s0 * M.ld;
+        g.B = A + s1 + (int64_t)s0 * M.ld;
+        g.C = A + s1 + (int64_t)s1 * M.ld;
+        g.m = M.m - s1;
+        g.n = M.n - s1;
+        g.k = nb;
+        g.lda = g.ldb = g.ldc = M.ld;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        gj.push_back(g);
+      }
+    }
+    emit_potrf(pj);
+    emit_trsm(tj);
+    emit_gemm(OP_GEMM_NT, gj, true);
+  }
+}
+
+// X(m x n) <- X L^{-T}, L n x n lower, blocked by TILE columns.
+template <typename T>
+void Engine<T>::trsm_right(const std::vector<TrsmMat>& mats) {
+  const int NB = dev::TILE;
+  int maxn = 0;
+  for (auto& M : mats) maxn = std::max(maxn, M.n);
+  for (int s0 = 0; s0 < maxn; s0 += NB) {
+    std::vector<dev::TrsmJob> tj;
+    std::vector<dev::GemmJob> gj;
+    for (auto& M : mats) {
+      if (M.n <= s0 || M.m <= 0) continue;
+      const int nb = std::min(NB, M.n - s0);
+      T* X = (T*)M.X;
+      const T* L = (const T*)M.L;
+      tj.push_back({L + s0 + (int64_t)s0 * M.ldl, X + (int64_t)s0 * M.ldx, M.m, nb,
+                    M.ldl, M.ldx});
+      const int s1 = s0 + nb;
+      if (M.n > s1) {
+        dev::GemmJob g;
+        g.A = X + (int64_t)s0 * M.ldx;
+        g.B = L + s1 + (int64_t)s0 * M.ldl;
+        g.C = X + (int64_t)s1 * M.ldx;
+        g.m = M.m;
+        g.n = M.n - s1;
+        g.k = nb;
+        g.lda = M.ldx;
+        g.ldb = M.ldl;
+        g.ldc = M.ldx;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        gj.push_back(g);
+      }
+    }
+    emit_trsm(tj);
+    emit_gemm(OP_GEMM_NT, gj, false);
+  }
+}
+
+// Thin QR by shifted CholeskyQR3 (Fukaya et al. 2020: one shifted pass, then
+// CholeskyQR2): each Y (m x r) becomes an orthonormal basis of its range.
+struct QrMat {
+  char* Y;
+  int m, r;
+  char* W;  // r x r scratch
+  int node;
+};
+
+template <typename T>
+void Engine<T>::qr(const std::vector<QrMat>& jobs) {
+  for (int pass = 0; pass < 3; ++pass) {
+    std::vector<dev::GemmJob> gram;
+    std::vector<dev::ShiftJob> sh;
+    std::vector<CholMat> ch;
+    std::vector<TrsmMat> tr;
+    for (auto& J : jobs) {
+      dev::GemmJob g;
+      g.A = J.Y;
+      g.B = J.Y;
+      g.C = J.W;
+      g.m = J.r;
+      g.n = J.r;
+      g.k = J.m;
+      g.lda = g.ldb = J.m;
+      g.ldc = J.r;
+      g.alpha = 1.0;
+      g.beta = 0.0;
+      gram.push_back(g);
+      sh.push_back({J.W, J.r, J.r, J.m, 0});
+      ch.push_back({J.W, J.r, J.r, J.r, J.node, 1});
+      tr.push_back({J.Y, J.m, J.m, J.W, J.r, J.r});
+    }
+    emit_gemm(OP_GEMM_TN, gram, false);
+    if (pass == 0) {
+      Step st{OP_SHIFT, (int)sh.size(), (int64_t)sjobs_.size()};
+      sjobs_.insert(sjobs_.end(), sh.begin(), sh.end());
+      prog_.push_back(st);
+    }
+    panel_chol(ch);
+    trsm_right(tr);
+  }
+}
+
+// Randomized range finder on X (rows x cols): Y = X^T G, power iterations
+// Y = X^T (X Y), orthonormal basis U of range(Y), V = X U.  Every product is
+// orthonormalized before the next multiply (subspace iteration): the range
+// is that of the un-normalized iteration, but each QR sees cond(X) instead
+// of cond(X)^3.
+template <typename T>
+void Engine<T>::sketch(const std::vector<SketchJob>& jobs) {
+  std::vector<dev::GemmJob> g0, g1, g2, g3;
+  std::vector<QrMat> qu, qz;
+  for (auto& J : jobs) {
+    dev::GemmJob g;
+    g.A = J.X;
+    g.B = J.G;
+    g.C = J.U;
+    g.m = J.cols;
+    g.n = J.r;
+    g.k = J.rows;
+    g.lda = J.ldx;
+    g.ldb = J.r;
+    g.ldc = J.cols;
+    g.alpha = 1.0;
+    g.beta = 0.0;
+    g0.push_back(g);
+    qu.push_back({J.U, J.cols, J.r, J.W, J.node});
+    qz.push_back({J.Z, J.rows, J.r, J.W, J.node});
+  }
+  emit_gemm(OP_GEMM_TT, g0, false);
+  qr(qu);
+  for (int it = 0; it < plan_.opt.power_iterations; ++it) {
+    g1.clear();
+    g2.clear();
+    for (auto& J : jobs) {
+      dev::GemmJob a;
+      a.A = J.X;
+      a.B = J.U;
+      a.C = J.Z;
+      a.m = J.rows;
+      a.n = J.r;
+      a.k = J.cols;
+      a.lda = J.ldx;
+      a.ldb = J.cols;
+      a.ldc = J.rows;
+      a.alpha = 1.0;
+      a.beta = 0.0;
+      g1.push_back(a);
+      dev::GemmJob b;
+      b.A = J.X;
+      b.B = J.Z;
+      b.C = J.U;
+      b.m = J.cols;
+      b.n = J.r;
+      b.k = J.rows;
+      b.lda = J.ldx;
+      b.ldb = J.rows;
+      b.ldc = J.cols;
+      b.alpha = 1.0;
+      b.beta = 0.0;
+      g2.push_back(b);
+    }
+    emit_gemm(OP_GEMM_NN, g1, false);
+    qr(qz);
+    emit_gemm(OP_GEMM_TN, g2, false);
+    qr(qu);
+  }
+  for (auto& J : jobs) {
+    dev::GemmJob g;
+    g.A = J.X;
+    g.B = J.U;
+    g.C = J.V;
+    g.m = J.rows;
+    g.n = J.r;
+    g.k = J.cols;
+    g.lda = J.ldx;
+    g.ldb = J.cols;
+    g.ldc = J.rows;
+    g.alpha = 1.0;
+    g.beta = 0.0;
+    g3.push_back(g);
+  }
+  emit_gemm(OP_GEMM_NN, g3, false);
+}
+
+// C(rows x rows, lower) -= V (U^T U) V^T
+template <typename T>
+void Engine<T>::lowrank_update(char* C, int ldc, int rows, const char* V,
+                               const char* U, int cols, int r, char* K, char* Tm) {
+  dev::GemmJob g;
+  g.A = U;
+  g.B = U;
+  g.C = K;
+  g.m = r;
+  g.n = r;
+  g.k = cols;
+  g.lda = g.ldb = cols;
+  g.ldc = r;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  emit_gemm(OP_GEMM_TN, {g}, false);
+  g.A = V;
+  g.B = K;
+  g.C = Tm;
+  g.m = rows;
+  g.n = r;
+  g.k = r;
+  g.lda = rows;
+  g.ldb = r;
+  g.ldc = rows;
+  emit_gemm(OP_GEMM_NN, {g}, false);
+  g.A = Tm;
+  g.B = V;
+  g.C = C;
+  g.m = rows;
+  g.n = rows;
+  g.k = r;
+  g.lda = rows;
+  g.ldb = rows;
+  g.ldc = ldc;
+  g.alpha = -1.0;
+  g.beta = 1.0;
+  emit_gemm(OP_GEMM_NT, {g}, true);
+}
+
+// X (m x width of node q) <- X L_q^{-T} through the HODLR tree
+// (DiagonalBlock.cpp:1151-1300, right-side transpose branch).
+template <typename T>
+void Engine<T>::hsolve_right(const Supernode& s, int q, char* X, int m, int ldx,
+                             char* Tm) {
+  const HNode& h = s.hnodes[q];
+  if (h.leaf()) {
+    int b = h.r1 - h.r0;
+    trsm_right({{X, m, ldx, fp(h.off_leaf), b, b}});
+    return;
+  }
+  const HNode& L = s.hnodes[h.left];
+  const HNode& R = s.hnodes[h.right];
+  const int cl = L.r1 - L.r0, cr = R.r1 - R.r0;
+  char* Xl = (char*)((T*)X + (int64_t)(L.r0 - h.r0) * ldx);
+  char* Xr = (char*)((T*)X + (int64_t)(R.r0 - h.r0) * ldx);
+  hsolve_right(s, h.left, Xl, m, ldx, Tm);
+  dev::GemmJob g;
+  g.A = Xl;
+  g.B = fp(h.off_U);
+  g.C = Tm;
+  g.m = m;
+  g.n = h.rank;
+  g.k = cl;
+  g.lda = ldx;
+  g.ldb = cl;
+  g.ldc = m;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  emit_gemm(OP_GEMM_NN, {g}, false);
+  g.A = Tm;
+  g.B = fp(h.off_V);
+  g.C = Xr;
+  g.m = m;
+  g.n = cr;
+  g.k = h.rank;
+  g.lda = m;
+  g.ldb = cr;
+  g.ldc = ldx;
+  g.alpha = -1.0;
+  g.beta = 1.0;
+  emit_gemm(OP_GEMM_NT, {g}, false);
+  hsolve_right(s, h.right, Xr, m, ldx, Tm);
+}
+
+template <typename T>
+void Engine<T>::layout_scratch() {
+  // per compressed supernode: Z (R r), W (r r), K (r r), Tm (max(R,c) rmax);
+  // per internal tree node: Z (rows r), W, K (r r), Tm (rows r)
+  int64_t off = 0;
+  node_scratch_.assign(plan_.nodes.size(), -1);
+  hnode_scratch_.assign(plan_.nodes.size(), {});
+  for (size_t j = 0; j < plan_.nodes.size(); ++j) {
+    const Supernode& s = plan_.nodes[j];
+    if (!s.compress) continue;
+    int rmax = s.rank;
+    for (auto& h : s.hnodes) rmax = std::max(rmax, h.rank);
+    node_scratch_[j] = off;
+    const int64_t R = s.R(), r = s.rank;
+    off += R * r + 2 * r * r + (int64_t)std::max<int64_t>(R, s.ncols) * rmax;
+    hnode_scratch_[j].assign(s.hnodes.size(), -1);
+    for (size_t q = 0; q < s.hnodes.size(); ++q) {
+      const HNode& h = s.hnodes[q];
+      if (h.leaf()) continue;
+      const int64_t rows = s.hnodes[h.right].r1 - s.hnodes[h.right].r0;
+      hnode_scratch_[j][q] = off;
+      off += 2 * rows * h.rank + 2 * (int64_t)h.rank * h.rank;
+    }
+  }
+  scratch_elems_ = off;
+}
+
+template <typename T>
+void Engine<T>::emit_compressed(int j) {
+  const Supernode& s = plan_.nodes[j];
+  const int c = s.ncols, R = s.R(), ld = c + R;
+  T* P = Wb() + s.off_work_panel;
+  const int64_t nsb = scratch_base() + node_scratch_[j];
+  int rmax = s.rank;
+  for (auto& h : s.hnodes) rmax = std::max(rmax, h.rank);
+  char* Tm = wp(nsb + (int64_t)R * s.rank + 2 * (int64_t)s.rank * s.rank);
+  const T* G = Wb() + rng_base();
+  auto hsk = [&](int q) -> SketchJob {
+    const HNode& h = s.hnodes[q];
+    const HNode& L = s.hnodes[h.left];
+    const HNode& Rt = s.hnodes[h.right];
+    const int rows = Rt.r1 - Rt.r0, cols = L.r1 - L.r0;
+    const int64_t sb = scratch_base() + hnode_scratch_[j][q];
+    SketchJob J;
+    J.X = (char*)(P + Rt.r0 + (int64_t)L.r0 * ld);
+    J.rows = rows;
+    J.cols = cols;
+    J.ldx = ld;
+    J.G = (const char*)(G + h.rng_off);
+    J.r = h.rank;
+    J.U = fp(h.off_U);
+    J.V = fp(h.off_V);
+    J.Z = wp(sb);
+    J.W = wp(sb + (int64_t)rows * h.rank);
+    J.node = j;
+    return J;
+  };
+  for (const HStep& st : s.hsteps) {
+    const HNode& h = s.hnodes[st.node];
+    const int b = h.r1 - h.r0;
+    if (st.kind == HStep::EXPLICIT || st.kind == HStep::LEAF) {
+      panel_chol({{(char*)(P + h.r0 + (int64_t)h.r0 * ld), b, b, ld, j, 0}});
+      std::vector<SketchJob> sk;
+      std::vector<dev::CopyJob> cp;
+      std::function<void(int)> pre = [&](int q) {
+        const HNode& x = s.hnodes[q];
+        if (x.leaf()) {
+          int bb = x.r1 - x.r0;
+          cp.push_back({P + x.r0 + (int64_t)x.r0 * ld, F() + x.off_leaf, bb, bb, ld, bb});
+          return;
+        }
+        sk.push_back(hsk(q));
+        pre(x.left);
+        pre(x.right);
+      };
+      pre(st.node);
+      if (!sk.empty()) sketch(sk);
+      emit_copy(cp);
+    } else {
+      // recursive off-diagonal block: L_N = S_N L_left^{-T}, compress, update
+      SketchJob J = hsk(st.node);
+      hsolve_right(s, h.left, J.X, J.rows, ld, Tm);
+      sketch({J});
+      const HNode& Rt = s.hnodes[h.right];
+      const int64_t sb = scratch_base() + hnode_scratch_[j][st.node];
+      char* K = wp(sb + (int64_t)J.rows * h.rank + (int64_t)h.rank * h.rank);
+      char* T2 = wp(sb + (int64_t)J.rows * h.rank + 2 * (int64_t)h.rank * h.rank);
+      lowrank_update((char*)(P + Rt.r0 + (int64_t)Rt.r0 * ld), ld, J.rows, J.V, J.U,
+                     J.cols, h.rank, K, T2);
+    }
+  }
+  if (R > 0) {
+    int root = 0;
+    for (int q = 0; q < (int)s.hnodes.size(); ++q)
+      if (s.hnodes[q].parent < 0) root = q;
+    char* X = (char*)(P + c);
+    hsolve_right(s, root, X, R, ld, Tm);
+    SketchJob J;
+    J.X = X;
+    J.rows = R;
+    J.cols = c;
+    J.ldx = ld;
+    J.G = (const char*)(G + s.rng_off);
+    J.r = s.rank;
+    J.U = fp(s.off_U);
+    J.V = fp(s.off_V);
+    J.Z = wp(nsb);
+    J.W = wp(nsb + (int64_t)R * s.rank);
+    J.node = j;
+    sketch({J});
+    char* K = wp(nsb + (int64_t)R * s.rank + (int64_t)s.rank * s.rank);
+    lowrank_update(wp(upd_base() + s.off_update), R, R, J.V, J.U, c, s.rank, K, Tm);
+  }
+}
+
+template <typename T>
+void Engine<T>::setup_static() {
+  const Plan& P = plan_;
+  const int ns = (int)P.nodes.size();
+  // permuted lower pattern and the destination of every entry in its panel
+  LowerCsc L = permuted_lower(A_, P.pinv);
+  std::vector<int64_t> a_dst(L.i.size());
+  for (int j = 0; j < ns; ++j) {
+    const Supernode& s = P.nodes[j];
+    const int c = s.ncols, ld = c + s.R(), last = s.first + c - 1;
+    for (int k = s.first; k <= last; ++k)
+      for (int e = L.p[k]; e < L.p[k + 1]; ++e) {
+        int r = L.i[e];
+        int pos;
+        if (r <= last) pos = r - s.first;
+        else pos = c + (int)(std::lower_bound(s.rows.begin(), s.rows.end(), r) - s.rows.begin());
+        a_dst[e] = (int64_t)pos + (int64_t)(k - s.first) * ld;
+      }
+  }
+  dlowp_.upload(L.p, st_);
+  da_dst_.upload(a_dst, st_);
+  da_src_.upload(L.src, st_);
+  // children, relmaps, rows
+  std::vector<int> children, relmap, rows;
+  std::vector<int> child_off(ns), relmap_off(ns);
+  std::vector<int64_t> rows_off(ns);
+  for (int j = 0; j < ns; ++j) {
+    const Supernode& s = P.nodes[j];
+    child_off[j] = (int)children.size();
+    children.insert(children.end(), s.children.begin(), s.children.end());
+    relmap_off[j] = (int)relmap.size();
+    relmap.insert(relmap.end(), s.relmap.begin(), s.relmap.end());
+    if (s.relmap.empty()) relmap.resize(relmap.size() + s.rows.size(), 0);
+    rows_off[j] = (int64_t)rows.size();
+    rows.insert(rows.end(), s.rows.begin(), s.rows.end());
+  }
+  dchildren_.upload(children, st_);
+  drelmap_.upload(relmap, st_);
+  drows_.upload(rows, st_);
+  std::vector<int> permv(P.perm);
+  dperm_.upload(permv, st_);
+  // solve descriptors
+  std::vector<SolveNode> sn(ns);
+  std::vector<SolveHNode> ht;
+  std::vector<int> hops;
+  int64_t fvo = 0;
+  for (int j = 0; j < ns; ++j) {
+    const Supernode& s = P.nodes[j];
+    SolveNode& d = sn[j];
+    d.first = s.first;
+    d.ncols = s.ncols;
+    d.R = s.R();
+    d.compress = s.compress ? 1 : 0;
+    d.panel = s.off_panel;
+    d.V = s.off_V;
+    d.U = s.off_U;
+    d.rank = s.rank;
+    d.htree = (int)ht.size();
+    d.hroot = 0;
+    d.child_off = child_off[j];
+    d.nchild = (int)s.children.size();
+    d.relmap_off = relmap_off[j];
+    d.rows_off = rows_off[j];
+    d.fv_off = fvo;
+    fvo += s.ncols + s.R();
+    d.hops_off = (int)hops.size();
+    d.nhops = 0;
+    if (s.compress) {
+      int root = 0;
+      for (size_t q = 0; q < s.hnodes.size(); ++q)
+        if (s.hnodes[q].parent < 0) root = (int)q;
+      std::function<void(int)> inorder = [&](int q) {
+        const HNode& h = s.hnodes[q];
+        if (!h.leaf()) inorder(h.left);
+        hops.push_back(q);
+        if (!h.leaf()) inorder(h.right);
+      };
+      inorder(root);
+      d.nhops = (int)hops.size() - d.hops_off;
+      for (size_t q = 0; q < s.hnodes.size(); ++q) {
+        const HNode& h = s.hnodes[q];
+        SolveHNode x;
+        x.r0 = h.r0;
+        x.r1 = h.r1;
+        x.left = h.left;
+        x.right = h.right;
+        x.rank = h.rank;
+        x.pad = 0;
+        x.leaf = h.off_leaf;
+        x.V = h.off_V;
+        x.U = h.off_U;
+        ht.push_back(x);
+        if (h.parent < 0) d.hroot = (int)q;
+      }
+    }
+  }
+  fv_stride_ = fvo;
+  dsnodes_.upload(sn, st_);
+  dhtree_.upload(ht, st_);
+  dhops_.upload(hops, st_);
+  std::vector<int> levnodes;
+  lev_off_.clear();
+  for (auto& lv : P.levels) {
+    lev_off_.push_back((int)levnodes.size());
+    levnodes.insert(levnodes.end(), lv.begin(), lv.end());
+  }
+  lev_off_.push_back((int)levnodes.size());
+  dlevnodes_.upload(levnodes, st_);
+  std::vector<int> status = {0, INT_MAX};
+  dstatus_.upload(status, st_);
+  factor_.alloc(std::max<int64_t>(P.factor_elems, 1) * sizeof(T));
+  layout_scratch();
+  RSC_HIP(hipStreamSynchronize(st_));
+}
+
+template <typename T>
+void Engine<T>::build_program() {
+  const Plan& P = plan_;
+  const int ns = (int)P.nodes.size();
+  prog_.clear();
+  gjobs_.clear();
+  gtiles_.clear();
+  pjobs_.clear();
+  tjobs_.clear();
+  ttiles_.clear();
+  cjobs_.clear();
+  ctiles_.clear();
+  sjobs_.clear();
+  tasks_.clear();
+  const int64_t welems = rng_base() + P.rng_count;
+  work_.alloc(std::max<int64_t>(welems, 1) * sizeof(T));
+  // Gaussian sketches: the host MT19937 stream, converted on the device
+  if (P.rng_count > 0) {
+    std::vector<double> g(P.rng_count);
+    gaussian_stream(P.opt.seed, P.rng_count, g.data());
+    DBuf tmp;
+    tmp.upload(g, st_);
+    dev::launch_convert<T>(tmp.as<double>(), Wb() + rng_base(), P.rng_count, st_);
+    RSC_HIP(hipStreamSynchronize(st_));
+  }
+  // front descriptors
+  std::vector<dev::FrontDev> fr(ns);
+  {
+    int co = 0;
+    int64_t ro = 0;
+    for (int j = 0; j < ns; ++j) {
+      const Supernode& s = P.nodes[j];
+      dev::FrontDev& f = fr[j];
+      f.panel = s.compress ? (int64_t)((Wb() + s.off_work_panel) - F())
+                           : s.off_panel;
+      f.update = s.R() > 0 ? (int64_t)((Wb() + upd_base() + s.off_update) - F()) : 0;
+      f.ncols = s.ncols;
+      f.R = s.R();
+      f.first = s.first;
+      f.child_off = co;
+      f.nchild = (int)s.children.size();
+      co += f.nchild;
+      f.relmap_off = ro;
+      ro += (int64_t)s.rows.size();
+    }
+  }
+  dfronts_.upload(fr, st_);
+  for (size_t lv = 0; lv < P.levels.size(); ++lv) {
+    const std::vector<int>& nodes = P.levels[lv];
+    Step st{OP_ASM, 0, (int64_t)tasks_.size()};
+    for (int j : nodes) {
+      const int w = P.nodes[j].ncols + P.nodes[j].R();
+      for (int c0 = 0; c0 < w; c0 += 16) tasks_.push_back({j, c0});
+    }
+    st.count = (int)(tasks_.size() - st.first);
+    if (st.count > 0) prog_.push_back(st);
+    std::vector<CholMat> mats;
+    std::vector<dev::GemmJob> syrk;
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      if (s.compress) continue;
+      const int c = s.ncols, R = s.R(), ld = c + R;
+      mats.push_back({fp(s.off_panel), c, ld, ld, j, 0});
+      if (R > 0) {
+        dev::GemmJob g;
+        g.A = F() + s.off_panel + c;
+        g.B = F() + s.off_panel + c;
+        g.C = Wb() + upd_base() + s.off_update;
+        g.m = R;
+        g.n = R;
+        g.k = c;
+        g.lda = g.ldb = ld;
+        g.ldc = R;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        syrk.push_back(g);
+      }
+    }
+    panel_chol(mats);
+    emit_gemm(OP_GEMM_NT, syrk, true);
+    for (int j : nodes)
+      if (P.nodes[j].compress) emit_compressed(j);
+  }
+  dgjobs_.upload(gjobs_, st_);
+  dgtiles_.upload(gtiles_, st_);
+  dpjobs_.upload(pjobs_, st_);
+  dtjobs_.upload(tjobs_, st_);
+  dttiles_.upload(ttiles_, st_);
+  dcjobs_.upload(cjobs_, st_);
+  dctiles_.upload(ctiles_, st_);
+  dsjobs_.upload(sjobs_, st_);
+  dtasks_.upload(tasks_, st_);
+  RSC_HIP(hipStreamSynchronize(st_));
+  program_ready_ = true;
+}
+
+template <typename T>
+void Engine<T>::numeric(const double* Ax) {
+  if (!program_ready_) build_program();
+  std::vector<double> ax(Ax, Ax + A_.nnz());
+  dAx_.upload(ax, st_);
+  int status0[2] = {0, INT_MAX};
+  RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
+                         hipMemcpyHostToDevice, st_));
+  for (const Step& s : prog_) {
+    switch (s.op) {
+      case OP_ASM:
+        dev::launch_assemble<T>(F(), dfronts_.as<dev::FrontDev>(),
+                                dtasks_.as<dev::AsmTask>() + s.first, s.count,
+                                dchildren_.as<int>(), drelmap_.as<int>(),
+                                dlowp_.as<int>(), da_dst_.as<int64_t>(),
+                                da_src_.as<int64_t>(), dAx_.as<double>(), st_);
+        break;
+      case OP_GEMM_NN:
+      case OP_GEMM_TN:
+      case OP_GEMM_NT:
+      case OP_GEMM_TT: {
+        bool ta = s.op == OP_GEMM_TN || s.op == OP_GEMM_TT;
+        bool tb = s.op == OP_GEMM_NT || s.op == OP_GEMM_TT;
+        dev::launch_gemm<T>(ta, tb, dgjobs_.as<dev::GemmJob>(),
+                            dgtiles_.as<dev::GemmTile>() + s.first, s.count, st_);
+        break;
+      }
+      case OP_POTRF:
+        dev::launch_potrf<T>(dpjobs_.as<dev::PotrfJob>() + s.first, s.count,
+                             dstatus_.as<int>(), st_);
+        break;
+      case OP_TRSM:
+        dev::launch_trsm<T>(dtjobs_.as<dev::TrsmJob>(),
+                            dttiles_.as<dev::RowTile>() + s.first, s.count, st_);
+        break;
+      case OP_COPY:
+        dev::launch_copy<T>(dcjobs_.as<dev::CopyJob>(),
+                            dctiles_.as<dev::RowTile>() + s.first, s.count, st_);
+        break;
+      case OP_SHIFT:
+        dev::launch_shift<T>(dsjobs_.as<dev::ShiftJob>() + s.first, s.count, st_);
+        break;
+    }
+  }
+  RSC_HIP(hipGetLastError());
+  int status[2];
+  RSC_HIP(hipMemcpyAsync(status, dstatus_.p, sizeof(status), hipMemcpyDeviceToHost, st_));
+  RSC_HIP(hipStreamSynchronize(st_));
+  if (status[0] != 0)
+    throw IndefiniteError("indefinite diagonal block in supernode " +
+                          std::to_string(status[1]));
+}
+
+template <typename T>
+void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
+                      hipStream_t s) {
+  const int n = plan_.n;
+  if (k > fv_k_) {
+    dfv_.alloc((size_t)fv_stride_ * k * sizeof(double));
+    dw_.alloc((size_t)n * k * sizeof(double));
+    fv_k_ = k;
+  }
+  double* w = dw_.as<double>();
+  dev::launch_gather(B, ldb, w, dperm_.as<int>(), n, k, s);
+  const int nlev = (int)plan_.levels.size();
+  for (int lv = 0; lv < nlev; ++lv)
+    dev::launch_forward<T>(F(), dsnodes_.as<SolveNode>(),
+                           dlevnodes_.as<int>() + lev_off_[lv],
+                           lev_off_[lv + 1] - lev_off_[lv], dhtree_.as<SolveHNode>(),
+                           dhops_.as<int>(), dchildren_.as<int>(), drelmap_.as<int>(), w, n, k,
+                           dfv_.as<double>(), fv_stride_, s);
+  for (int lv = nlev - 1; lv >= 0; --lv)
+    dev::launch_backward<T>(F(), dsnodes_.as<SolveNode>(),
+                            dlevnodes_.as<int>() + lev_off_[lv],
+                            lev_off_[lv + 1] - lev_off_[lv], dhtree_.as<SolveHNode>(),
+                            dhops_.as<int>(), drows_.as<int>(), w, n, k, dfv_.as<double>(),
+                            fv_stride_, s);
+  dev::launch_scatter(w, X, ldx, dperm_.as<int>(), n, k, s);
+}
+
+}  // namespace
+
+std::unique_ptr<FactorEngine> make_engine(const Plan& plan, const Csc& A,
+                                          hipStream_t stream) {
+  if (plan.opt.factor_precision == RSC_FP32)
+    return std::unique_ptr<FactorEngine>(new Engine<float>(plan, A, stream));
+  return std::unique_ptr<FactorEngine>(new Engine<double>(plan, A, stream));
+}
+
+}  // namespace rsc

@@ -1,0 +1,95 @@
+// Batched dense kernels of the device factor and solve (gfx950).  All
+// matrices are column-major.  Each launch processes a device array of jobs
+// of varying size; the host precomputes the tile lists.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rsc {
+namespace dev {
+
+constexpr int TILE = 64;  // GEMM output tile and POTRF/TRSM block width
+
+// C(m x n) = beta*C + alpha*op(A)*op(B); op(A) is m x k, op(B) is k x n.
+struct GemmJob {
+  const void* A;
+  const void* B;
+  void* C;
+  int m, n, k, lda, ldb, ldc;
+  double alpha, beta;
+};
+struct GemmTile {
+  int job, tm, tn;
+};
+
+// In-place Cholesky of an n x n (n <= TILE) lower block.  tolerant != 0
+// (Gram matrices of the thin QR): a non-positive pivot is replaced by
+// u * max(initial tile diagonal) instead of being reported.
+struct PotrfJob {
+  void* A;
+  int n, lda, node, tolerant;
+};
+
+// B(m x nb) <- B * L^{-T}, L nb x nb lower (nb <= TILE).
+struct TrsmJob {
+  const void* L;
+  void* B;
+  int m, nb, ldl, ldb;
+};
+struct RowTile {
+  int job, row0;
+};
+
+// dst(m x n) = src(m x n)
+struct CopyJob {
+  const void* src;
+  void* dst;
+  int m, n, lds, ldd;
+};
+
+// W(r x r) += s*I with s = 11 (m r + r (r+1)) u trace(W)
+struct ShiftJob {
+  void* W;
+  int r, ld, m, pad;
+};
+
+// per-node front geometry for the assembly kernel
+struct FrontDev {
+  int64_t panel;    // element offset of the (c+R) x c panel in the arena
+  int64_t update;   // element offset of the R x R update matrix (-1 if R==0)
+  int ncols, R;
+  int first;        // first permuted column
+  int child_off, nchild;
+  int64_t relmap_off;  // offset of this node's relmap (R entries)
+};
+struct AsmTask {
+  int node, col0;
+};
+
+template <typename T>
+void launch_gemm(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
+                 int ntiles, hipStream_t s);
+template <typename T>
+void launch_potrf(const PotrfJob* jobs, int njobs, int* status, hipStream_t s);
+template <typename T>
+void launch_trsm(const TrsmJob* jobs, const RowTile* tiles, int ntiles,
+                 hipStream_t s);
+template <typename T>
+void launch_copy(const CopyJob* jobs, const RowTile* tiles, int ntiles,
+                 hipStream_t s);
+template <typename T>
+void launch_shift(const ShiftJob* jobs, int njobs, hipStream_t s);
+// Assemble fronts: zero, scatter A (values converted to T), extend-add the
+// children's update matrices.
+template <typename T>
+void launch_assemble(T* arena, const FrontDev* fronts, const AsmTask* tasks,
+                     int ntasks, const int* children, const int* relmap,
+                     const int* lowp, const int64_t* a_dst, const int64_t* a_src,
+                     const double* Ax, hipStream_t s);
+template <typename T>
+void launch_convert(const double* src, T* dst, int64_t count, hipStream_t s);
+
+}  // namespace dev
+}  // namespace rsc

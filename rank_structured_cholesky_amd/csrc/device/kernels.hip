@@ -1,0 +1,402 @@
+// Batched dense kernels for gfx950 (CDNA4): MFMA GEMM, tile Cholesky,
+// tile triangular solve, 2D copies, CholeskyQR shift and the multifrontal
+// front assembly.  Compiled with -Xclang -target-feature -Xclang
+// -packed-fp32-ops disabled; no packed-FP32 instructions are written here.
+#include "kernels.h"
+
+#include <cmath>
+
+namespace rsc {
+namespace dev {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Acc;
+// Accumulator row of element v in lane l differs between the two shapes:
+// f64 16x16x4 holds rows (l/16) + 4v, f32 16x16x4 holds rows 4(l/16) + v
+// (column l%16 in both).
+template <>
+struct Acc<double> {
+  typedef d4 type;
+  static __device__ inline d4 mfma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Acc<float> {
+  typedef f4 type;
+  static __device__ inline f4 mfma(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// ---------------------------------------------------------------------------
+// GEMM: 64x64 output tile per 256-thread workgroup; each wave owns a 32x32
+// quadrant computed as 2x2 MFMA 16x16x4 blocks.  K is staged through LDS in
+// slabs of 16.  MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; accumulator element v of lane l is D[Acc<T>::row(l,v)][l%16].
+// ---------------------------------------------------------------------------
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
+                                              const GemmTile* __restrict__ tiles) {
+  constexpr int KS = 16;
+  __shared__ T As[KS][TILE + 1];
+  __shared__ T Bs[KS][TILE + 1];
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmJob J = jobs[tl.job];
+  const T* __restrict__ A = (const T*)J.A;
+  const T* __restrict__ B = (const T*)J.B;
+  T* __restrict__ C = (T*)J.C;
+  const int m0 = tl.tm * TILE, n0 = tl.tn * TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  typedef typename Acc<T>::type V;
+  V acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 4; ++v) acc[a][b][v] = 0;
+  for (int k0 = 0; k0 < J.k; k0 += KS) {
+    for (int e = tid; e < KS * TILE; e += 256) {
+      int i, kk;
+      if (!TA) { i = e % TILE; kk = e / TILE; }
+      else { kk = e % KS; i = e / KS; }
+      int gi = m0 + i, gk = k0 + kk;
+      T v = 0;
+      if (gi < J.m && gk < J.k)
+        v = TA ? A[gk + (int64_t)gi * J.lda] : A[gi + (int64_t)gk * J.lda];
+      As[kk][i] = v;
+    }
+    for (int e = tid; e < KS * TILE; e += 256) {
+      int j, kk;
+      if (!TB) { kk = e % KS; j = e / KS; }
+      else { j = e % TILE; kk = e / TILE; }
+      int gj = n0 + j, gk = k0 + kk;
+      T v = 0;
+      if (gj < J.n && gk < J.k)
+        v = TB ? B[gj + (int64_t)gk * J.ldb] : B[gk + (int64_t)gj * J.ldb];
+      Bs[kk][j] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 4) {
+      T a0 = As[kk + (lane >> 4)][wm + (lane & 15)];
+      T a1 = As[kk + (lane >> 4)][wm + 16 + (lane & 15)];
+      T b0 = Bs[kk + (lane >> 4)][wn + (lane & 15)];
+      T b1 = Bs[kk + (lane >> 4)][wn + 16 + (lane & 15)];
+      acc[0][0] = Acc<T>::mfma(a0, b0, acc[0][0]);
+      acc[0][1] = Acc<T>::mfma(a0, b1, acc[0][1]);
+      acc[1][0] = Acc<T>::mfma(a1, b0, acc[1][0]);
+      acc[1][1] = Acc<T>::mfma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  const T alpha = (T)J.alpha, beta = (T)J.beta;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 4; ++v) {
+        int gi = m0 + wm + a * 16 + Acc<T>::row(lane, v);
+        int gj = n0 + wn + b * 16 + (lane & 15);
+        if (gi < J.m && gj < J.n) {
+          T* p = C + gi + (int64_t)gj * J.ldc;
+          T r = alpha * acc[a][b][v];
+          if (beta != (T)0) r += beta * (*p);
+          *p = r;
+        }
+      }
+}
+
+template <typename T>
+void launch_gemm(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
+                 int ntiles, hipStream_t s) {
+  if (ntiles <= 0) return;
+  if (!ta && !tb) k_gemm<T, false, false><<<ntiles, 256, 0, s>>>(jobs, tiles);
+  else if (ta && !tb) k_gemm<T, true, false><<<ntiles, 256, 0, s>>>(jobs, tiles);
+  else if (!ta && tb) k_gemm<T, false, true><<<ntiles, 256, 0, s>>>(jobs, tiles);
+  else k_gemm<T, true, true><<<ntiles, 256, 0, s>>>(jobs, tiles);
+}
+
+// ---------------------------------------------------------------------------
+// Tile Cholesky (n <= 64) in LDS, right-looking.  A non-positive pivot sets
+// status[0] = 1 and status[1] = min(node); the pivot is replaced by 1 so the
+// remaining arithmetic stays finite (the factorization is then retried).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf(const PotrfJob* __restrict__ jobs,
+                                               int* __restrict__ status) {
+  __shared__ T S[TILE][TILE + 1];
+  __shared__ int bad;
+  __shared__ T dfloor;
+  const PotrfJob J = jobs[blockIdx.x];
+  T* A = (T*)J.A;
+  const int n = J.n, tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < n * n; e += 256) {
+    int i = e % n, j = e / n;
+    if (i >= j) S[i][j] = A[i + (int64_t)j * J.lda];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    T mx = 0;
+    for (int i = 0; i < n; ++i) mx = S[i][i] > mx ? S[i][i] : mx;
+    const T u = sizeof(T) == 8 ? (T)1.1102230246251565e-16 : (T)5.960464477539063e-08;
+    dfloor = mx > (T)0 ? mx * u : (T)1;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      T d = S[k][k];
+      if (!(d > (T)0)) {
+        if (J.tolerant) {
+          d = dfloor;
+        } else {
+          bad = 1;
+          d = (T)1;
+        }
+      }
+      S[k][k] = sqrt(d);
+    }
+    __syncthreads();
+    const T piv = S[k][k];
+    for (int i = k + 1 + tid; i < n; i += 256) S[i][k] /= piv;
+    __syncthreads();
+    const int w = n - k - 1;
+    for (int e = tid; e < w * w; e += 256) {
+      int i = k + 1 + e % w, j = k + 1 + e / w;
+      if (i >= j) S[i][j] -= S[i][k] * S[j][k];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += 256) {
+    int i = e % n, j = e / n;
+    if (i >= j) A[i + (int64_t)j * J.lda] = S[i][j];
+  }
+  if (tid == 0 && bad) {
+    atomicOr(&status[0], 1);
+    atomicMin(&status[1], J.node);
+  }
+}
+
+template <typename T>
+void launch_potrf(const PotrfJob* jobs, int njobs, int* status, hipStream_t s) {
+  if (njobs <= 0) return;
+  k_potrf<T><<<njobs, 256, 0, s>>>(jobs, status);
+}
+
+// ---------------------------------------------------------------------------
+// Row-block triangular solve B <- B L^{-T}: 64 rows of B per workgroup held
+// in LDS, forward substitution over the nb columns.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm(const TrsmJob* __restrict__ jobs,
+                                              const RowTile* __restrict__ tiles) {
+  __shared__ T Ls[TILE][TILE + 1];
+  __shared__ T Bs[TILE][TILE + 1];
+  const RowTile tl = tiles[blockIdx.x];
+  const TrsmJob J = jobs[tl.job];
+  const T* L = (const T*)J.L;
+  T* B = (T*)J.B;
+  const int nb = J.nb, tid = threadIdx.x;
+  const int rows = min(TILE, J.m - tl.row0);
+  for (int e = tid; e < nb * nb; e += 256) {
+    int i = e % nb, j = e / nb;
+    Ls[i][j] = (i >= j) ? L[i + (int64_t)j * J.ldl] : (T)0;
+  }
+  for (int e = tid; e < TILE * nb; e += 256) {
+    int r = e % TILE, j = e / TILE;
+    Bs[r][j] = (r < rows) ? B[(tl.row0 + r) + (int64_t)j * J.ldb] : (T)0;
+  }
+  __syncthreads();
+  const int r = tid & 63, g = tid >> 6;
+  for (int k = 0; k < nb; ++k) {
+    if (g == 0) Bs[r][k] /= Ls[k][k];
+    __syncthreads();
+    const T bk = Bs[r][k];
+    for (int j = k + 1 + g; j < nb; j += 4) Bs[r][j] -= bk * Ls[j][k];
+    __syncthreads();
+  }
+  for (int e = tid; e < TILE * nb; e += 256) {
+    int rr = e % TILE, j = e / TILE;
+    if (rr < rows) B[(tl.row0 + rr) + (int64_t)j * J.ldb] = Bs[rr][j];
+  }
+}
+
+template <typename T>
+void launch_trsm(const TrsmJob* jobs, const RowTile* tiles, int ntiles,
+                 hipStream_t s) {
+  if (ntiles <= 0) return;
+  k_trsm<T><<<ntiles, 256, 0, s>>>(jobs, tiles);
+}
+
+// ---------------------------------------------------------------------------
+// 2D copy; tiles of 64 columns.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy(const CopyJob* __restrict__ jobs,
+                                              const RowTile* __restrict__ tiles) {
+  const RowTile tl = tiles[blockIdx.x];
+  const CopyJob J = jobs[tl.job];
+  const T* src = (const T*)J.src;
+  T* dst = (T*)J.dst;
+  const int c0 = tl.row0, c1 = min(J.n, c0 + TILE);
+  for (int j = c0; j < c1; ++j)
+    for (int i = threadIdx.x; i < J.m; i += 256)
+      dst[i + (int64_t)j * J.ldd] = src[i + (int64_t)j * J.lds];
+}
+
+template <typename T>
+void launch_copy(const CopyJob* jobs, const RowTile* tiles, int ntiles,
+                 hipStream_t s) {
+  if (ntiles <= 0) return;
+  k_copy<T><<<ntiles, 256, 0, s>>>(jobs, tiles);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal shift of a Gram matrix (shifted CholeskyQR, Fukaya et al. 2020).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_shift(const ShiftJob* __restrict__ jobs) {
+  __shared__ double red[256];
+  const ShiftJob J = jobs[blockIdx.x];
+  T* W = (T*)J.W;
+  double t = 0;
+  for (int i = threadIdx.x; i < J.r; i += 256) t += (double)W[i + (int64_t)i * J.ld];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double u = sizeof(T) == 8 ? 1.1102230246251565e-16 : 5.960464477539063e-08;
+  const double sft = 11.0 * ((double)J.m * J.r + (double)J.r * (J.r + 1)) * u * red[0];
+  for (int i = threadIdx.x; i < J.r; i += 256) W[i + (int64_t)i * J.ld] += (T)sft;
+}
+
+template <typename T>
+void launch_shift(const ShiftJob* jobs, int njobs, hipStream_t s) {
+  if (njobs <= 0) return;
+  k_shift<T><<<njobs, 256, 0, s>>>(jobs);
+}
+
+// ---------------------------------------------------------------------------
+// Multifrontal front assembly for one level.  A task covers 16 front columns
+// of one supernode: zero them, scatter the matrix entries of those columns,
+// then add every child's update matrix entries that land in them, child by
+// child (deterministic, no atomics).
+// ---------------------------------------------------------------------------
+constexpr int ASM_W = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble(
+    T* __restrict__ arena, const FrontDev* __restrict__ fronts,
+    const AsmTask* __restrict__ tasks, const int* __restrict__ children,
+    const int* __restrict__ relmap, const int* __restrict__ lowp,
+    const int64_t* __restrict__ a_dst, const int64_t* __restrict__ a_src,
+    const double* __restrict__ Ax) {
+  const AsmTask tk = tasks[blockIdx.x];
+  const FrontDev F = fronts[tk.node];
+  const int c = F.ncols, R = F.R, ld = c + R;
+  const int t0 = tk.col0, t1 = min(t0 + ASM_W, c + R);
+  T* panel = arena + F.panel;
+  T* upd = arena + F.update;
+  // 1. zero
+  for (int tb = t0; tb < t1; ++tb) {
+    if (tb < c) {
+      T* col = panel + (int64_t)tb * ld;
+      for (int i = threadIdx.x; i < ld; i += 256) col[i] = 0;
+    } else {
+      T* col = upd + (int64_t)(tb - c) * R;
+      for (int i = threadIdx.x; i < R; i += 256) col[i] = 0;
+    }
+  }
+  __syncthreads();
+  // 2. matrix entries of panel columns
+  for (int tb = t0; tb < min(t1, c); ++tb) {
+    const int k = F.first + tb;
+    for (int e = lowp[k] + threadIdx.x; e < lowp[k + 1]; e += 256)
+      panel[a_dst[e]] = (T)Ax[a_src[e]];
+  }
+  __syncthreads();
+  // 3. children, one at a time
+  for (int q = 0; q < F.nchild; ++q) {
+    const FrontDev Cf = fronts[children[F.child_off + q]];
+    const int* rm = relmap + Cf.relmap_off;
+    const int Rc = Cf.R;
+    const T* U = arena + Cf.update;
+    // child columns b with rm[b] in [t0, t1): contiguous since rm is sorted
+    int lo = 0, hi = Rc;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (rm[mid] < t0) lo = mid + 1; else hi = mid;
+    }
+    const int b0 = lo;
+    hi = Rc;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (rm[mid] < t1) lo = mid + 1; else hi = mid;
+    }
+    const int b1 = lo;
+    for (int b = b0; b < b1; ++b) {
+      const int tb = rm[b];
+      const T* ucol = U + (int64_t)b * Rc;
+      if (tb < c) {
+        T* col = panel + (int64_t)tb * ld;
+        for (int a = b + threadIdx.x; a < Rc; a += 256) col[rm[a]] += ucol[a];
+      } else {
+        T* col = upd + (int64_t)(tb - c) * R - c;
+        for (int a = b + threadIdx.x; a < Rc; a += 256) col[rm[a]] += ucol[a];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+void launch_assemble(T* arena, const FrontDev* fronts, const AsmTask* tasks,
+                     int ntasks, const int* children, const int* relmap,
+                     const int* lowp, const int64_t* a_dst, const int64_t* a_src,
+                     const double* Ax, hipStream_t s) {
+  if (ntasks <= 0) return;
+  k_assemble<T><<<ntasks, 256, 0, s>>>(arena, fronts, tasks, children, relmap,
+                                       lowp, a_dst, a_src, Ax);
+}
+
+template <typename T>
+__global__ void k_convert(const double* __restrict__ src, T* __restrict__ dst,
+                          int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (T)src[i];
+}
+
+template <typename T>
+void launch_convert(const double* src, T* dst, int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_convert<T><<<(int)blocks, 256, 0, s>>>(src, dst, count);
+}
+
+#define RSC_INST(T)                                                            \
+  template void launch_gemm<T>(bool, bool, const GemmJob*, const GemmTile*,    \
+                               int, hipStream_t);                             \
+  template void launch_potrf<T>(const PotrfJob*, int, int*, hipStream_t);      \
+  template void launch_trsm<T>(const TrsmJob*, const RowTile*, int,            \
+                               hipStream_t);                                   \
+  template void launch_copy<T>(const CopyJob*, const RowTile*, int,            \
+                               hipStream_t);                                   \
+  template void launch_shift<T>(const ShiftJob*, int, hipStream_t);            \
+  template void launch_assemble<T>(T*, const FrontDev*, const AsmTask*, int,   \
+                                   const int*, const int*, const int*,         \
+                                   const int64_t*, const int64_t*,             \
+                                   const double*, hipStream_t);               \
+  template void launch_convert<T>(const double*, T*, int64_t, hipStream_t);
+
+RSC_INST(double)
+RSC_INST(float)
+
+}  // namespace dev
+}  // namespace rsc

@@ -1,0 +1,259 @@
+// Preconditioned conjugate gradients on the device, k right-hand sides
+// advanced together (PCG_SOLVER::solvePCG, src/rschol/linearalgebra/
+// PCG_SOLVER.cpp:226-363): every column keeps its own alpha, beta, delta and
+// stop test; one SpMV with k vectors and one preconditioner apply with k
+// right-hand sides per iteration.  All vector arithmetic is FP64 and every
+// reduction is a fixed-order two-stage sum (run-to-run identical).
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "pcg.h"
+#include "solve.h"
+
+namespace rsc {
+namespace dev {
+
+constexpr int RB = 256;    // reduction block
+constexpr int RMAX = 512;  // partial sums per column
+
+struct PcgCol {
+  double delta, alpha, beta, bnorm, maxr, rr;
+  int iters, active, next_active, pad;
+};
+
+__device__ inline double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// partial[col*RMAX + b] = sum over this block's slice of x.*y
+__global__ __launch_bounds__(RB) void k_dot(int n, const double* __restrict__ x, int ldx,
+                                            const double* __restrict__ y, int ldy,
+                                            double* __restrict__ partial) {
+  __shared__ double sh[RB];
+  const int col = blockIdx.y;
+  const double* xc = x + (int64_t)col * ldx;
+  const double* yc = y + (int64_t)col * ldy;
+  double acc = 0;
+  for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RB)
+    acc += xc[i] * yc[i];
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[col * RMAX + blockIdx.x] = s;
+}
+
+__device__ double sum_partials(const double* p, int nb, double* sh) {
+  double acc = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) acc += p[b];
+  return block_sum(acc, sh);
+}
+
+// phase 0: bnorm; phase 1: delta = r.d (initial); phase 2: alpha from d.q;
+// phase 3: delta/beta/maxr from r.s and r.r
+__global__ __launch_bounds__(RB) void k_scalars(int phase, const double* __restrict__ p1,
+                                                const double* __restrict__ p2, int nb,
+                                                PcgCol* __restrict__ st, double eps,
+                                                int max_iters) {
+  __shared__ double sh[RB];
+  const int col = blockIdx.x;
+  double a = sum_partials(p1 + col * RMAX, nb, sh);
+  double b = p2 ? sum_partials(p2 + col * RMAX, nb, sh) : 0.0;
+  if (threadIdx.x != 0) return;
+  PcgCol& c = st[col];
+  if (phase == 0) {
+    c.bnorm = sqrt(a);
+  } else if (phase == 1) {
+    c.delta = a;
+    c.maxr = 2.0 * eps;
+    c.iters = 0;
+    c.active = (max_iters > 0 && c.maxr > eps) ? 1 : 0;
+    c.next_active = c.active;
+  } else if (phase == 2) {
+    if (!c.active) return;
+    c.alpha = (fabs(a) > 0.0) ? c.delta / a : a;
+  } else {
+    if (!c.active) return;
+    double old = c.delta;
+    c.delta = a;
+    c.beta = c.delta / old;
+    c.rr = b;
+    c.maxr = sqrt(b) / c.bnorm;
+    c.iters += 1;
+    c.next_active = (c.iters < max_iters && c.maxr > eps) ? 1 : 0;
+  }
+}
+
+// r = b - r (r holds A x on entry)
+__global__ void k_residual(int n, int k, const double* __restrict__ b, int ldb,
+                           double* __restrict__ r) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * k) return;
+  int i = (int)(t % n), j = (int)(t / n);
+  r[t] = b[i + (int64_t)j * ldb] - r[t];
+}
+
+// x += alpha d ; r -= alpha q
+__global__ void k_update_xr(int n, int k, const PcgCol* __restrict__ st,
+                            double* __restrict__ x, int ldx, double* __restrict__ r,
+                            const double* __restrict__ d, const double* __restrict__ q) {
+  const int col = blockIdx.y;
+  if (!st[col].active) return;
+  const double al = st[col].alpha;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t o = i + (int64_t)col * n;
+    x[i + (int64_t)col * ldx] += al * d[o];
+    r[o] -= al * q[o];
+  }
+}
+
+// d = beta d + s
+__global__ void k_update_d(int n, int k, const PcgCol* __restrict__ st,
+                           double* __restrict__ d, const double* __restrict__ s) {
+  const int col = blockIdx.y;
+  if (!st[col].active) return;
+  const double be = st[col].beta;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t o = i + (int64_t)col * n;
+    d[o] = be * d[o] + s[o];
+  }
+}
+
+__global__ void k_commit(int k, PcgCol* __restrict__ st, int* __restrict__ flags,
+                         int* __restrict__ any) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    st[c].active = st[c].next_active;
+    flags[c] = st[c].active;
+    if (st[c].active) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *any = cnt;
+}
+
+__global__ void k_flags(int k, const PcgCol* __restrict__ st, int* __restrict__ flags) {
+  for (int c = threadIdx.x; c < k; c += blockDim.x) flags[c] = st[c].active;
+}
+
+}  // namespace dev
+
+void DevCsr::upload(const Csc& A, hipStream_t s) {
+  n = A.n;
+  nnz = A.nnz();
+  p.upload(A.p, s);
+  i.upload(A.i, s);
+  x.upload(A.x, s);
+  lanes = (nnz > 16 * (int64_t)n) ? 32 : 8;
+}
+
+void DevCsr::update_values(const double* vals, hipStream_t s) {
+  RSC_HIP(hipMemcpyAsync(x.p, vals, nnz * sizeof(double), hipMemcpyHostToDevice, s));
+}
+
+void DevCsr::spmv(const double* X, int ldx, double* Y, int ldy, int k,
+                  const int* active, hipStream_t s) const {
+  dev::launch_spmv(n, p.as<int>(), i.as<int>(), x.as<double>(), X, ldx, Y, ldy, k,
+                   active, s, lanes);
+}
+
+void pcg_solve_device(FactorEngine& M, const DevCsr& A, int k, const double* B,
+                      int ldb, double* X, int ldx, double eps, int max_iters,
+                      rsc_pcg_result* res, hipStream_t s) {
+  using namespace dev;
+  auto t0 = std::chrono::steady_clock::now();
+  const int n = A.n;
+  DBuf r, d, q, sv, part1, part2, stb, flags, anyb;
+  const size_t vb = (size_t)n * k * sizeof(double);
+  r.alloc(vb);
+  d.alloc(vb);
+  q.alloc(vb);
+  sv.alloc(vb);
+  part1.alloc((size_t)k * RMAX * sizeof(double));
+  part2.alloc((size_t)k * RMAX * sizeof(double));
+  stb.alloc((size_t)k * sizeof(PcgCol));
+  flags.alloc((size_t)k * sizeof(int));
+  anyb.alloc(sizeof(int));
+  RSC_HIP(hipMemsetAsync(stb.p, 0, (size_t)k * sizeof(PcgCol), s));
+  PcgCol* st = stb.as<PcgCol>();
+  double *R = r.as<double>(), *D = d.as<double>(), *Q = q.as<double>(),
+         *S = sv.as<double>();
+  const int nb = (int)std::min<int64_t>(RMAX, ((int64_t)n + RB * 4 - 1) / (RB * 4));
+  const dim3 dgrid(nb, k);
+  const int ublocks = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
+  const dim3 ugrid(ublocks, k);
+  const unsigned eb = (unsigned)(((int64_t)n * k + 255) / 256);
+  // ||b||
+  k_dot<<<dgrid, RB, 0, s>>>(n, B, ldb, B, ldb, part1.as<double>());
+  k_scalars<<<k, RB, 0, s>>>(0, part1.as<double>(), nullptr, nb, st, eps, max_iters);
+  // r = b - A x ; d = M^{-1} r ; delta = r.d
+  A.spmv(X, ldx, R, n, k, nullptr, s);
+  k_residual<<<eb, 256, 0, s>>>(n, k, B, ldb, R);
+  M.apply(k, R, n, D, n, s);
+  k_dot<<<dgrid, RB, 0, s>>>(n, R, n, D, n, part1.as<double>());
+  k_scalars<<<k, RB, 0, s>>>(1, part1.as<double>(), nullptr, nb, st, eps, max_iters);
+  k_flags<<<1, 256, 0, s>>>(k, st, flags.as<int>());
+  int* any_host = nullptr;
+  RSC_HIP(hipHostMalloc((void**)&any_host, sizeof(int), hipHostMallocDefault));
+  *any_host = 1;
+  const int poll = 4;
+  int it = 0;
+  try {
+    while (it < max_iters) {
+      A.spmv(D, n, Q, n, k, flags.as<int>(), s);
+      k_dot<<<dgrid, RB, 0, s>>>(n, D, n, Q, n, part1.as<double>());
+      k_scalars<<<k, RB, 0, s>>>(2, part1.as<double>(), nullptr, nb, st, eps, max_iters);
+      k_update_xr<<<ugrid, 256, 0, s>>>(n, k, st, X, ldx, R, D, Q);
+      M.apply(k, R, n, S, n, s);
+      k_dot<<<dgrid, RB, 0, s>>>(n, R, n, S, n, part1.as<double>());
+      k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, part2.as<double>());
+      k_scalars<<<k, RB, 0, s>>>(3, part1.as<double>(), part2.as<double>(), nb, st, eps,
+                                 max_iters);
+      k_update_d<<<ugrid, 256, 0, s>>>(n, k, st, D, S);
+      k_commit<<<1, 256, 0, s>>>(k, st, flags.as<int>(), anyb.as<int>());
+      ++it;
+      if (it % poll == 0 || it == max_iters) {
+        RSC_HIP(hipMemcpyAsync(any_host, anyb.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        RSC_HIP(hipStreamSynchronize(s));
+        if (*any_host == 0) break;
+      }
+    }
+  } catch (...) {
+    (void)hipHostFree(any_host);
+    throw;
+  }
+  (void)hipHostFree(any_host);
+  // true residual
+  A.spmv(X, ldx, R, n, k, nullptr, s);
+  k_residual<<<eb, 256, 0, s>>>(n, k, B, ldb, R);
+  k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, part1.as<double>());
+  std::vector<PcgCol> hst(k);
+  std::vector<double> hp((size_t)k * RMAX);
+  RSC_HIP(hipMemcpyAsync(hst.data(), stb.p, k * sizeof(PcgCol), hipMemcpyDeviceToHost, s));
+  RSC_HIP(hipMemcpyAsync(hp.data(), part1.p, hp.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  RSC_HIP(hipStreamSynchronize(s));
+  RSC_HIP(hipGetLastError());
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (res)
+    for (int c = 0; c < k; ++c) {
+      double rr = 0;
+      for (int b = 0; b < nb; ++b) rr += hp[(size_t)c * RMAX + b];
+      res[c].iterations = hst[c].iters;
+      res[c].recurrence_relres = hst[c].maxr;
+      res[c].true_relres = std::sqrt(rr) / hst[c].bnorm;
+      res[c].seconds = secs;
+    }
+}
+
+}  // namespace rsc

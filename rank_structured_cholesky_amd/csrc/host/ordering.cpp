@@ -1,0 +1,588 @@
+// Host ordering: nested dissection, elimination tree, column counts,
+// relaxed supernodes and separator reordering.
+//
+// The supernode construction and the separator reordering restate the
+// reference's NestedDissection::BuildRelaxedSupernodes
+// (src/rschol/ordering/NestedDissection.cpp:1266-1667) and
+// NestedDissection::ReorderNestedSeparators (:728-910) with SeparatorTree
+// (src/rschol/datastructure/SeparatorTree.cpp:189-589).  Nested dissection
+// itself is CHOLMOD/METIS in the reference (CHOLMOD_Environment.cpp:326-432);
+// here it is this file's own bisection producing the same (perm, cParent,
+// cMember) triple.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <stdexcept>
+
+#include "rsc_host.h"
+
+namespace rsc {
+
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+
+// --------------------------------------------------------------------------
+// Nested dissection
+// --------------------------------------------------------------------------
+namespace {
+
+struct Dissector {
+  const Csc& A;
+  const double* xyz;
+  int nd_small;
+  std::vector<int> perm;
+  std::vector<int> cparent;
+  std::vector<int> cmember;
+  std::vector<int> stamp;   // stamp[v] == cur -> v in current set
+  std::vector<int> side;    // 0 = L, 1 = R side, 2 = separator
+  std::vector<int> level;   // BFS levels
+  int cur = 0;
+
+  Dissector(const Csc& A_, const double* xyz_, int nd_small_)
+      : A(A_), xyz(xyz_), nd_small(std::max(nd_small_, 1)) {
+    perm.reserve(A.n);
+    cmember.assign(A.n, -1);
+    stamp.assign(A.n, -1);
+    side.assign(A.n, 0);
+    level.assign(A.n, -1);
+  }
+
+  void mark_set(const std::vector<int>& nodes) {
+    ++cur;
+    for (int v : nodes) stamp[v] = cur;
+  }
+
+  // Pseudo-peripheral node of the subgraph induced by `nodes`
+  // (George-Liu), BFS levels left in level[] and the level list in lv.
+  int bfs(int root, std::vector<int>& order, std::vector<int>& lvstart) {
+    order.clear();
+    lvstart.clear();
+    order.push_back(root);
+    level[root] = 0;
+    size_t head = 0;
+    int curlev = -1;
+    while (head < order.size()) {
+      int v = order[head++];
+      if (level[v] != curlev) {
+        curlev = level[v];
+        lvstart.push_back((int)head - 1);
+      }
+      for (int p = A.p[v]; p < A.p[v + 1]; ++p) {
+        int u = A.i[p];
+        if (u == v || stamp[u] != cur || level[u] >= 0) continue;
+        level[u] = level[v] + 1;
+        order.push_back(u);
+      }
+    }
+    lvstart.push_back((int)order.size());
+    return curlev;
+  }
+
+  void clear_levels(const std::vector<int>& order) {
+    for (int v : order) level[v] = -1;
+  }
+
+  // Split `nodes` into L / S / R.  Returns false when no split is possible.
+  bool split(const std::vector<int>& nodes, std::vector<int>& L,
+             std::vector<int>& S, std::vector<int>& R) {
+    L.clear();
+    S.clear();
+    R.clear();
+    mark_set(nodes);
+    bool ok = false;
+    if (xyz) {
+      double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+      double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+      for (int v : nodes)
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = std::min(lo[d], xyz[3 * (int64_t)v + d]);
+          hi[d] = std::max(hi[d], xyz[3 * (int64_t)v + d]);
+        }
+      int axis = 0;
+      double best = -1.0;
+      for (int d = 0; d < 3; ++d)
+        if (hi[d] - lo[d] > best) {
+          best = hi[d] - lo[d];
+          axis = d;
+        }
+      if (best > 0.0) {
+        std::vector<double> c(nodes.size());
+        for (size_t k = 0; k < nodes.size(); ++k)
+          c[k] = xyz[3 * (int64_t)nodes[k] + axis];
+        std::vector<double> tmp(c);
+        size_t mid = tmp.size() / 2;
+        std::nth_element(tmp.begin(), tmp.begin() + mid, tmp.end());
+        double m = tmp[mid];
+        // L = coord < m, R side = coord >= m; when L would be empty use <=.
+        bool strict = false;
+        for (double v : c)
+          if (v < m) { strict = true; break; }
+        for (size_t k = 0; k < nodes.size(); ++k) {
+          bool left = strict ? (c[k] < m) : (c[k] <= m);
+          side[nodes[k]] = left ? 0 : 1;
+        }
+        ok = true;
+      }
+    }
+    if (!ok) {
+      // level-structure bisection from a pseudo-peripheral node
+      std::vector<int> order, lvstart;
+      int root = nodes[0];
+      int ecc = bfs(root, order, lvstart);
+      for (int it = 0; it < 8; ++it) {
+        int best = -1, bestdeg = 1 << 30;
+        for (int k = lvstart[lvstart.size() - 2]; k < lvstart.back(); ++k) {
+          int v = order[k];
+          int deg = A.p[v + 1] - A.p[v];
+          if (deg < bestdeg) { bestdeg = deg; best = v; }
+        }
+        clear_levels(order);
+        int e2 = bfs(best, order, lvstart);
+        if (e2 <= ecc) {
+          if (e2 < ecc) { clear_levels(order); bfs(root, order, lvstart); }
+          break;
+        }
+        ecc = e2;
+        root = best;
+      }
+      int nlev = (int)lvstart.size() - 1;
+      if (nlev < 3) {
+        clear_levels(order);
+        return false;
+      }
+      int half = (int)nodes.size() / 2, k = 1;
+      while (k < nlev - 1 && lvstart[k + 1] <= half) ++k;
+      for (int v : nodes) side[v] = 1;  // unreached -> R
+      for (int q = 0; q < lvstart[k]; ++q) side[order[q]] = 0;
+      clear_levels(order);
+    }
+    // separator: R-side vertices adjacent to the L side
+    for (int v : nodes) {
+      if (side[v] == 0) {
+        L.push_back(v);
+        continue;
+      }
+      bool adj = false;
+      for (int p = A.p[v]; p < A.p[v + 1] && !adj; ++p) {
+        int u = A.i[p];
+        if (stamp[u] == cur && side[u] == 0) adj = true;
+      }
+      if (adj) S.push_back(v);
+      else R.push_back(v);
+    }
+    if (L.empty() || (R.empty() && S.empty())) return false;
+    return true;
+  }
+
+  // Order a leaf component by recursive bisection without new components.
+  void order_leaf(std::vector<int>& nodes) {
+    if (nodes.size() <= 8) {
+      std::sort(nodes.begin(), nodes.end());
+      for (int v : nodes) perm.push_back(v);
+      return;
+    }
+    std::vector<int> L, S, R;
+    if (!split(nodes, L, S, R)) {
+      std::sort(nodes.begin(), nodes.end());
+      for (int v : nodes) perm.push_back(v);
+      return;
+    }
+    order_leaf(L);
+    order_leaf(R);
+    std::sort(S.begin(), S.end());
+    for (int v : S) perm.push_back(v);
+  }
+
+  int new_component() {
+    cparent.push_back(-1);
+    return (int)cparent.size() - 1;
+  }
+
+  std::vector<int> dissect(std::vector<int>& nodes) {
+    std::vector<int> roots;
+    if (nodes.empty()) return roots;
+    std::vector<int> L, S, R;
+    if ((int)nodes.size() <= nd_small || !split(nodes, L, S, R)) {
+      int c = new_component();
+      size_t start = perm.size();
+      order_leaf(nodes);
+      for (size_t k = start; k < perm.size(); ++k) cmember[perm[k]] = c;
+      roots.push_back(c);
+      return roots;
+    }
+    std::vector<int>().swap(nodes);
+    std::vector<int> rl = dissect(L);
+    std::vector<int> rr = dissect(R);
+    if (S.empty()) {
+      rl.insert(rl.end(), rr.begin(), rr.end());
+      return rl;
+    }
+    int c = new_component();
+    std::sort(S.begin(), S.end());
+    for (int v : S) {
+      perm.push_back(v);
+      cmember[v] = c;
+    }
+    for (int r : rl) cparent[r] = c;
+    for (int r : rr) cparent[r] = c;
+    roots.push_back(c);
+    return roots;
+  }
+};
+
+}  // namespace
+
+NDResult nested_dissection(const Csc& A, const double* xyz, int nd_small) {
+  Dissector d(A, xyz, nd_small);
+  std::vector<int> all(A.n);
+  std::iota(all.begin(), all.end(), 0);
+  d.dissect(all);
+  if ((int)d.perm.size() != A.n) throw std::runtime_error("ND lost vertices");
+  NDResult r;
+  r.perm = std::move(d.perm);
+  r.cparent = std::move(d.cparent);
+  r.cmember = std::move(d.cmember);
+  return r;
+}
+
+std::vector<Separator> separators_from_nd(const NDResult& nd, int n) {
+  int nc = (int)nd.cparent.size();
+  std::vector<Separator> seps(nc);
+  for (int c = 0; c < nc; ++c)
+    if (nd.cparent[c] >= 0) seps.at(nd.cparent[c]).nchildren += 1;
+  // components must appear contiguously and in order in perm
+  // (CHOLMOD_Environment.cpp:386-415)
+  int current = 0;
+  for (int k = 0; k < n; ++k) {
+    int comp = nd.cmember[nd.perm[k]];
+    if (comp == current + 1) current += 1;
+    else if (comp != current)
+      throw std::runtime_error("nested dissection components not contiguous");
+    seps[comp].size += 1;
+  }
+  int first = 0;
+  for (int c = 0; c < nc; ++c) {
+    seps[c].first = first;
+    seps[c].last = first + seps[c].size - 1;
+    first += seps[c].size;
+  }
+  return seps;
+}
+
+// --------------------------------------------------------------------------
+// Elimination tree and column counts (textbook algorithms: Liu's etree with
+// path compression and the Gilbert-Ng-Peyton skeleton count, as in CSparse
+// cs_etree / cs_counts which the reference calls at
+// CSparse_Interface.cpp:224-288).
+// --------------------------------------------------------------------------
+std::vector<int> elimination_tree(const Csc& A, const std::vector<int>& pinv) {
+  int n = A.n;
+  std::vector<int> perm(n);
+  for (int o = 0; o < n; ++o) perm[pinv[o]] = o;
+  std::vector<int> parent(n, -1), ancestor(n, -1);
+  for (int k = 0; k < n; ++k) {
+    int j = perm[k];
+    for (int p = A.p[j]; p < A.p[j + 1]; ++p) {
+      int i = pinv[A.i[p]];
+      while (i != -1 && i < k) {
+        int inext = ancestor[i];
+        ancestor[i] = k;
+        if (inext == -1) parent[i] = k;
+        i = inext;
+      }
+    }
+  }
+  return parent;
+}
+
+std::vector<int> column_counts(const Csc& A, const std::vector<int>& pinv,
+                               const std::vector<int>& parent,
+                               const std::vector<int>& post) {
+  int n = A.n;
+  std::vector<int> perm(n);
+  for (int o = 0; o < n; ++o) perm[pinv[o]] = o;
+  std::vector<int> delta(n), ancestor(n), maxfirst(n, -1), prevleaf(n, -1),
+      first(n, -1);
+  for (int k = 0; k < n; ++k) {
+    int j = post[k];
+    delta[j] = (first[j] == -1) ? 1 : 0;
+    for (; j != -1 && first[j] == -1; j = parent[j]) first[j] = k;
+  }
+  for (int i = 0; i < n; ++i) ancestor[i] = i;
+  for (int k = 0; k < n; ++k) {
+    int j = post[k];
+    if (parent[j] != -1) delta[parent[j]]--;
+    // column j of the transpose of triu(C): rows i > j with C(j,i) != 0
+    int oj = perm[j];
+    for (int p = A.p[oj]; p < A.p[oj + 1]; ++p) {
+      int i = pinv[A.i[p]];
+      // leaf test (cs_leaf)
+      if (i <= j || first[j] <= maxfirst[i]) continue;
+      maxfirst[i] = first[j];
+      int jprev = prevleaf[i];
+      prevleaf[i] = j;
+      int jleaf = (jprev == -1) ? 1 : 2;
+      int q;
+      if (jleaf == 1) {
+        q = i;
+      } else {
+        for (q = jprev; q != ancestor[q]; q = ancestor[q]) {
+        }
+        for (int s = jprev, sp; s != q; s = sp) {
+          sp = ancestor[s];
+          ancestor[s] = q;
+        }
+      }
+      delta[j]++;
+      if (jleaf == 2) delta[q]--;
+    }
+    if (parent[j] != -1) ancestor[j] = parent[j];
+  }
+  for (int j = 0; j < n; ++j)
+    if (parent[j] != -1) delta[parent[j]] += delta[j];
+  return delta;
+}
+
+// --------------------------------------------------------------------------
+// Relaxed supernodes: the reference's adaptation of CHOLMOD's supernodal
+// analysis (NestedDissection.cpp:1266-1667).  Compressed separators and all
+// their ancestors are single fundamental supernodes (:1356-1385); other
+// columns form fundamental supernodes (:1387-1433) which are amalgamated by
+// the CHOLMOD relaxation rule (:1476-1574), never across a compressed one.
+// --------------------------------------------------------------------------
+std::vector<SupernodeSpec> build_relaxed_supernodes(
+    const std::vector<int>& Parent, const std::vector<int>& cc,
+    const std::vector<Separator>& seps, const std::vector<int>& sep_parent,
+    int max_block_size, int nrelax0, int nrelax1, int nrelax2, double zrelax0,
+    double zrelax1, double zrelax2) {
+  const int n = (int)Parent.size();
+  const int EMPTY = -1;
+  std::vector<int> nchild(n, 0);
+  for (int j = 0; j < n; ++j)
+    if (Parent[j] >= 0) nchild[Parent[j]]++;
+
+  int ns = (int)seps.size();
+  std::vector<char> compress(ns, 0);
+  for (int s = 0; s < ns; ++s) {
+    if (seps[s].size > max_block_size && seps[s].nchildren > 0) {
+      compress[s] = 1;
+      for (int a = sep_parent[s]; a >= 0; a = sep_parent[a]) compress[a] = 1;
+    }
+  }
+
+  std::vector<int> Super;
+  std::vector<char> IsSep;
+  Super.reserve(n + 1);
+  int col = 0;
+  bool newSegment = true;
+  for (int s = 0; s < ns; ++s) {
+    const Separator& sp = seps[s];
+    if (compress[s]) {
+      Super.push_back(sp.first);
+      IsSep.push_back(1);
+      newSegment = true;
+    } else {
+      if (newSegment) {
+        Super.push_back(sp.first);
+        IsSep.push_back(0);
+        col = sp.first + 1;
+        newSegment = false;
+      }
+      for (; col <= sp.last; ++col) {
+        if (Parent[col - 1] != col || cc[col - 1] != cc[col] + 1 ||
+            nchild[col] > 1) {
+          Super.push_back(col);
+          IsSep.push_back(0);
+        }
+      }
+    }
+  }
+  int nfsuper = (int)Super.size();
+  Super.push_back(n);
+
+  std::vector<int> SuperMap(n);
+  for (int s = 0; s < nfsuper; ++s)
+    for (int k = Super[s]; k < Super[s + 1]; ++k) SuperMap[k] = s;
+  std::vector<int> Sparent(nfsuper);
+  for (int s = 0; s < nfsuper; ++s) {
+    int par = Parent[Super[s + 1] - 1];
+    Sparent[s] = (par == EMPTY) ? EMPTY : SuperMap[par];
+  }
+
+  std::vector<int> Merged(nfsuper, EMPTY), Nscol(nfsuper), Zeros(nfsuper, 0),
+      Snz(nfsuper);
+  for (int s = 0; s < nfsuper; ++s) {
+    Nscol[s] = Super[s + 1] - Super[s];
+    Snz[s] = cc[Super[s]];
+  }
+  for (int s = nfsuper - 2; s >= 0; --s) {
+    if (Sparent[s] == EMPTY) continue;
+    if (IsSep[s] || IsSep[s + 1]) continue;
+    int ss;
+    for (ss = Sparent[s]; Merged[ss] != EMPTY; ss = Merged[ss]) {
+    }
+    int sparent = ss;
+    for (ss = Sparent[s]; Merged[ss] != EMPTY;) {
+      int snext = Merged[ss];
+      Merged[ss] = sparent;
+      ss = snext;
+    }
+    if (sparent != s + 1) continue;
+    int nscol0 = Nscol[s], nscol1 = Nscol[s + 1];
+    int nsc = nscol0 + nscol1;
+    int totzeros = Zeros[s + 1];
+    bool merge;
+    if (nsc <= nrelax0) {
+      merge = true;
+    } else {
+      double lnz0 = Snz[s], lnz1 = Snz[s + 1];
+      double xnewzeros = nscol0 * (lnz1 + nscol0 - lnz0);
+      int newzeros = nscol0 * (Snz[s + 1] + nscol0 - Snz[s]);
+      if (xnewzeros == 0) {
+        merge = true;
+      } else {
+        double xtotzeros = ((double)totzeros) + xnewzeros;
+        double xns = (double)nsc;
+        double xtotsize = (xns * (xns + 1) / 2) + xns * (lnz1 - nscol1);
+        double z = xtotzeros / xtotsize;
+        totzeros += newzeros;
+        merge = ((nsc <= nrelax1 && z < zrelax0) ||
+                 (nsc <= nrelax2 && z < zrelax1) || (z < zrelax2));
+      }
+    }
+    if (merge) {
+      Zeros[s] = totzeros;
+      Merged[s + 1] = s;
+      Snz[s] = nscol0 + Snz[s + 1];
+      Nscol[s] += Nscol[s + 1];
+    }
+  }
+
+  std::vector<SupernodeSpec> out;
+  std::vector<int> starts;
+  std::vector<char> flags;
+  for (int s = 0; s < nfsuper; ++s)
+    if (Merged[s] == EMPTY) {
+      starts.push_back(Super[s]);
+      flags.push_back(IsSep[s]);
+    }
+  starts.push_back(n);
+  for (size_t k = 0; k + 1 < starts.size(); ++k) {
+    SupernodeSpec sp;
+    sp.first = starts[k];
+    sp.last = starts[k + 1] - 1;
+    sp.compress = flags[k] != 0;
+    out.push_back(sp);
+  }
+  return out;
+}
+
+// --------------------------------------------------------------------------
+// Separator tree (SeparatorTree.cpp:189-367, 470-500, 530-589): longest-axis
+// split of the bounding box, std::sort of the points along that axis, split
+// at size/2, with the "full tree" rule that keeps both subtrees at equal
+// height.
+// --------------------------------------------------------------------------
+namespace {
+
+struct SepTree {
+  const std::vector<double>& pts;  // 3 per point
+  std::vector<int> order;
+  std::vector<int> leaves;
+
+  explicit SepTree(const std::vector<double>& p) : pts(p) {}
+
+  int longest_axis(const std::vector<int>& data) const {
+    double mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    double mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int v : data)
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = std::min(mn[d], pts[3 * (size_t)v + d]);
+        mx[d] = std::max(mx[d], pts[3 * (size_t)v + d]);
+      }
+    double maxLength = -1.0;
+    int maxAxis = 0;
+    for (int d = 0; d < 3; ++d)
+      if (mx[d] - mn[d] > maxLength) {
+        maxAxis = d;
+        maxLength = mx[d] - mn[d];
+      }
+    return maxAxis;
+  }
+
+  // returns the height; appends leaves in left-to-right order
+  int build(std::vector<int>& data, int maxLeaf) {
+    if ((int)data.size() <= maxLeaf) {
+      for (int v : data) order.push_back(v);
+      leaves.push_back((int)data.size());
+      return 0;
+    }
+    int axis = longest_axis(data);
+    const std::vector<double>& P = pts;
+    std::sort(data.begin(), data.end(), [&P, axis](int a, int b) {
+      return P[3 * (size_t)a + axis] < P[3 * (size_t)b + axis];
+    });
+    int leftSize = (int)data.size() / 2;
+    std::vector<int> L(data.begin(), data.begin() + leftSize);
+    std::vector<int> R(data.begin() + leftSize, data.end());
+    if ((int)L.size() == (int)data.size() || R.empty()) {
+      for (int v : data) order.push_back(v);
+      leaves.push_back((int)data.size());
+      return 0;
+    }
+    int lh, rh;
+    if ((int)R.size() > maxLeaf && (int)L.size() <= maxLeaf)
+      lh = build(L, (int)L.size() - 1);
+    else
+      lh = build(L, maxLeaf);
+    if ((int)L.size() > maxLeaf && (int)R.size() <= maxLeaf)
+      rh = build(R, (int)R.size() - 1);
+    else
+      rh = build(R, maxLeaf);
+    return 1 + std::max(lh, rh);
+  }
+};
+
+}  // namespace
+
+void separator_tree_order(const std::vector<double>& pts3, int max_leaf,
+                          std::vector<int>& order, std::vector<int>& leaf_sizes) {
+  SepTree t(pts3);
+  std::vector<int> data(pts3.size() / 3);
+  std::iota(data.begin(), data.end(), 0);
+  t.build(data, max_leaf);
+  order = t.order;
+  leaf_sizes = t.leaves;
+}
+
+std::vector<std::vector<int>> reorder_nested_separators(
+    const std::vector<SupernodeSpec>& nodes, const double* xyz,
+    int max_block_size, std::vector<int>& perm, bool reorder) {
+  std::vector<std::vector<int>> blocks(nodes.size());
+  for (size_t s = 0; s < nodes.size(); ++s) {
+    const SupernodeSpec& nd = nodes[s];
+    int size = nd.ncols();
+    if (size <= max_block_size) continue;
+    std::vector<double> pts(3 * (size_t)size, 0.0);
+    if (xyz)
+      for (int k = 0; k < size; ++k)
+        for (int d = 0; d < 3; ++d)
+          pts[3 * (size_t)k + d] = xyz[3 * (size_t)perm[nd.first + k] + d];
+    std::vector<int> order, leaves;
+    separator_tree_order(pts, max_block_size, order, leaves);
+    if (reorder && xyz) {
+      std::vector<int> old(perm.begin() + nd.first, perm.begin() + nd.last + 1);
+      for (int k = 0; k < size; ++k) perm[nd.first + k] = old[order[k]];
+    }
+    std::vector<int>& b = blocks[s];
+    b.push_back(0);
+    for (int l : leaves) b.push_back(b.back() + l);
+  }
+  return blocks;
+}
+
+}  // namespace rsc

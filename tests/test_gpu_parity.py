@@ -1,0 +1,117 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle
+on the same ordering and the same sketch stream."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import rank_structured_cholesky_amd as R
+from rank_structured_cholesky_amd import _lib
+from helpers import oracle_for, relerr, symbolic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec,tol", [(R.RSC_FP64, 1e-13), (R.RSC_FP32, 1e-5)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (70, 33, 19), (128, 64, 200), (5, 130, 64)])
+def test_gemm_selftest(prec, tol, ta, tb, m, n, k):
+    rng = np.random.default_rng(m * 1000 + n * 10 + k)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    Cm = rng.standard_normal((m, n))
+    Af, Bf, Cf = (np.asfortranarray(x) for x in (A, B, Cm))
+    out = Cf.copy(order="F")
+    _lib.check(_lib.lib().rsc_selftest_gemm(prec, ta, tb, m, n, k, -1.5, _lib.dptr(Af),
+                                            Af.shape[0], _lib.dptr(Bf), Bf.shape[0], 0.5,
+                                            _lib.dptr(out), m))
+    opA = A.T if ta else A
+    opB = B.T if tb else B
+    ref = 0.5 * Cm - 1.5 * opA @ opB
+    assert relerr(out, ref) < tol * max(1, np.sqrt(k))
+
+
+# name: (matrix builder, symbolic options, oracle extra, apply tolerance).
+# Apply tolerance: both sides are FP64 with the same sketch stream.  Where
+# every sketch is well conditioned (3D cases) only rounding separates them.
+# In 2D the separator blocks' singular values fall below u^(1/3) of the
+# largest within the rank, so the reference method's own basis (Householder
+# QR of the cubed sketch) is rounding noise in those directions; the device
+# path resolves them (orthonormalized subspace iteration), and the two agree
+# to the reference's noise floor (~1e-6 of ||L21||).
+CASES = {
+    "p2d48_exact": (lambda: R.poisson2d(48), dict(max_dense_block=10**9), {}, 1e-12),
+    "p2d64_c31": (lambda: R.poisson2d(64), dict(max_dense_block=31), {}, 1e-4),
+    "p3d16_c63": (lambda: R.poisson3d(16, 1.0, 1234), dict(max_dense_block=63), {}, 1e-9),
+    "p3d20_hodlr": (lambda: R.poisson3d(20, 1.0, 1234),
+                    dict(max_dense_block=63, max_diagonal_block=50,
+                         explicit_block_threshold=120), dict(explicit_threshold=120), 1e-8),
+}
+
+
+def _factor(A, xyz, **opts):
+    M = R.FactorPreconditioner()
+    mdb = opts.pop("max_dense_block")
+    mdiag = opts.pop("max_diagonal_block", 400)
+    M.initialize(A, xyz, mdb, mdiag, **opts)
+    return M
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_apply_matches_oracle_fp64(name):
+    build, sopts, okw, tol = CASES[name]
+    A, xyz = build()
+    sym = symbolic(A, xyz, **sopts)
+    O = oracle_for(A, sym, **okw)
+    M = _factor(A, xyz, **dict(sopts))
+    assert np.array_equal(M.permutation(), sym["perm"])
+    b = R.uniform_rhs(A._nrow, 5678)
+    x = np.empty_like(b)
+    M.solve(x, b)
+    xo = O.apply(b)
+    assert relerr(x, xo) < tol, relerr(x, xo)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_pcg_iterations_match_oracle(name):
+    build, sopts, okw, _ = CASES[name]
+    A, xyz = build()
+    sym = symbolic(A, xyz, **sopts)
+    O = oracle_for(A, sym, **okw)
+    M = _factor(A, xyz, **dict(sopts))
+    b = R.uniform_rhs(A._nrow, 5678)
+    x = np.zeros_like(b)
+    res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M)
+    xo, ro = O.pcg(b, 1e-8, 500)
+    assert abs(res["iterations"] - ro["iterations"]) <= 1, (res, ro)
+    assert res["true_relres"] < 1e-7
+    assert relerr(A.multiply(x), b) < 1e-7
+    if "exact" in name:
+        assert res["iterations"] == 1
+
+
+def test_fp32_factor_converges():
+    A, xyz = R.poisson3d(20, 1.0, 1234)
+    M = _factor(A, xyz, max_dense_block=63, factor_precision=R.RSC_FP32)
+    M64 = _factor(A, xyz, max_dense_block=63)
+    b = R.uniform_rhs(A._nrow, 5678)
+    x = np.zeros_like(b)
+    x64 = np.zeros_like(b)
+    r32 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M)
+    r64 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x64, b, M64)
+    assert r32["true_relres"] < 1e-7
+    assert r32["iterations"] <= r64["iterations"] + 3
+
+
+def test_block_pcg_equals_single():
+    A, xyz = R.poisson3d(16, 1.0, 1234)
+    M = _factor(A, xyz, max_dense_block=63)
+    n = A._nrow
+    B = R.uniform_rhs(4 * n, 5678).reshape(4, n).T.copy()
+    X = np.zeros_like(B)
+    res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(X, B, M)
+    for j in range(4):
+        x = np.zeros(n)
+        r1 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, B[:, j].copy(), M)
+        assert res[j]["iterations"] == r1["iterations"]
+        assert np.array_equal(x, X[:, j])

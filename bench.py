@@ -1,0 +1,224 @@
+"""Benchmark: PCG solves/s to 1e-8 relative residual with the rank-structured
+preconditioner resident in HBM, on one MI355X.
+
+Workload (default): 3D 7-point variable-coefficient Poisson, 64^3 cells
+(n = 262,144), log-normal face coefficients (mu 0, sigma 1, seed 1234),
+Dirichlet boundary; separators with more than 255 unknowns compressed,
+ranks capped at 64; one uniform[-1,1] right-hand side (seed 5678).
+A step = one PCG solve (recurrence relres <= 1e-8) of that right-hand side
+with the factor and all vectors already in HBM.  The FP64 factor is the
+headline; the mixed (FP32 factor, FP64 PCG) variant is reported beside it.
+
+Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: every
+configuration here runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+CONFIGS = {
+    "poisson3d_64": dict(N=64, sigma=1.0, max_dense_block=255, max_rank=64, nrhs=1),
+    "poisson3d_128": dict(N=128, sigma=2.0, max_dense_block=255, max_rank=0, nrhs=16),
+    "poisson2d_256": dict(N2=256, max_dense_block=127, max_rank=32, nrhs=1),
+}
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="poisson3d_64", choices=sorted(CONFIGS))
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor leg")
+    return ap.parse_args()
+
+
+def build_problem(cfg):
+    import rank_structured_cholesky_amd as R
+    if "N2" in cfg:
+        A, xyz = R.poisson2d(cfg["N2"])
+    else:
+        A, xyz = R.poisson3d(cfg["N"], cfg["sigma"], 1234)
+    B = R.uniform_rhs(A._nrow * cfg["nrhs"], 5678).reshape(cfg["nrhs"], A._nrow).T
+    return A, xyz, np.asfortranarray(B)
+
+
+def factor(A, xyz, cfg, precision):
+    import rank_structured_cholesky_amd as R
+    M = R.FactorPreconditioner()
+    t0 = time.perf_counter()
+    M.initialize(A, xyz, cfg["max_dense_block"], 400, max_rank=cfg["max_rank"],
+                 factor_precision=precision)
+    return M, time.perf_counter() - t0
+
+
+def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps):
+    import torch
+
+    import rank_structured_cholesky_amd as R
+    from rank_structured_cholesky_amd import _lib
+    L = _lib.lib()
+    M, t_factor = factor(A, xyz, cfg, precision)
+    info = M.info()
+    n, k = B.shape
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")  # k x n rows = columns
+    dX = torch.zeros_like(dB)
+    res = (_lib.PcgResult * k)()
+    stream = torch.cuda.Stream()
+
+    def solve():
+        dX.zero_()
+        st = L.rsc_block_pcg_solve(M.handle, k, C.c_void_p(dB.data_ptr()), n,
+                                   C.c_void_p(dX.data_ptr()), n, eps, 10000, res,
+                                   C.c_void_p(stream.cuda_stream))
+        if st not in (_lib.RSC_OK, _lib.RSC_NOT_CONVERGED):
+            _lib.check(st)
+
+    for _ in range(warmup):
+        solve()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        solve()
+        lat.append(time.perf_counter() - t1)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    iters = [r.iterations for r in res]
+    relres = [r.true_relres for r in res]
+    # preconditioner apply timed with events on the stream it runs on
+    dY = torch.empty_like(dB)
+    napp = 10
+    with torch.cuda.stream(stream):
+        L.rsc_apply(M.handle, k, C.c_void_p(dB.data_ptr()), n, C.c_void_p(dY.data_ptr()),
+                    n, C.c_void_p(stream.cuda_stream))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(napp):
+            L.rsc_apply(M.handle, k, C.c_void_p(dB.data_ptr()), n,
+                        C.c_void_p(dY.data_ptr()), n, C.c_void_p(stream.cuda_stream))
+        e1.record(stream)
+    e1.synchronize()
+    apply_s = e0.elapsed_time(e1) / 1e3 / napp
+    x = dX.cpu().numpy().T
+    err = float(np.linalg.norm(A.multiply(x[:, 0]) - B[:, 0]) / np.linalg.norm(B[:, 0]))
+    out = dict(t_factor=t_factor, info=info, total=total, lat=lat, iters=iters,
+               relres=relres, apply_s=apply_s, check_relres=err, k=k)
+    del M
+    return out
+
+
+def cpu_leg(A, xyz, B, cfg, eps):
+    """Oracle (FP64 CPU restatement of the reference) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_for, symbolic
+    N = cfg.get("N", 0)
+    sample_N = min(N, 40) if N else 0
+    import rank_structured_cholesky_amd as R
+    if sample_N:
+        As, xs = R.poisson3d(sample_N, cfg["sigma"], 1234)
+        bs = R.uniform_rhs(As._nrow, 5678)
+        desc = f"poisson3d {sample_N}^3 (n={As._nrow}), same method/options"
+    else:
+        As, xs, bs = A, xyz, B[:, 0].copy()
+        desc = f"poisson2d n={A._nrow}"
+    t0 = time.perf_counter()
+    sym = symbolic(As, xs, max_dense_block=cfg["max_dense_block"], max_rank=cfg["max_rank"])
+    O = oracle_for(As, sym, max_rank=cfg["max_rank"])
+    tf = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    _, info = O.pcg(bs, eps, 10000)
+    ts = time.perf_counter() - t1
+    return dict(value=1.0 / ts, factor_s=tf, solve_s=ts, iterations=info["iterations"],
+                cores=1, sample=desc + "; factor %.1fs, one PCG solve %.2fs (%d it)" % (
+                    tf, ts, info["iterations"]))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "single-GPU build: run with --gpus 1 (no multi-GPU "
+                                   "path exists in this build)"}))
+        sys.exit(2)
+    cfg = CONFIGS[a.config]
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    A, xyz, B = build_problem(cfg)
+    g64 = gpu_leg(A, xyz, B, cfg, 0, a.steps, a.warmup, a.eps)
+    mixed = None
+    if not a.no_mixed:
+        mixed = gpu_leg(A, xyz, B, cfg, 1, a.steps, a.warmup, a.eps)
+    cpu = None if a.no_cpu else cpu_leg(A, xyz, B, cfg, a.eps)
+    steps = a.steps
+    ms = g64["total"] / steps * 1e3
+    value = steps * g64["k"] / g64["total"]
+    lat = sorted(g64["lat"])
+    p50 = statistics.median(lat) * 1e3
+    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))] * 1e3
+    apply_bytes = g64["info"]["apply_bytes"] * g64["k"]
+    achieved = apply_bytes / g64["apply_s"] / 1e9
+    line = {
+        "metric": "PCG solves/s to 1e-8 relres (FP64 factor; mixed FP32 factor beside)",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": a.config, "n": A._nrow, "nnz": int(A._nzmax),
+                   "max_dense_block": cfg["max_dense_block"], "max_rank": cfg["max_rank"],
+                   "nrhs": g64["k"], "eps": a.eps, "parallelism": "single GPU"},
+        "pcg_iterations": g64["iters"],
+        "true_relres": g64["relres"],
+        "p50_ms": round(p50, 3),
+        "p99_ms": round(p99, 3),
+        "factor_s": round(g64["t_factor"], 4),
+        "numeric_factor_s": round(g64["info"]["numeric_seconds"], 4),
+        "symbolic_s": round(g64["info"]["symbolic_seconds"], 4),
+        "factor_bytes": g64["info"]["factor_bytes"],
+        "roofline": {"bound": "hbm", "kernel": "preconditioner apply (fwd+bwd sweeps)",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "apply_ms": round(g64["apply_s"] * 1e3, 4),
+                     "apply_bytes": apply_bytes},
+    }
+    if mixed is not None:
+        line["mixed"] = {"solves_per_s": round(steps * mixed["k"] / mixed["total"], 3),
+                         "factor_s": round(mixed["t_factor"], 4),
+                         "numeric_factor_s": round(mixed["info"]["numeric_seconds"], 4),
+                         "pcg_iterations": mixed["iters"], "true_relres": mixed["relres"],
+                         "apply_ms": round(mixed["apply_s"] * 1e3, 4),
+                         "pivot_replacements": mixed["info"]["pivot_replacements"],
+                         "end_to_end_s": round(mixed["t_factor"] + mixed["total"] / steps, 4)}
+        line["end_to_end_s_fp64"] = round(g64["t_factor"] + g64["total"] / steps, 4)
+    if cpu is not None:
+        line["cpu_baseline"] = {"value": round(cpu["value"], 4), "unit": "solves/s",
+                                "cores": cpu["cores"], "kind": "port",
+                                "sample": cpu["sample"]}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

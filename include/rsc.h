@@ -109,6 +109,7 @@ typedef struct {
   double factor_flops;          /* algorithmic flops of the numeric factor */
   double apply_bytes;           /* algorithmic bytes of one preconditioner apply */
   double spmv_bytes;            /* algorithmic bytes of one SpMV */
+  int pivot_replacements;       /* FP32 factor: pivots raised to the floor */
 } rsc_factor_info;
 
 typedef struct rsc_factor rsc_factor;

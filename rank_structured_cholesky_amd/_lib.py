@@ -40,7 +40,7 @@ class FactorInfo(C.Structure):
         ("factor_bytes", C.c_int64), ("workspace_bytes", C.c_int64),
         ("symbolic_seconds", C.c_double), ("numeric_seconds", C.c_double),
         ("factor_flops", C.c_double), ("apply_bytes", C.c_double),
-        ("spmv_bytes", C.c_double),
+        ("spmv_bytes", C.c_double), ("pivot_replacements", C.c_int),
     ]
 
 

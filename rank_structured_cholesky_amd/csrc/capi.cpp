@@ -142,6 +142,7 @@ rsc_status factor_numeric(rsc_factor* f, const double* values) {
   f->info.numeric_seconds = std::chrono::duration<double>(clk::now() - t0).count();
   f->info.factor_bytes = f->eng->factor_bytes();
   f->info.workspace_bytes = f->eng->workspace_bytes();
+  f->info.pivot_replacements = f->eng->pivot_replacements();
   f->info.factor_flops = f->plan.factor_flops;
   const double es = f->plan.opt.factor_precision == RSC_FP32 ? 4.0 : 8.0;
   f->info.apply_bytes = f->plan.apply_bytes_per_elem * es + 24.0 * f->plan.n +

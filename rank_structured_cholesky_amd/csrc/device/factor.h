@@ -87,6 +87,7 @@ class FactorEngine {
                      hipStream_t s) = 0;                      // device pointers
   virtual int64_t factor_bytes() const = 0;
   virtual int64_t workspace_bytes() const = 0;
+  virtual int pivot_replacements() const = 0;
   virtual void release_workspace() = 0;
 };
 

@@ -84,6 +84,7 @@ class Engine final : public FactorEngine {
              hipStream_t s) override;
   int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
   int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
+  int pivot_replacements() const override { return pivot_replacements_; }
   void release_workspace() override {
     (void)hipStreamSynchronize(st_);
     work_.reset();
@@ -103,6 +104,7 @@ class Engine final : public FactorEngine {
   std::vector<std::vector<int>> levnode_off_;  // per level: offset in dlevnodes
   std::vector<int> lev_off_;
   bool program_ready_ = false;
+  int pivot_replacements_ = 0;
   // program
   std::vector<Step> prog_;
   std::vector<dev::GemmJob> gjobs_;
@@ -118,6 +120,8 @@ class Engine final : public FactorEngine {
   std::vector<int64_t> node_scratch_;   // per supernode
   std::vector<std::vector<int64_t>> hnode_scratch_;
 
+  // FP32 factors replace failed pivots instead of reporting them
+  static int factor_tol() { return sizeof(T) == 4 ? 2 : 0; }
   T* F() const { return factor_.as<T>(); }
   T* Wb() const { return work_.as<T>(); }
   char* fp(int64_t off) const { return (char*)(F() + off); }
@@ -548,7 +552,7 @@ void Engine<T>::emit_compressed(int j) {
     const HNode& h = s.hnodes[st.node];
     const int b = h.r1 - h.r0;
     if (st.kind == HStep::EXPLICIT || st.kind == HStep::LEAF) {
-      panel_chol({{(char*)(P + h.r0 + (int64_t)h.r0 * ld), b, b, ld, j, 0}});
+      panel_chol({{(char*)(P + h.r0 + (int64_t)h.r0 * ld), b, b, ld, j, factor_tol()}});
       std::vector<SketchJob> sk;
       std::vector<dev::CopyJob> cp;
       std::function<void(int)> pre = [&](int q) {
@@ -710,7 +714,7 @@ void Engine<T>::setup_static() {
   }
   lev_off_.push_back((int)levnodes.size());
   dlevnodes_.upload(levnodes, st_);
-  std::vector<int> status = {0, INT_MAX};
+  std::vector<int> status = {0, INT_MAX, 0};
   dstatus_.upload(status, st_);
   factor_.alloc(std::max<int64_t>(P.factor_elems, 1) * sizeof(T));
   layout_scratch();
@@ -779,7 +783,7 @@ void Engine<T>::build_program() {
       const Supernode& s = P.nodes[j];
       if (s.compress) continue;
       const int c = s.ncols, R = s.R(), ld = c + R;
-      mats.push_back({fp(s.off_panel), c, ld, ld, j, 0});
+      mats.push_back({fp(s.off_panel), c, ld, ld, j, factor_tol()});
       if (R > 0) {
         dev::GemmJob g;
         g.A = F() + s.off_panel + c;
@@ -818,7 +822,7 @@ void Engine<T>::numeric(const double* Ax) {
   if (!program_ready_) build_program();
   std::vector<double> ax(Ax, Ax + A_.nnz());
   dAx_.upload(ax, st_);
-  int status0[2] = {0, INT_MAX};
+  int status0[3] = {0, INT_MAX, 0};
   RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
                          hipMemcpyHostToDevice, st_));
   for (const Step& s : prog_) {
@@ -858,9 +862,10 @@ void Engine<T>::numeric(const double* Ax) {
     }
   }
   RSC_HIP(hipGetLastError());
-  int status[2];
+  int status[3];
   RSC_HIP(hipMemcpyAsync(status, dstatus_.p, sizeof(status), hipMemcpyDeviceToHost, st_));
   RSC_HIP(hipStreamSynchronize(st_));
+  pivot_replacements_ = status[2];
   if (status[0] != 0)
     throw IndefiniteError("indefinite diagonal block in supernode " +
                           std::to_string(status[1]));

@@ -24,9 +24,11 @@ struct GemmTile {
   int job, tm, tn;
 };
 
-// In-place Cholesky of an n x n (n <= TILE) lower block.  tolerant != 0
-// (Gram matrices of the thin QR): a non-positive pivot is replaced by
-// u * max(initial tile diagonal) instead of being reported.
+// In-place Cholesky of an n x n (n <= TILE) lower block.  tolerant:
+//   0  a non-positive pivot is reported (status[0] = 1, status[1] = node)
+//   1  (Gram matrices of the thin QR) it is replaced by u * max(tile diagonal)
+//   2  (FP32 factor) it is replaced by 1e-5 * max(tile diagonal) and counted
+//      in status[2]; L keeps a positive diagonal, so L L^T stays SPD
 struct PotrfJob {
   void* A;
   int n, lda, node, tolerant;

@@ -143,13 +143,19 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfJob* __restrict__ jobs
     T mx = 0;
     for (int i = 0; i < n; ++i) mx = S[i][i] > mx ? S[i][i] : mx;
     const T u = sizeof(T) == 8 ? (T)1.1102230246251565e-16 : (T)5.960464477539063e-08;
-    dfloor = mx > (T)0 ? mx * u : (T)1;
+    const T rel = J.tolerant == 2 ? (T)1e-5 : u;
+    dfloor = mx > (T)0 ? mx * rel : (T)1;
   }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     if (tid == 0) {
       T d = S[k][k];
-      if (!(d > (T)0)) {
+      if (J.tolerant == 2) {
+        if (!(d > dfloor)) {
+          d = dfloor;
+          bad += 1;
+        }
+      } else if (!(d > (T)0)) {
         if (J.tolerant) {
           d = dfloor;
         } else {
@@ -175,8 +181,12 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfJob* __restrict__ jobs
     if (i >= j) A[i + (int64_t)j * J.lda] = S[i][j];
   }
   if (tid == 0 && bad) {
-    atomicOr(&status[0], 1);
-    atomicMin(&status[1], J.node);
+    if (J.tolerant == 2) {
+      atomicAdd(&status[2], bad);
+    } else {
+      atomicOr(&status[0], 1);
+      atomicMin(&status[1], J.node);
+    }
   }
 }
 

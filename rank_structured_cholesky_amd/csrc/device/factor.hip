@@ -19,6 +19,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 
@@ -38,13 +39,17 @@ enum OpKind {
   OP_POTRF,
   OP_TRSM,
   OP_COPY,
-  OP_SHIFT
+  OP_SHIFT,
+  OP_EYE,
+  OP_CHECK,
+  OP_CONV
 };
 
 struct Step {
   int op;
   int count;
   int64_t first;
+  int f64 = 0;  // 1: run the op in FP64 on FP64 buffers (QR of an FP32 factor)
 };
 
 struct CholMat {
@@ -97,7 +102,7 @@ class Engine final : public FactorEngine {
   hipStream_t st_;
   DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
-      dctiles_, dsjobs_, dstatus_;
+      dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
   DBuf dsnodes_, dhtree_, dhops_, drows_, dlevnodes_, dperm_, dfv_, dw_;
   int64_t fv_stride_ = 0;
   int fv_k_ = 0;
@@ -115,7 +120,26 @@ class Engine final : public FactorEngine {
   std::vector<dev::CopyJob> cjobs_;
   std::vector<dev::RowTile> ctiles_;
   std::vector<dev::ShiftJob> sjobs_;
+  std::vector<dev::EyeJob> ejobs_;
+  std::vector<dev::CheckJob> kjobs_;
+  bool debug_check_ = false;
+  std::vector<std::string> step_desc_;
+  std::vector<std::string> step_ctx_;
+  std::string ctx_;
+  int cur_f64_ = 0;
+  int64_t esize() const { return cur_f64_ ? 8 : (int64_t)sizeof(T); }
+  std::vector<dev::ConvJob> vjobs_;
+  int64_t qr64_elems_ = 0;  // FP64 QR scratch (in T elements)
+  void push(const Step& st0) {
+    Step st = st0;
+    st.f64 = cur_f64_;
+    prog_.push_back(st);
+    step_ctx_.push_back(ctx_);
+  }
+  void add_checks();
+  std::vector<dev::RowTile> etiles_;
   std::vector<dev::AsmTask> tasks_;
+  int64_t inv_elems_ = 0;
   int64_t scratch_elems_ = 0;
   std::vector<int64_t> node_scratch_;   // per supernode
   std::vector<std::vector<int64_t>> hnode_scratch_;
@@ -129,6 +153,8 @@ class Engine final : public FactorEngine {
   int64_t upd_base() const { return plan_.work_elems; }
   int64_t scratch_base() const { return plan_.work_elems + plan_.update_elems; }
   int64_t rng_base() const { return scratch_base() + scratch_elems_; }
+  int64_t inv_base() const { return rng_base() + plan_.rng_count; }
+  int64_t qr64_base() const { return inv_base() + inv_elems_; }
 
   void setup_static();
   void build_program();
@@ -139,6 +165,7 @@ class Engine final : public FactorEngine {
   void emit_potrf(const std::vector<dev::PotrfJob>& jobs);
   void emit_trsm(const std::vector<dev::TrsmJob>& jobs);
   void emit_copy(const std::vector<dev::CopyJob>& jobs);
+  void invert(const std::vector<CholMat>& tris);
   void panel_chol(const std::vector<CholMat>& mats);
   void trsm_right(const std::vector<TrsmMat>& mats);
   void qr(const std::vector<struct QrMat>& jobs);
@@ -165,7 +192,7 @@ void Engine<T>::emit_gemm(int op, const std::vector<dev::GemmJob>& jobs,
       }
   }
   st.count = (int)(gtiles_.size() - st.first);
-  if (st.count > 0) prog_.push_back(st);
+  if (st.count > 0) push(st);
 }
 
 template <typename T>
@@ -173,7 +200,7 @@ void Engine<T>::emit_potrf(const std::vector<dev::PotrfJob>& jobs) {
   if (jobs.empty()) return;
   Step st{OP_POTRF, (int)jobs.size(), (int64_t)pjobs_.size()};
   pjobs_.insert(pjobs_.end(), jobs.begin(), jobs.end());
-  prog_.push_back(st);
+  push(st);
 }
 
 template <typename T>
@@ -186,7 +213,7 @@ void Engine<T>::emit_trsm(const std::vector<dev::TrsmJob>& jobs) {
     for (int r = 0; r < J.m; r += dev::TILE) ttiles_.push_back({jid, r});
   }
   st.count = (int)(ttiles_.size() - st.first);
-  if (st.count > 0) prog_.push_back(st);
+  if (st.count > 0) push(st);
 }
 
 template <typename T>
@@ -199,7 +226,37 @@ void Engine<T>::emit_copy(const std::vector<dev::CopyJob>& jobs) {
     for (int c = 0; c < J.n; c += dev::TILE) ctiles_.push_back({jid, c});
   }
   st.count = (int)(ctiles_.size() - st.first);
-  if (st.count > 0) prog_.push_back(st);
+  if (st.count > 0) push(st);
+}
+
+// Replace each n x n lower triangle L (leading dimension ld) by L^{-T}
+// (upper): X = I, X <- X L^{-T} (blocked triangular solve), copy back.  The
+// solves then apply diagonal blocks as matrix-vector products.
+template <typename T>
+void Engine<T>::invert(const std::vector<CholMat>& tris) {
+  if (tris.empty()) return;
+  std::vector<dev::EyeJob> ej;
+  std::vector<TrsmMat> tm;
+  std::vector<dev::CopyJob> cp;
+  int64_t off = inv_base();
+  for (auto& t : tris) {
+    T* X = Wb() + off;
+    off += (int64_t)t.n * t.n;
+    ej.push_back({X, t.n, t.n});
+    tm.push_back({(char*)X, t.n, t.n, t.A, t.n, t.ld});
+    cp.push_back({X, t.A, t.n, t.n, t.n, t.ld});
+  }
+  if (off - inv_base() > inv_elems_) throw std::runtime_error("inverse scratch too small");
+  Step st{OP_EYE, 0, (int64_t)etiles_.size()};
+  for (auto& J : ej) {
+    int jid = (int)ejobs_.size();
+    ejobs_.push_back(J);
+    for (int c = 0; c < J.n; c += dev::TILE) etiles_.push_back({jid, c});
+  }
+  st.count = (int)(etiles_.size() - st.first);
+  if (st.count > 0) push(st);
+  trsm_right(tm);
+  emit_copy(cp);
 }
 
 // Blocked right-looking Cholesky of the leading n columns of an m x n panel
@@ -213,20 +270,22 @@ void Engine<T>::panel_chol(const std::vector<CholMat>& mats) {
     std::vector<dev::PotrfJob> pj;
     std::vector<dev::TrsmJob> tj;
     std::vector<dev::GemmJob> gj;
+    const int64_t es = esize();
     for (auto& M : mats) {
       if (M.n <= s0) continue;
       const int nb = std::min(NB, M.n - s0);
-      T* A = (T*)M.A;
-      T* D = A + s0 + (int64_t)s0 * M.ld;
+      char* A = M.A;
+      auto at = [&](int64_t i, int64_t j) { return A + (i + j * M.ld) * es; };
+      char* D = at(s0, s0);
       pj.push_back({D, nb, M.ld, M.node, M.tolerant});
       const int below = M.m - s0 - nb;
-      if (below > 0) tj.push_back({D, D + nb, below, nb, M.ld, M.ld});
+      if (below > 0) tj.push_back({D, at(s0 + nb, s0), below, nb, M.ld, M.ld});
       const int s1 = s0 + nb;
       if (M.n > s1) {
         dev::GemmJob g;
-        g.A = A + s1 + (int64_t)s0 * M.ld;
-        g.B = A + s1 + (int64_t)s0 * M.ld;
-        g.C = A + s1 + (int64_t)s1 * M.ld;
+        g.A = at(s1, s0);
+        g.B = at(s1, s0);
+        g.C = at(s1, s1);
         g.m = M.m - s1;
         g.n = M.n - s1;
         g.k = nb;
@@ -251,19 +310,20 @@ void Engine<T>::trsm_right(const std::vector<TrsmMat>& mats) {
   for (int s0 = 0; s0 < maxn; s0 += NB) {
     std::vector<dev::TrsmJob> tj;
     std::vector<dev::GemmJob> gj;
+    const int64_t es = esize();
     for (auto& M : mats) {
       if (M.n <= s0 || M.m <= 0) continue;
       const int nb = std::min(NB, M.n - s0);
-      T* X = (T*)M.X;
-      const T* L = (const T*)M.L;
-      tj.push_back({L + s0 + (int64_t)s0 * M.ldl, X + (int64_t)s0 * M.ldx, M.m, nb,
-                    M.ldl, M.ldx});
+      char* X = M.X;
+      const char* L = M.L;
+      tj.push_back({L + (s0 + (int64_t)s0 * M.ldl) * es, X + (int64_t)s0 * M.ldx * es, M.m,
+                    nb, M.ldl, M.ldx});
       const int s1 = s0 + nb;
       if (M.n > s1) {
         dev::GemmJob g;
-        g.A = X + (int64_t)s0 * M.ldx;
-        g.B = L + s1 + (int64_t)s0 * M.ldl;
-        g.C = X + (int64_t)s1 * M.ldx;
+        g.A = X + (int64_t)s0 * M.ldx * es;
+        g.B = L + (s1 + (int64_t)s0 * M.ldl) * es;
+        g.C = X + (int64_t)s1 * M.ldx * es;
         g.m = M.m;
         g.n = M.n - s1;
         g.k = nb;
@@ -290,7 +350,30 @@ struct QrMat {
 };
 
 template <typename T>
-void Engine<T>::qr(const std::vector<QrMat>& jobs) {
+void Engine<T>::qr(const std::vector<QrMat>& jobs0) {
+  std::vector<QrMat> jobs = jobs0;
+  const bool to64 = sizeof(T) == 4;
+  if (to64) {
+    // FP32 factor: orthonormalize in FP64 (CholeskyQR's guarantees need
+    // 11 (m r + r (r+1)) u < 1, which FP32 does not give at these sizes)
+    double* base = (double*)(Wb() + qr64_base());
+    int64_t off = 0;
+    std::vector<dev::ConvJob> cv;
+    for (auto& J : jobs) {
+      double* Y64 = base + off;
+      off += (int64_t)J.m * J.r;
+      double* W64 = base + off;
+      off += (int64_t)J.r * J.r;
+      cv.push_back({J.Y, Y64, (int64_t)J.m * J.r, 1, 0});
+      J.Y = (char*)Y64;
+      J.W = (char*)W64;
+    }
+    if (2 * off > qr64_elems_) throw std::runtime_error("qr64 scratch too small");
+    Step st{OP_CONV, (int)cv.size(), (int64_t)vjobs_.size()};
+    vjobs_.insert(vjobs_.end(), cv.begin(), cv.end());
+    push(st);
+    cur_f64_ = 1;
+  }
   for (int pass = 0; pass < 3; ++pass) {
     std::vector<dev::GemmJob> gram;
     std::vector<dev::ShiftJob> sh;
@@ -317,10 +400,19 @@ void Engine<T>::qr(const std::vector<QrMat>& jobs) {
     if (pass == 0) {
       Step st{OP_SHIFT, (int)sh.size(), (int64_t)sjobs_.size()};
       sjobs_.insert(sjobs_.end(), sh.begin(), sh.end());
-      prog_.push_back(st);
+      push(st);
     }
     panel_chol(ch);
     trsm_right(tr);
+  }
+  if (to64) {
+    cur_f64_ = 0;
+    std::vector<dev::ConvJob> cv;
+    for (size_t q = 0; q < jobs.size(); ++q)
+      cv.push_back({jobs[q].Y, jobs0[q].Y, (int64_t)jobs[q].m * jobs[q].r, 0, 0});
+    Step st{OP_CONV, (int)cv.size(), (int64_t)vjobs_.size()};
+    vjobs_.insert(vjobs_.end(), cv.begin(), cv.end());
+    push(st);
   }
 }
 
@@ -498,6 +590,21 @@ void Engine<T>::layout_scratch() {
   int64_t off = 0;
   node_scratch_.assign(plan_.nodes.size(), -1);
   hnode_scratch_.assign(plan_.nodes.size(), {});
+  inv_elems_ = 0;
+  for (auto& lv : plan_.levels) {
+    int64_t unc = 0;
+    for (int j : lv)
+      if (!plan_.nodes[j].compress) unc += (int64_t)plan_.nodes[j].ncols * plan_.nodes[j].ncols;
+    inv_elems_ = std::max(inv_elems_, unc);
+  }
+  for (size_t j = 0; j < plan_.nodes.size(); ++j) {
+    const Supernode& s = plan_.nodes[j];
+    if (!s.compress) continue;
+    int64_t lv = 0;
+    for (auto& h : s.hnodes)
+      if (h.leaf()) lv += (int64_t)(h.r1 - h.r0) * (h.r1 - h.r0);
+    inv_elems_ = std::max(inv_elems_, lv);
+  }
   for (size_t j = 0; j < plan_.nodes.size(); ++j) {
     const Supernode& s = plan_.nodes[j];
     if (!s.compress) continue;
@@ -516,6 +623,33 @@ void Engine<T>::layout_scratch() {
     }
   }
   scratch_elems_ = off;
+  // FP64 QR scratch (FP32 factors): largest concurrent batch of QRs, each
+  // max(rows, cols) x r + r x r doubles, counted in T elements
+  qr64_elems_ = 0;
+  if (sizeof(T) == 4) {
+    int64_t mx = 0;
+    for (const Supernode& s : plan_.nodes) {
+      if (!s.compress) continue;
+      for (const HStep& st : s.hsteps) {
+        int64_t sum = 0;
+        std::function<void(int)> acc = [&](int q) {
+          const HNode& h = s.hnodes[q];
+          if (h.leaf()) return;
+          int64_t rows = s.hnodes[h.right].r1 - s.hnodes[h.right].r0;
+          int64_t cols = s.hnodes[h.left].r1 - s.hnodes[h.left].r0;
+          sum += std::max(rows, cols) * h.rank + (int64_t)h.rank * h.rank;
+          if (st.kind == HStep::EXPLICIT) {
+            acc(h.left);
+            acc(h.right);
+          }
+        };
+        if (st.kind != HStep::LEAF) acc(st.node);
+        mx = std::max(mx, sum);
+      }
+      mx = std::max(mx, (int64_t)std::max(s.R(), s.ncols) * s.rank + (int64_t)s.rank * s.rank);
+    }
+    qr64_elems_ = 2 * mx;
+  }
 }
 
 template <typename T>
@@ -551,6 +685,8 @@ void Engine<T>::emit_compressed(int j) {
   for (const HStep& st : s.hsteps) {
     const HNode& h = s.hnodes[st.node];
     const int b = h.r1 - h.r0;
+    ctx_ = "node " + std::to_string(j) + " diag step kind " + std::to_string((int)st.kind) +
+           " tree node " + std::to_string(st.node);
     if (st.kind == HStep::EXPLICIT || st.kind == HStep::LEAF) {
       panel_chol({{(char*)(P + h.r0 + (int64_t)h.r0 * ld), b, b, ld, j, factor_tol()}});
       std::vector<SketchJob> sk;
@@ -587,7 +723,9 @@ void Engine<T>::emit_compressed(int j) {
     for (int q = 0; q < (int)s.hnodes.size(); ++q)
       if (s.hnodes[q].parent < 0) root = q;
     char* X = (char*)(P + c);
+    ctx_ = "node " + std::to_string(j) + " L21 solve";
     hsolve_right(s, root, X, R, ld, Tm);
+    ctx_ = "node " + std::to_string(j) + " off-diagonal sketch";
     SketchJob J;
     J.X = X;
     J.rows = R;
@@ -604,6 +742,78 @@ void Engine<T>::emit_compressed(int j) {
     char* K = wp(nsb + (int64_t)R * s.rank + (int64_t)s.rank * s.rank);
     lowrank_update(wp(upd_base() + s.off_update), R, R, J.V, J.U, c, s.rank, K, Tm);
   }
+  ctx_ = "node " + std::to_string(j) + " invert leaves";
+  std::vector<CholMat> leaves;
+  for (auto& h : s.hnodes)
+    if (h.leaf()) leaves.push_back({fp(h.off_leaf), h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, j, 0});
+  invert(leaves);
+}
+
+// Debug: after every program step, scan that step's outputs for non-finite
+// values; the first failing step index is reported with its description.
+template <typename T>
+void Engine<T>::add_checks() {
+  std::vector<Step> out;
+  step_desc_.clear();
+  const char* names[] = {"assemble", "gemm_nn", "gemm_tn", "gemm_nt", "gemm_tt",
+                         "potrf", "trsm", "copy", "shift", "eye", "check", "conv"};
+  for (size_t si = 0; si < prog_.size(); ++si) {
+    const Step& st = prog_[si];
+    out.push_back(st);
+    const int tag = (int)step_desc_.size();
+    std::vector<dev::CheckJob> jobs;
+    std::string desc = names[st.op];
+    if (st.op >= OP_GEMM_NN && st.op <= OP_GEMM_TT) {
+      int last = -1;
+      for (int t = 0; t < st.count; ++t) {
+        const dev::GemmTile& tl = gtiles_[st.first + t];
+        if (tl.job == last) continue;
+        last = tl.job;
+        const dev::GemmJob& J = gjobs_[tl.job];
+        jobs.push_back({J.C, J.m, J.n, J.ldc, tag});
+      }
+    } else if (st.op == OP_POTRF) {
+      for (int t = 0; t < st.count; ++t) {
+        const dev::PotrfJob& J = pjobs_[st.first + t];
+        jobs.push_back({J.A, J.n, J.n, J.lda, tag});
+        if (t == 0) desc += " node " + std::to_string(J.node);
+      }
+    } else if (st.op == OP_TRSM) {
+      int last = -1;
+      for (int t = 0; t < st.count; ++t) {
+        const dev::RowTile& tl = ttiles_[st.first + t];
+        if (tl.job == last) continue;
+        last = tl.job;
+        const dev::TrsmJob& J = tjobs_[tl.job];
+        jobs.push_back({J.B, J.m, J.nb, J.ldb, tag});
+      }
+    } else if (st.op == OP_COPY) {
+      int last = -1;
+      for (int t = 0; t < st.count; ++t) {
+        const dev::RowTile& tl = ctiles_[st.first + t];
+        if (tl.job == last) continue;
+        last = tl.job;
+        const dev::CopyJob& J = cjobs_[tl.job];
+        jobs.push_back({J.dst, J.m, J.n, J.ldd, tag});
+      }
+    } else if (st.op == OP_SHIFT) {
+      for (int t = 0; t < st.count; ++t) {
+        const dev::ShiftJob& J = sjobs_[st.first + t];
+        jobs.push_back({J.W, J.r, J.r, J.ld, tag});
+      }
+    }
+    desc += " (step " + std::to_string(si) + ", " + std::to_string(jobs.size()) +
+            " jobs) in " + step_ctx_[si];
+    if (!jobs.empty())
+      desc += " first job " + std::to_string(jobs[0].rows) + "x" +
+              std::to_string(jobs[0].cols) + " ld " + std::to_string(jobs[0].ld);
+    step_desc_.push_back(desc);
+    if (jobs.empty()) continue;
+    Step ck{OP_CHECK, (int)jobs.size(), (int64_t)kjobs_.size(), st.f64};
+    kjobs_.insert(kjobs_.end(), jobs.begin(), jobs.end());
+    out.push_back(ck);
+  }
+  prog_.swap(out);
 }
 
 template <typename T>
@@ -670,7 +880,7 @@ void Engine<T>::setup_static() {
     d.relmap_off = relmap_off[j];
     d.rows_off = rows_off[j];
     d.fv_off = fvo;
-    fvo += s.ncols + s.R();
+    fvo += 2 * (int64_t)s.ncols + s.R();
     d.hops_off = (int)hops.size();
     d.nhops = 0;
     if (s.compress) {
@@ -714,7 +924,9 @@ void Engine<T>::setup_static() {
   }
   lev_off_.push_back((int)levnodes.size());
   dlevnodes_.upload(levnodes, st_);
-  std::vector<int> status = {0, INT_MAX, 0};
+  std::vector<int> status = {0, INT_MAX, 0, INT_MAX};
+  const char* dbg = std::getenv("RSC_DEBUG_CHECK");
+  debug_check_ = dbg && dbg[0] == '1';
   dstatus_.upload(status, st_);
   factor_.alloc(std::max<int64_t>(P.factor_elems, 1) * sizeof(T));
   layout_scratch();
@@ -726,6 +938,7 @@ void Engine<T>::build_program() {
   const Plan& P = plan_;
   const int ns = (int)P.nodes.size();
   prog_.clear();
+  step_ctx_.clear();
   gjobs_.clear();
   gtiles_.clear();
   pjobs_.clear();
@@ -734,8 +947,12 @@ void Engine<T>::build_program() {
   cjobs_.clear();
   ctiles_.clear();
   sjobs_.clear();
+  ejobs_.clear();
+  etiles_.clear();
+  kjobs_.clear();
+  vjobs_.clear();
   tasks_.clear();
-  const int64_t welems = rng_base() + P.rng_count;
+  const int64_t welems = qr64_base() + qr64_elems_;
   work_.alloc(std::max<int64_t>(welems, 1) * sizeof(T));
   // Gaussian sketches: the host MT19937 stream, converted on the device
   if (P.rng_count > 0) {
@@ -770,13 +987,14 @@ void Engine<T>::build_program() {
   dfronts_.upload(fr, st_);
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
     const std::vector<int>& nodes = P.levels[lv];
+    ctx_ = "level " + std::to_string(lv) + " uncompressed";
     Step st{OP_ASM, 0, (int64_t)tasks_.size()};
     for (int j : nodes) {
       const int w = P.nodes[j].ncols + P.nodes[j].R();
       for (int c0 = 0; c0 < w; c0 += 16) tasks_.push_back({j, c0});
     }
     st.count = (int)(tasks_.size() - st.first);
-    if (st.count > 0) prog_.push_back(st);
+    if (st.count > 0) push(st);
     std::vector<CholMat> mats;
     std::vector<dev::GemmJob> syrk;
     for (int j : nodes) {
@@ -801,9 +1019,12 @@ void Engine<T>::build_program() {
     }
     panel_chol(mats);
     emit_gemm(OP_GEMM_NT, syrk, true);
+    invert(mats);
     for (int j : nodes)
       if (P.nodes[j].compress) emit_compressed(j);
+
   }
+  if (debug_check_) add_checks();
   dgjobs_.upload(gjobs_, st_);
   dgtiles_.upload(gtiles_, st_);
   dpjobs_.upload(pjobs_, st_);
@@ -812,6 +1033,10 @@ void Engine<T>::build_program() {
   dcjobs_.upload(cjobs_, st_);
   dctiles_.upload(ctiles_, st_);
   dsjobs_.upload(sjobs_, st_);
+  dejobs_.upload(ejobs_, st_);
+  dkjobs_.upload(kjobs_, st_);
+  dvjobs_.upload(vjobs_, st_);
+  detiles_.upload(etiles_, st_);
   dtasks_.upload(tasks_, st_);
   RSC_HIP(hipStreamSynchronize(st_));
   program_ready_ = true;
@@ -822,7 +1047,7 @@ void Engine<T>::numeric(const double* Ax) {
   if (!program_ready_) build_program();
   std::vector<double> ax(Ax, Ax + A_.nnz());
   dAx_.upload(ax, st_);
-  int status0[3] = {0, INT_MAX, 0};
+  int status0[4] = {0, INT_MAX, 0, INT_MAX};
   RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
                          hipMemcpyHostToDevice, st_));
   for (const Step& s : prog_) {
@@ -840,32 +1065,67 @@ void Engine<T>::numeric(const double* Ax) {
       case OP_GEMM_TT: {
         bool ta = s.op == OP_GEMM_TN || s.op == OP_GEMM_TT;
         bool tb = s.op == OP_GEMM_NT || s.op == OP_GEMM_TT;
-        dev::launch_gemm<T>(ta, tb, dgjobs_.as<dev::GemmJob>(),
-                            dgtiles_.as<dev::GemmTile>() + s.first, s.count, st_);
+        if (s.f64)
+          dev::launch_gemm<double>(ta, tb, dgjobs_.as<dev::GemmJob>(),
+                                   dgtiles_.as<dev::GemmTile>() + s.first, s.count, st_);
+        else
+          dev::launch_gemm<T>(ta, tb, dgjobs_.as<dev::GemmJob>(),
+                              dgtiles_.as<dev::GemmTile>() + s.first, s.count, st_);
         break;
       }
       case OP_POTRF:
-        dev::launch_potrf<T>(dpjobs_.as<dev::PotrfJob>() + s.first, s.count,
-                             dstatus_.as<int>(), st_);
+        if (s.f64)
+          dev::launch_potrf<double>(dpjobs_.as<dev::PotrfJob>() + s.first, s.count,
+                                    dstatus_.as<int>(), st_);
+        else
+          dev::launch_potrf<T>(dpjobs_.as<dev::PotrfJob>() + s.first, s.count,
+                               dstatus_.as<int>(), st_);
         break;
       case OP_TRSM:
-        dev::launch_trsm<T>(dtjobs_.as<dev::TrsmJob>(),
-                            dttiles_.as<dev::RowTile>() + s.first, s.count, st_);
+        if (s.f64)
+          dev::launch_trsm<double>(dtjobs_.as<dev::TrsmJob>(),
+                                   dttiles_.as<dev::RowTile>() + s.first, s.count, st_);
+        else
+          dev::launch_trsm<T>(dtjobs_.as<dev::TrsmJob>(),
+                              dttiles_.as<dev::RowTile>() + s.first, s.count, st_);
+        break;
+      case OP_CONV:
+        dev::launch_convjobs(dvjobs_.as<dev::ConvJob>() + s.first, s.count, st_);
         break;
       case OP_COPY:
         dev::launch_copy<T>(dcjobs_.as<dev::CopyJob>(),
                             dctiles_.as<dev::RowTile>() + s.first, s.count, st_);
         break;
       case OP_SHIFT:
-        dev::launch_shift<T>(dsjobs_.as<dev::ShiftJob>() + s.first, s.count, st_);
+        if (s.f64)
+          dev::launch_shift<double>(dsjobs_.as<dev::ShiftJob>() + s.first, s.count, st_);
+        else
+          dev::launch_shift<T>(dsjobs_.as<dev::ShiftJob>() + s.first, s.count, st_);
+        break;
+      case OP_CHECK:
+        if (s.f64)
+          dev::launch_check<double>(dkjobs_.as<dev::CheckJob>() + s.first, s.count,
+                                    dstatus_.as<int>(), st_);
+        else
+          dev::launch_check<T>(dkjobs_.as<dev::CheckJob>() + s.first, s.count,
+                               dstatus_.as<int>(), st_);
+        break;
+      case OP_EYE:
+        dev::launch_eye<T>(dejobs_.as<dev::EyeJob>(),
+                           detiles_.as<dev::RowTile>() + s.first, s.count, st_);
         break;
     }
   }
   RSC_HIP(hipGetLastError());
-  int status[3];
+  int status[4];
   RSC_HIP(hipMemcpyAsync(status, dstatus_.p, sizeof(status), hipMemcpyDeviceToHost, st_));
   RSC_HIP(hipStreamSynchronize(st_));
   pivot_replacements_ = status[2];
+  if (status[3] != INT_MAX)
+    throw DeviceError("non-finite factor values first after program step " +
+                      std::to_string(status[3]) + ": " +
+                      (status[3] < (int)step_desc_.size() ? step_desc_[status[3]]
+                                                          : std::string("?")));
   if (status[0] != 0)
     throw IndefiniteError("indefinite diagonal block in supernode " +
                           std::to_string(status[1]));

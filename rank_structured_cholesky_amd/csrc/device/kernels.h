@@ -51,6 +51,28 @@ struct CopyJob {
   int m, n, lds, ldd;
 };
 
+// A(n x n, lda) = I
+struct EyeJob {
+  void* A;
+  int n, lda;
+};
+
+// dst = src converted between FP32 and FP64 (to64 != 0: float -> double)
+struct ConvJob {
+  const void* src;
+  void* dst;
+  int64_t count;
+  int to64, pad;
+};
+void launch_convjobs(const ConvJob* jobs, int njobs, hipStream_t s);
+
+// Debug scan: status[3] = min(tag) over jobs whose rows x cols block
+// (leading dimension ld) holds a non-finite value.
+struct CheckJob {
+  const void* p;
+  int rows, cols, ld, tag;
+};
+
 // W(r x r) += s*I with s = 11 (m r + r (r+1)) u trace(W)
 struct ShiftJob {
   void* W;
@@ -81,6 +103,10 @@ void launch_trsm(const TrsmJob* jobs, const RowTile* tiles, int ntiles,
 template <typename T>
 void launch_copy(const CopyJob* jobs, const RowTile* tiles, int ntiles,
                  hipStream_t s);
+template <typename T>
+void launch_eye(const EyeJob* jobs, const RowTile* tiles, int ntiles, hipStream_t s);
+template <typename T>
+void launch_check(const CheckJob* jobs, int njobs, int* status, hipStream_t s);
 template <typename T>
 void launch_shift(const ShiftJob* jobs, int njobs, hipStream_t s);
 // Assemble fronts: zero, scatter A (values converted to T), extend-add the

@@ -265,6 +265,48 @@ void launch_copy(const CopyJob* jobs, const RowTile* tiles, int ntiles,
 }
 
 // ---------------------------------------------------------------------------
+// Identity fill; tiles of 64 columns.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_eye(const EyeJob* __restrict__ jobs,
+                                             const RowTile* __restrict__ tiles) {
+  const RowTile tl = tiles[blockIdx.x];
+  const EyeJob J = jobs[tl.job];
+  T* A = (T*)J.A;
+  const int c1 = min(J.n, tl.row0 + TILE);
+  for (int j = tl.row0; j < c1; ++j)
+    for (int i = threadIdx.x; i < J.n; i += 256)
+      A[i + (int64_t)j * J.lda] = (i == j) ? (T)1 : (T)0;
+}
+
+template <typename T>
+void launch_eye(const EyeJob* jobs, const RowTile* tiles, int ntiles, hipStream_t s) {
+  if (ntiles <= 0) return;
+  k_eye<T><<<ntiles, 256, 0, s>>>(jobs, tiles);
+}
+
+// ---------------------------------------------------------------------------
+// Debug scan for non-finite values (one workgroup per job).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_check(const CheckJob* __restrict__ jobs,
+                                               int* __restrict__ status) {
+  const CheckJob J = jobs[blockIdx.x];
+  const T* p = (const T*)J.p;
+  int bad = 0;
+  for (int j = 0; j < J.cols; ++j)
+    for (int i = threadIdx.x; i < J.rows; i += 256)
+      if (!isfinite((double)p[i + (int64_t)j * J.ld])) bad = 1;
+  if (bad) atomicMin(&status[3], J.tag);
+}
+
+template <typename T>
+void launch_check(const CheckJob* jobs, int njobs, int* status, hipStream_t s) {
+  if (njobs <= 0) return;
+  k_check<T><<<njobs, 256, 0, s>>>(jobs, status);
+}
+
+// ---------------------------------------------------------------------------
 // Diagonal shift of a Gram matrix (shifted CholeskyQR, Fukaya et al. 2020).
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -374,6 +416,20 @@ void launch_assemble(T* arena, const FrontDev* fronts, const AsmTask* tasks,
                                        lowp, a_dst, a_src, Ax);
 }
 
+__global__ __launch_bounds__(256) void k_convjobs(const ConvJob* __restrict__ jobs) {
+  const ConvJob J = jobs[blockIdx.y];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < J.count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (J.to64) ((double*)J.dst)[i] = (double)((const float*)J.src)[i];
+    else ((float*)J.dst)[i] = (float)((const double*)J.src)[i];
+  }
+}
+
+void launch_convjobs(const ConvJob* jobs, int njobs, hipStream_t s) {
+  if (njobs <= 0) return;
+  k_convjobs<<<dim3(64, njobs), 256, 0, s>>>(jobs);
+}
+
 template <typename T>
 __global__ void k_convert(const double* __restrict__ src, T* __restrict__ dst,
                           int64_t count) {
@@ -399,6 +455,8 @@ void launch_convert(const double* src, T* dst, int64_t count, hipStream_t s) {
   template void launch_copy<T>(const CopyJob*, const RowTile*, int,            \
                                hipStream_t);                                   \
   template void launch_shift<T>(const ShiftJob*, int, hipStream_t);            \
+  template void launch_eye<T>(const EyeJob*, const RowTile*, int, hipStream_t); \
+  template void launch_check<T>(const CheckJob*, int, int*, hipStream_t);       \
   template void launch_assemble<T>(T*, const FrontDev*, const AsmTask*, int,   \
                                    const int*, const int*, const int*,         \
                                    const int64_t*, const int64_t*,             \

@@ -6,7 +6,8 @@
 // 1306-1378) and the CSR SpMV of the PCG operator.
 //
 // Level-scheduled: one launch per elimination-tree level and direction, one
-// workgroup per supernode.  The forward sweep is multifrontal: a supernode's
+// workgroup per supernode; diagonal blocks are applied through their stored
+// inverses (L^{-T}), so every step is a matrix-vector product.  The forward sweep is multifrontal: a supernode's
 // front vector receives its children's update vectors (no atomics), solves
 // with its diagonal block and leaves its own update vector for the parent.
 // Vectors are FP64; the factor may be FP32 (widened on load).
@@ -46,76 +47,91 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
 }
 
 // ---- workgroup helpers ----------------------------------------------------
+// Dense diagonal triangles are stored inverted and transposed (Mt = L^{-T},
+// upper triangular, leading dimension ld), so both triangular solves are
+// matrix-vector products without per-column barriers.
 
-// f[0:b] <- L^{-1} f[0:b]; rows b..b+extra of the same columns are updated
-// too (L stored b+extra rows tall with leading dimension ld).
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Column-reduction GEMV: out[q] = sum_{i<m} W[i + q*ld] * src[i] for q < r.
+// Each wave owns 8 columns at a time (8 independent load streams per lane),
+// lanes stride the contiguous rows, then 8 wavefront reductions.
 template <typename T>
-__device__ void wg_lower_solve(const T* __restrict__ L, int ld, int b, int extra,
-                               double* f) {
-  const int tid = threadIdx.x;
-  for (int kk = 0; kk < b; ++kk) {
-    if (tid == 0) f[kk] = f[kk] / (double)L[kk + (int64_t)kk * ld];
-    __syncthreads();
-    const double y = f[kk];
-    const T* col = L + (int64_t)kk * ld;
-    for (int i = kk + 1 + tid; i < b + extra; i += 256) f[i] -= (double)col[i] * y;
-    __syncthreads();
+__device__ void wg_colreduce(const T* __restrict__ W, int64_t ld, int m, int r,
+                             const double* src, double* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q0 = wave * 8; q0 < r; q0 += 32) {
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0;
+    const int nq = min(8, r - q0);
+    for (int i = lane; i < m; i += 64) {
+      const double xi = src[i];
+      const T* base = W + i + (int64_t)q0 * ld;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < nq) acc[u] += (double)base[(int64_t)u * ld] * xi;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      double v = wave_sum(acc[u]);
+      if (lane == 0 && u < nq) out[q0 + u] = v;
+    }
   }
 }
 
-// z[0:b] <- L^{-T} z[0:b], L b x b lower (leading dimension ld), row form.
-template <typename T>
-__device__ void wg_upper_solve(const T* __restrict__ L, int ld, int b, double* z) {
-  const int tid = threadIdx.x;
-  for (int kk = b - 1; kk >= 0; --kk) {
-    if (tid == 0) z[kk] = z[kk] / (double)L[kk + (int64_t)kk * ld];
-    __syncthreads();
-    const double x = z[kk];
-    for (int i = tid; i < kk; i += 256) z[i] -= (double)L[kk + (int64_t)i * ld] * x;
-    __syncthreads();
+// Row-form GEMV: dst[i] (+|-)= sum_{q<r} W[i + q*ld] * t[q] for i < m, one
+// thread per row (coalesced), inner loop unrolled for loads in flight.
+template <typename T, bool SUB>
+__device__ void wg_rowgemv(const T* __restrict__ W, int64_t ld, int m, int r,
+                           const double* t, double* dst) {
+  for (int i = threadIdx.x; i < m; i += 256) {
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    const T* row = W + i;
+    int q = 0;
+    for (; q + 4 <= r; q += 4) {
+      a0 += (double)row[(int64_t)q * ld] * t[q];
+      a1 += (double)row[(int64_t)(q + 1) * ld] * t[q + 1];
+      a2 += (double)row[(int64_t)(q + 2) * ld] * t[q + 2];
+      a3 += (double)row[(int64_t)(q + 3) * ld] * t[q + 3];
+    }
+    for (; q < r; ++q) a0 += (double)row[(int64_t)q * ld] * t[q];
+    const double acc = (a0 + a1) + (a2 + a3);
+    if (SUB) dst[i] -= acc;
+    else dst[i] = acc;
   }
 }
 
-// dst[0:m] -= V (m x r) * (U^T src)  with U (c x r): one low-rank pair.
+// out = L^{-1} f with Mt = L^{-T}: out[i] = sum_k Mt[k,i] f[k]
 template <typename T>
-__device__ void wg_lowrank(const T* __restrict__ U, int c, const T* __restrict__ V,
-                           int m, int r, const double* src, double* dst,
-                           double* tsh) {
-  const int tid = threadIdx.x;
-  for (int q = tid; q < r; q += 256) {
-    const T* u = U + (int64_t)q * c;
-    double acc = 0;
-    for (int i = 0; i < c; ++i) acc += (double)u[i] * src[i];
-    tsh[q] = acc;
-  }
-  __syncthreads();
-  for (int i = tid; i < m; i += 256) {
-    double acc = 0;
-    for (int q = 0; q < r; ++q) acc += (double)V[i + (int64_t)q * m] * tsh[q];
-    dst[i] -= acc;
-  }
-  __syncthreads();
+__device__ void wg_apply_linv(const T* __restrict__ Mt, int ld, int b, const double* f,
+                              double* out) {
+  wg_colreduce<T>(Mt, ld, b, b, f, out);
 }
 
-// dst[0:c] -= U (c x r) * (V^T src), V (m x r): transposed low-rank pair.
+// out = L^{-T} z with Mt = L^{-T}: out[i] = sum_k Mt[i,k] z[k] (the strictly
+// lower part of Mt holds exact zeros)
 template <typename T>
-__device__ void wg_lowrank_t(const T* __restrict__ U, int c, const T* __restrict__ V,
-                             int m, int r, const double* src, double* dst,
-                             double* tsh) {
-  const int tid = threadIdx.x;
-  for (int q = tid; q < r; q += 256) {
-    const T* v = V + (int64_t)q * m;
-    double acc = 0;
-    for (int i = 0; i < m; ++i) acc += (double)v[i] * src[i];
-    tsh[q] = acc;
-  }
-  __syncthreads();
-  for (int i = tid; i < c; i += 256) {
-    double acc = 0;
-    for (int q = 0; q < r; ++q) acc += (double)U[i + (int64_t)q * c] * tsh[q];
-    dst[i] -= acc;
-  }
-  __syncthreads();
+__device__ void wg_apply_ltinv(const T* __restrict__ Mt, int ld, int b, const double* z,
+                               double* out) {
+  wg_rowgemv<T, false>(Mt, ld, b, b, z, out);
+}
+
+// t[q] = sum_i W[i,q] src[i] for q < r (W m x r)
+template <typename T>
+__device__ void wg_tmatvec(const T* __restrict__ W, int m, int r, const double* src,
+                           double* t) {
+  wg_colreduce<T>(W, m, m, r, src, t);
+}
+
+// dst[i] -= sum_q W[i,q] t[q] for i < m (W m x r, ld ldw)
+template <typename T>
+__device__ void wg_matvec_sub(const T* __restrict__ W, int m, int r, int64_t ldw,
+                              const double* t, double* dst) {
+  wg_rowgemv<T, true>(W, ldw, m, r, t, dst);
 }
 
 // ---- forward sweep ---------------------------------------------------------
@@ -133,6 +149,7 @@ __global__ __launch_bounds__(256) void k_forward(
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
   for (int col = 0; col < k; ++col) {
     double* f = fv + S.fv_off + (int64_t)col * fvs;
+    double* tmp = f + c + R;
     double* wc = w + (int64_t)col * n;
     for (int i = tid; i < c; i += 256) f[i] = wc[S.first + i];
     for (int i = tid; i < R; i += 256) f[c + i] = 0.0;
@@ -145,23 +162,39 @@ __global__ __launch_bounds__(256) void k_forward(
       __syncthreads();
     }
     if (!S.compress) {
-      wg_lower_solve<T>(F + S.panel, c + R, c, R, f);
+      const T* P = F + S.panel;
+      const int ld = c + R;
+      wg_apply_linv<T>(P, ld, c, f, tmp);
+      __syncthreads();
+      // f_R -= L21 y
+      wg_rowgemv<T, true>(P + c, ld, R, c, tmp, f + c);
+      for (int i = tid; i < c; i += 256) wc[S.first + i] = tmp[i];
+      __syncthreads();
     } else {
       const SolveHNode* H = htree + S.htree;
       for (int o = 0; o < S.nhops; ++o) {
         const SolveHNode h = H[hops[S.hops_off + o]];
         if (h.left < 0) {
-          wg_lower_solve<T>(F + h.leaf, h.r1 - h.r0, h.r1 - h.r0, 0, f + h.r0);
+          const int b = h.r1 - h.r0;
+          wg_apply_linv<T>(F + h.leaf, b, b, f + h.r0, tmp);
+          __syncthreads();
+          for (int i = tid; i < b; i += 256) f[h.r0 + i] = tmp[i];
         } else {
           const SolveHNode L = H[h.left], Rt = H[h.right];
-          wg_lowrank<T>(F + h.U, L.r1 - L.r0, F + h.V, Rt.r1 - Rt.r0, h.rank,
-                        f + L.r0, f + Rt.r0, tsh);
+          wg_tmatvec<T>(F + h.U, L.r1 - L.r0, h.rank, f + L.r0, tsh);
+          __syncthreads();
+          wg_matvec_sub<T>(F + h.V, Rt.r1 - Rt.r0, h.rank, Rt.r1 - Rt.r0, tsh, f + Rt.r0);
         }
+        __syncthreads();
       }
-      if (R > 0) wg_lowrank<T>(F + S.U, c, F + S.V, R, S.rank, f, f + c, tsh);
+      if (R > 0) {
+        wg_tmatvec<T>(F + S.U, c, S.rank, f, tsh);
+        __syncthreads();
+        wg_matvec_sub<T>(F + S.V, R, S.rank, R, tsh, f + c);
+      }
+      for (int i = tid; i < c; i += 256) wc[S.first + i] = f[i];
+      __syncthreads();
     }
-    for (int i = tid; i < c; i += 256) wc[S.first + i] = f[i];
-    __syncthreads();
   }
 }
 
@@ -177,11 +210,11 @@ __global__ __launch_bounds__(256) void k_backward(
   const int j = lnodes[blockIdx.x];
   const SolveNode S = nodes[j];
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
   const int* rw = rows + S.rows_off;
   for (int col = 0; col < k; ++col) {
     double* z = fv + S.fv_off + (int64_t)col * fvs;
     double* xr = z + c;  // gathered ancestor solution
+    double* tmp = z + c + R;
     double* wc = w + (int64_t)col * n;
     for (int i = tid; i < c; i += 256) z[i] = wc[S.first + i];
     for (int a = tid; a < R; a += 256) xr[a] = wc[rw[a]];
@@ -189,31 +222,41 @@ __global__ __launch_bounds__(256) void k_backward(
     if (!S.compress) {
       const T* P = F + S.panel;
       const int ld = c + R;
-      for (int kk = wave; kk < c; kk += 4) {
-        const T* colp = P + c + (int64_t)kk * ld;
-        double acc = 0;
-        for (int a = lane; a < R; a += 64) acc += (double)colp[a] * xr[a];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) z[kk] -= acc;
-      }
+      // z -= L21^T x_R
+      wg_colreduce<T>(P + c, ld, R, c, xr, tmp);
       __syncthreads();
-      wg_upper_solve<T>(P, ld, c, z);
+      for (int i = tid; i < c; i += 256) z[i] -= tmp[i];
+      __syncthreads();
+      wg_apply_ltinv<T>(P, ld, c, z, tmp);
+      __syncthreads();
+      for (int i = tid; i < c; i += 256) wc[S.first + i] = tmp[i];
+      __syncthreads();
     } else {
-      if (R > 0) wg_lowrank_t<T>(F + S.U, c, F + S.V, R, S.rank, xr, z, tsh);
+      if (R > 0) {
+        wg_tmatvec<T>(F + S.V, R, S.rank, xr, tsh);
+        __syncthreads();
+        wg_matvec_sub<T>(F + S.U, c, S.rank, c, tsh, z);
+        __syncthreads();
+      }
       const SolveHNode* H = htree + S.htree;
       for (int o = S.nhops - 1; o >= 0; --o) {
         const SolveHNode h = H[hops[S.hops_off + o]];
         if (h.left < 0) {
-          wg_upper_solve<T>(F + h.leaf, h.r1 - h.r0, h.r1 - h.r0, z + h.r0);
+          const int b = h.r1 - h.r0;
+          wg_apply_ltinv<T>(F + h.leaf, b, b, z + h.r0, tmp);
+          __syncthreads();
+          for (int i = tid; i < b; i += 256) z[h.r0 + i] = tmp[i];
         } else {
           const SolveHNode L = H[h.left], Rt = H[h.right];
-          wg_lowrank_t<T>(F + h.U, L.r1 - L.r0, F + h.V, Rt.r1 - Rt.r0, h.rank,
-                          z + Rt.r0, z + L.r0, tsh);
+          wg_tmatvec<T>(F + h.V, Rt.r1 - Rt.r0, h.rank, z + Rt.r0, tsh);
+          __syncthreads();
+          wg_matvec_sub<T>(F + h.U, L.r1 - L.r0, h.rank, L.r1 - L.r0, tsh, z + L.r0);
         }
+        __syncthreads();
       }
+      for (int i = tid; i < c; i += 256) wc[S.first + i] = z[i];
+      __syncthreads();
     }
-    for (int i = tid; i < c; i += 256) wc[S.first + i] = z[i];
-    __syncthreads();
   }
 }
 

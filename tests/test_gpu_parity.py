@@ -115,3 +115,22 @@ def test_block_pcg_equals_single():
         r1 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, B[:, j].copy(), M)
         assert res[j]["iterations"] == r1["iterations"]
         assert np.array_equal(x, X[:, j])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp32_factor_apply_close_to_oracle(name):
+    """FP32 factor (FP64 vectors): the apply stays within FP32 factor accuracy
+    of the FP64 oracle and PCG still reaches 1e-8."""
+    build, sopts, okw, tol = CASES[name]
+    A, xyz = build()
+    sym = symbolic(A, xyz, **sopts)
+    O = oracle_for(A, sym, **okw)
+    M = _factor(A, xyz, factor_precision=R.RSC_FP32, **dict(sopts))
+    b = R.uniform_rhs(A._nrow, 5678)
+    x = np.empty_like(b)
+    M.solve(x, b)
+    assert np.all(np.isfinite(x))
+    assert relerr(x, O.apply(b)) < max(1e-3, tol)
+    x = np.zeros_like(b)
+    res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M)
+    assert res["true_relres"] < 1e-7

@@ -103,7 +103,17 @@ class Engine final : public FactorEngine {
   DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
-  DBuf dsnodes_, dhtree_, dhops_, drows_, dlevnodes_, dperm_, dfv_, dw_;
+  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_;
+  struct SolvePhase {
+    int kind;  // 0 forward assemble, 1 backward gather, 2 task list
+    int first, count;
+  };
+  std::vector<std::vector<SolvePhase>> fwd_, bwd_;
+  int64_t tv_stride_ = 0, ntasks_ = 0;
+  std::vector<int64_t> fv_off_;
+  int64_t fvoff(int j) const { return fv_off_[j]; }
+  void build_solve_tasks(const std::vector<int64_t>& tv_node,
+                         const std::vector<std::vector<int64_t>>& tv_h);
   int64_t fv_stride_ = 0;
   int fv_k_ = 0;
   std::vector<std::vector<int>> levnode_off_;  // per level: offset in dlevnodes
@@ -174,6 +184,8 @@ class Engine final : public FactorEngine {
                       int cols, int r, char* K, char* Tm);
   void hsolve_right(const Supernode& s, int q, char* X, int m, int ldx, char* Tm);
   void emit_compressed(int j);
+  void emit_solve_helpers(int j);
+  std::vector<int64_t> wx_scratch_, wx_elems_;
 };
 
 template <typename T>
@@ -589,6 +601,8 @@ void Engine<T>::layout_scratch() {
   // per internal tree node: Z (rows r), W, K (r r), Tm (rows r)
   int64_t off = 0;
   node_scratch_.assign(plan_.nodes.size(), -1);
+  wx_scratch_.assign(plan_.nodes.size(), 0);
+  wx_elems_.assign(plan_.nodes.size(), 0);
   hnode_scratch_.assign(plan_.nodes.size(), {});
   inv_elems_ = 0;
   for (auto& lv : plan_.levels) {
@@ -614,6 +628,23 @@ void Engine<T>::layout_scratch() {
     const int64_t R = s.R(), r = s.rank;
     off += R * r + 2 * r * r + (int64_t)std::max<int64_t>(R, s.ncols) * rmax;
     hnode_scratch_[j].assign(s.hnodes.size(), -1);
+    {
+      int maxh = 0;
+      for (auto& h : s.hnodes) maxh = std::max(maxh, h.height);
+      int64_t best = 0;
+      for (int hh = 1; hh < maxh; ++hh) {
+        int64_t sum = 0;
+        for (auto& hm : s.hnodes) {
+          if (hm.leaf() || hm.height <= hh) continue;
+          for (auto& hn : s.hnodes)
+            if (!hn.leaf() && hn.height == hh) sum += (int64_t)hn.rank * hm.rank;
+        }
+        best = std::max(best, sum);
+      }
+      wx_scratch_[j] = off;
+      wx_elems_[j] = best;
+      off += best;
+    }
     for (size_t q = 0; q < s.hnodes.size(); ++q) {
       const HNode& h = s.hnodes[q];
       if (h.leaf()) continue;
@@ -747,6 +778,143 @@ void Engine<T>::emit_compressed(int j) {
   for (auto& h : s.hnodes)
     if (h.leaf()) leaves.push_back({fp(h.off_leaf), h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, j, 0});
   invert(leaves);
+  ctx_ = "node " + std::to_string(j) + " solve helpers";
+  emit_solve_helpers(j);
+}
+
+// Solve helpers of a compressed diagonal: for every internal tree node M
+// with children (A, B): W_M = L_B^{-1} V_M and X_M = L_A^{-T} U_M.  With
+// them the tree solves become one leaf step plus one low-rank correction per
+// tree height:  forward  y_B -= W_M (U_M^T y_A),  backward  x_A -= X_M
+// (V_M^T x_B), both bottom-up.  Computed here with the same recursions
+// applied to V_M / U_M: leaf blocks first (leaves hold L^{-T}), then the
+// corrections of every internal node N inside the subtree, by height.
+template <typename T>
+void Engine<T>::emit_solve_helpers(int j) {
+  const Supernode& s = plan_.nodes[j];
+  int maxh = 0;
+  for (auto& h : s.hnodes) maxh = std::max(maxh, h.height);
+  if (maxh == 0) return;
+  auto inside = [&](int q, int sub) {  // node q inside subtree sub
+    for (int a = q; a >= 0; a = s.hnodes[a].parent)
+      if (a == sub) return true;
+    return false;
+  };
+  std::vector<dev::GemmJob> tn, nn;
+  for (int M = 0; M < (int)s.hnodes.size(); ++M) {
+    const HNode& hm = s.hnodes[M];
+    if (hm.leaf()) continue;
+    const HNode& A = s.hnodes[hm.left];
+    const HNode& B = s.hnodes[hm.right];
+    const int cr = B.r1 - B.r0, cl = A.r1 - A.r0, r = hm.rank;
+    for (int q = 0; q < (int)s.hnodes.size(); ++q) {
+      const HNode& lf = s.hnodes[q];
+      if (!lf.leaf()) continue;
+      const int b = lf.r1 - lf.r0;
+      dev::GemmJob g;
+      g.m = b;
+      g.n = r;
+      g.k = b;
+      g.lda = b;
+      g.alpha = 1.0;
+      g.beta = 0.0;
+      if (inside(q, hm.right)) {  // W_M rows of this leaf = Mt^T V_M rows
+        g.A = fp(lf.off_leaf);
+        g.B = fp(hm.off_V + (lf.r0 - B.r0));
+        g.C = fp(hm.off_W + (lf.r0 - B.r0));
+        g.ldb = g.ldc = cr;
+        tn.push_back(g);
+      } else if (inside(q, hm.left)) {  // X_M rows of this leaf = Mt U_M rows
+        g.A = fp(lf.off_leaf);
+        g.B = fp(hm.off_U + (lf.r0 - A.r0));
+        g.C = fp(hm.off_X + (lf.r0 - A.r0));
+        g.ldb = g.ldc = cl;
+        nn.push_back(g);
+      }
+    }
+  }
+  emit_gemm(OP_GEMM_TN, tn, false);
+  emit_gemm(OP_GEMM_NN, nn, false);
+  const int64_t tbase = scratch_base() + wx_scratch_[j];
+  for (int h = 1; h < maxh; ++h) {
+    std::vector<dev::GemmJob> t1, t2, u1, u2;
+    int64_t toff = tbase;
+    for (int M = 0; M < (int)s.hnodes.size(); ++M) {
+      const HNode& hm = s.hnodes[M];
+      if (hm.leaf() || hm.height <= h) continue;
+      const HNode& A = s.hnodes[hm.left];
+      const HNode& B = s.hnodes[hm.right];
+      const int cr = B.r1 - B.r0, cl = A.r1 - A.r0, rm = hm.rank;
+      for (int N = 0; N < (int)s.hnodes.size(); ++N) {
+        const HNode& hn = s.hnodes[N];
+        if (hn.leaf() || hn.height != h) continue;
+        const HNode& NA = s.hnodes[hn.left];
+        const HNode& NB = s.hnodes[hn.right];
+        const int ncl = NA.r1 - NA.r0, ncr = NB.r1 - NB.r0, rn = hn.rank;
+        char* Tm = wp(toff);
+        toff += (int64_t)rn * rm;
+        dev::GemmJob g;
+        g.alpha = 1.0;
+        g.beta = 0.0;
+        if (inside(N, hm.right)) {
+          // T = U_N^T W_M[N.A rows]; W_M[N.B rows] -= W_N T
+          g.A = fp(hn.off_U);
+          g.B = fp(hm.off_W + (NA.r0 - B.r0));
+          g.C = Tm;
+          g.m = rn;
+          g.n = rm;
+          g.k = ncl;
+          g.lda = ncl;
+          g.ldb = cr;
+          g.ldc = rn;
+          t1.push_back(g);
+          dev::GemmJob u;
+          u.A = fp(hn.off_W);
+          u.B = Tm;
+          u.C = fp(hm.off_W + (NB.r0 - B.r0));
+          u.m = ncr;
+          u.n = rm;
+          u.k = rn;
+          u.lda = ncr;
+          u.ldb = rn;
+          u.ldc = cr;
+          u.alpha = -1.0;
+          u.beta = 1.0;
+          u1.push_back(u);
+        } else if (inside(N, hm.left)) {
+          // T = V_N^T X_M[N.B rows]; X_M[N.A rows] -= X_N T
+          g.A = fp(hn.off_V);
+          g.B = fp(hm.off_X + (NB.r0 - A.r0));
+          g.C = Tm;
+          g.m = rn;
+          g.n = rm;
+          g.k = ncr;
+          g.lda = ncr;
+          g.ldb = cl;
+          g.ldc = rn;
+          t2.push_back(g);
+          dev::GemmJob u;
+          u.A = fp(hn.off_X);
+          u.B = Tm;
+          u.C = fp(hm.off_X + (NA.r0 - A.r0));
+          u.m = ncl;
+          u.n = rm;
+          u.k = rn;
+          u.lda = ncl;
+          u.ldb = rn;
+          u.ldc = cl;
+          u.alpha = -1.0;
+          u.beta = 1.0;
+          u2.push_back(u);
+        }
+      }
+    }
+    if (toff - tbase > wx_elems_[j]) throw std::runtime_error("helper scratch too small");
+    emit_gemm(OP_GEMM_TN, t1, false);
+    emit_gemm(OP_GEMM_TN, t2, false);
+    emit_gemm(OP_GEMM_NN, u1, false);
+    emit_gemm(OP_GEMM_NN, u2, false);
+  }
 }
 
 // Debug: after every program step, scan that step's outputs for non-finite
@@ -857,65 +1025,41 @@ void Engine<T>::setup_static() {
   drows_.upload(rows, st_);
   std::vector<int> permv(P.perm);
   dperm_.upload(permv, st_);
-  // solve descriptors
-  std::vector<SolveNode> sn(ns);
-  std::vector<SolveHNode> ht;
-  std::vector<int> hops;
-  int64_t fvo = 0;
+  // solve: per-supernode vector descriptors and per-level task phases
+  std::vector<dev::VecNode> vn(ns);
+  std::vector<int64_t> tv_node(ns, -1);
+  fv_off_.assign(ns, 0);
+  std::vector<std::vector<int64_t>> tv_h(ns);
+  int64_t fvo = 0, tvo = 0;
   for (int j = 0; j < ns; ++j) {
     const Supernode& s = P.nodes[j];
-    SolveNode& d = sn[j];
+    dev::VecNode& d = vn[j];
     d.first = s.first;
     d.ncols = s.ncols;
     d.R = s.R();
-    d.compress = s.compress ? 1 : 0;
-    d.panel = s.off_panel;
-    d.V = s.off_V;
-    d.U = s.off_U;
-    d.rank = s.rank;
-    d.htree = (int)ht.size();
-    d.hroot = 0;
-    d.child_off = child_off[j];
     d.nchild = (int)s.children.size();
-    d.relmap_off = relmap_off[j];
-    d.rows_off = rows_off[j];
+    d.child_off = child_off[j];
+    d.pad = 0;
     d.fv_off = fvo;
-    fvo += 2 * (int64_t)s.ncols + s.R();
-    d.hops_off = (int)hops.size();
-    d.nhops = 0;
+    fv_off_[j] = fvo;
+    d.rows_off = rows_off[j];
+    d.relmap_off = relmap_off[j];
+    fvo += (int64_t)s.ncols + s.R();
     if (s.compress) {
-      int root = 0;
+      tv_node[j] = tvo;
+      tvo += s.rank;
+      tv_h[j].assign(s.hnodes.size(), -1);
       for (size_t q = 0; q < s.hnodes.size(); ++q)
-        if (s.hnodes[q].parent < 0) root = (int)q;
-      std::function<void(int)> inorder = [&](int q) {
-        const HNode& h = s.hnodes[q];
-        if (!h.leaf()) inorder(h.left);
-        hops.push_back(q);
-        if (!h.leaf()) inorder(h.right);
-      };
-      inorder(root);
-      d.nhops = (int)hops.size() - d.hops_off;
-      for (size_t q = 0; q < s.hnodes.size(); ++q) {
-        const HNode& h = s.hnodes[q];
-        SolveHNode x;
-        x.r0 = h.r0;
-        x.r1 = h.r1;
-        x.left = h.left;
-        x.right = h.right;
-        x.rank = h.rank;
-        x.pad = 0;
-        x.leaf = h.off_leaf;
-        x.V = h.off_V;
-        x.U = h.off_U;
-        ht.push_back(x);
-        if (h.parent < 0) d.hroot = (int)q;
-      }
+        if (!s.hnodes[q].leaf()) {
+          tv_h[j][q] = tvo;
+          tvo += s.hnodes[q].rank;
+        }
     }
   }
-  fv_stride_ = fvo;
-  dsnodes_.upload(sn, st_);
-  dhtree_.upload(ht, st_);
-  dhops_.upload(hops, st_);
+  fv_stride_ = std::max<int64_t>(fvo, 1);
+  tv_stride_ = std::max<int64_t>(tvo, 1);
+  dvnodes_.upload(vn, st_);
+  build_solve_tasks(tv_node, tv_h);
   std::vector<int> levnodes;
   lev_off_.clear();
   for (auto& lv : P.levels) {
@@ -1137,25 +1281,210 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   const int n = plan_.n;
   if (k > fv_k_) {
     dfv_.alloc((size_t)fv_stride_ * k * sizeof(double));
+    dtv_.alloc((size_t)tv_stride_ * k * sizeof(double));
     dw_.alloc((size_t)n * k * sizeof(double));
     fv_k_ = k;
   }
-  double* w = dw_.as<double>();
-  dev::launch_gather(B, ldb, w, dperm_.as<int>(), n, k, s);
+  dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(), (int64_t)n,
+                  fv_stride_, tv_stride_};
+  dev::launch_gather(B, ldb, vb.w, dperm_.as<int>(), n, k, s);
   const int nlev = (int)plan_.levels.size();
-  for (int lv = 0; lv < nlev; ++lv)
-    dev::launch_forward<T>(F(), dsnodes_.as<SolveNode>(),
-                           dlevnodes_.as<int>() + lev_off_[lv],
-                           lev_off_[lv + 1] - lev_off_[lv], dhtree_.as<SolveHNode>(),
-                           dhops_.as<int>(), dchildren_.as<int>(), drelmap_.as<int>(), w, n, k,
-                           dfv_.as<double>(), fv_stride_, s);
-  for (int lv = nlev - 1; lv >= 0; --lv)
-    dev::launch_backward<T>(F(), dsnodes_.as<SolveNode>(),
-                            dlevnodes_.as<int>() + lev_off_[lv],
-                            lev_off_[lv + 1] - lev_off_[lv], dhtree_.as<SolveHNode>(),
-                            dhops_.as<int>(), drows_.as<int>(), w, n, k, dfv_.as<double>(),
-                            fv_stride_, s);
-  dev::launch_scatter(w, X, ldx, dperm_.as<int>(), n, k, s);
+  auto run = [&](const std::vector<SolvePhase>& ph, int lv) {
+    for (const SolvePhase& p : ph) {
+      if (p.kind == 0)
+        dev::launch_fwd_assemble(dvnodes_.as<dev::VecNode>(),
+                                 dlevnodes_.as<int>() + lev_off_[lv],
+                                 lev_off_[lv + 1] - lev_off_[lv], dchildren_.as<int>(),
+                                 drelmap_.as<int>(), vb, k, s);
+      else if (p.kind == 1)
+        dev::launch_bwd_gather(dvnodes_.as<dev::VecNode>(),
+                               dlevnodes_.as<int>() + lev_off_[lv],
+                               lev_off_[lv + 1] - lev_off_[lv], drows_.as<int>(), vb, k, s);
+      else
+        dev::launch_gemv_tasks<T>(F(), dgv_.as<dev::GvTask>() + p.first, p.count, vb, k, s);
+    }
+  };
+  for (int lv = 0; lv < nlev; ++lv) run(fwd_[lv], lv);
+  for (int lv = nlev - 1; lv >= 0; --lv) run(bwd_[lv], lv);
+  dev::launch_scatter(vb.w, X, ldx, dperm_.as<int>(), n, k, s);
+}
+
+// Task phases of the level-scheduled solves (see solve.hip).
+template <typename T>
+void Engine<T>::build_solve_tasks(const std::vector<int64_t>& tv_node,
+                                  const std::vector<std::vector<int64_t>>& tv_h) {
+  const Plan& P = plan_;
+  std::vector<dev::GvTask> gv;
+  fwd_.assign(P.levels.size(), {});
+  bwd_.assign(P.levels.size(), {});
+  const int W = 0, FV = 1, TV = 2;
+  auto col = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
+                 int ys, int64_t yo, bool upper) {
+    for (int j0 = 0; j0 < cols; j0 += 32) {
+      dev::GvTask t{};
+      t.mat = mat + (int64_t)j0 * ld;
+      t.ld = ld;
+      t.rows = upper ? std::min(rows, j0 + 32) : rows;
+      t.cols = std::min(32, cols - j0);
+      t.kind = 1;
+      t.op = op;
+      t.xsel = xs;
+      t.ysel = ys;
+      t.xoff = xo;
+      t.yoff = yo + j0;
+      if (t.rows > 0) gv.push_back(t);
+    }
+  };
+  auto row = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
+                 int ys, int64_t yo, bool upper) {
+    for (int i0 = 0; i0 < rows; i0 += 256) {
+      dev::GvTask t{};
+      t.mat = mat + i0 + (upper ? (int64_t)i0 * ld : 0);
+      t.ld = ld;
+      t.rows = std::min(256, rows - i0);
+      t.cols = upper ? cols - i0 : cols;
+      t.kind = 0;
+      t.op = op;
+      t.xsel = xs;
+      t.ysel = ys;
+      t.xoff = xo + (upper ? i0 : 0);
+      t.yoff = yo + i0;
+      gv.push_back(t);
+    }
+  };
+  auto phase = [&](std::vector<SolvePhase>& out, size_t start) {
+    if (gv.size() > start) out.push_back({2, (int)start, (int)(gv.size() - start)});
+  };
+  for (size_t lv = 0; lv < P.levels.size(); ++lv) {
+    const std::vector<int>& nodes = P.levels[lv];
+    int maxh = 0;
+    for (int j : nodes)
+      for (auto& h : P.nodes[j].hnodes) maxh = std::max(maxh, h.height);
+    // ---- forward
+    std::vector<SolvePhase>& F = fwd_[lv];
+    F.push_back({0, 0, 0});
+    size_t st = gv.size();
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      const int64_t fo = 0;
+      (void)fo;
+      if (!s.compress) {
+        col(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
+      } else {
+        for (auto& h : s.hnodes)
+          if (h.leaf())
+            col(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
+                s.first + h.r0, true);
+      }
+    }
+    phase(F, st);
+    for (int hh = 1; hh <= maxh; ++hh) {
+      st = gv.size();
+      for (int j : nodes) {
+        const Supernode& s = P.nodes[j];
+        for (size_t q = 0; q < s.hnodes.size(); ++q) {
+          const HNode& h = s.hnodes[q];
+          if (h.leaf() || h.height != hh) continue;
+          const HNode& A = s.hnodes[h.left];
+          col(h.off_U, A.r1 - A.r0, A.r1 - A.r0, h.rank, 0, W, s.first + A.r0, TV, tv_h[j][q],
+              false);
+        }
+      }
+      phase(F, st);
+      st = gv.size();
+      for (int j : nodes) {
+        const Supernode& s = P.nodes[j];
+        for (size_t q = 0; q < s.hnodes.size(); ++q) {
+          const HNode& h = s.hnodes[q];
+          if (h.leaf() || h.height != hh) continue;
+          const HNode& B = s.hnodes[h.right];
+          row(h.off_W, B.r1 - B.r0, B.r1 - B.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + B.r0,
+              false);
+        }
+      }
+      phase(F, st);
+    }
+    st = gv.size();
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      if (s.compress && s.R() > 0)
+        col(s.off_U, s.ncols, s.ncols, s.rank, 0, W, s.first, TV, tv_node[j], false);
+    }
+    phase(F, st);
+    st = gv.size();
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      if (s.R() == 0) continue;
+      if (!s.compress)
+        row(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, W, s.first, FV,
+            fvoff(j) + s.ncols, false);
+      else
+        row(s.off_V, s.R(), s.R(), s.rank, 1, TV, tv_node[j], FV, fvoff(j) + s.ncols, false);
+    }
+    phase(F, st);
+    // ---- backward
+    std::vector<SolvePhase>& Bk = bwd_[lv];
+    Bk.push_back({1, 0, 0});
+    st = gv.size();
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      if (s.R() == 0) continue;
+      if (!s.compress)
+        col(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, FV, fvoff(j) + s.ncols,
+            FV, fvoff(j), false);
+      else
+        col(s.off_V, s.R(), s.R(), s.rank, 0, FV, fvoff(j) + s.ncols, TV, tv_node[j], false);
+    }
+    phase(Bk, st);
+    st = gv.size();
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      if (s.compress && s.R() > 0)
+        row(s.off_U, s.ncols, s.ncols, s.rank, 1, TV, tv_node[j], FV, fvoff(j), false);
+    }
+    phase(Bk, st);
+    st = gv.size();
+    for (int j : nodes) {
+      const Supernode& s = P.nodes[j];
+      if (!s.compress) {
+        row(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
+      } else {
+        for (auto& h : s.hnodes)
+          if (h.leaf())
+            row(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
+                s.first + h.r0, true);
+      }
+    }
+    phase(Bk, st);
+    for (int hh = 1; hh <= maxh; ++hh) {
+      st = gv.size();
+      for (int j : nodes) {
+        const Supernode& s = P.nodes[j];
+        for (size_t q = 0; q < s.hnodes.size(); ++q) {
+          const HNode& h = s.hnodes[q];
+          if (h.leaf() || h.height != hh) continue;
+          const HNode& B = s.hnodes[h.right];
+          col(h.off_V, B.r1 - B.r0, B.r1 - B.r0, h.rank, 0, W, s.first + B.r0, TV, tv_h[j][q],
+              false);
+        }
+      }
+      phase(Bk, st);
+      st = gv.size();
+      for (int j : nodes) {
+        const Supernode& s = P.nodes[j];
+        for (size_t q = 0; q < s.hnodes.size(); ++q) {
+          const HNode& h = s.hnodes[q];
+          if (h.leaf() || h.height != hh) continue;
+          const HNode& A = s.hnodes[h.left];
+          row(h.off_X, A.r1 - A.r0, A.r1 - A.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + A.r0,
+              false);
+        }
+      }
+      phase(Bk, st);
+    }
+  }
+  dgv_.upload(gv, st_);
+  ntasks_ = (int64_t)gv.size();
 }
 
 }  // namespace

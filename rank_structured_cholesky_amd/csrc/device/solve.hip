@@ -5,12 +5,16 @@
 // 4571-5250; DiagonalBlock::BlockNode::diagonalSolve, DiagonalBlock.cpp:
 // 1306-1378) and the CSR SpMV of the PCG operator.
 //
-// Level-scheduled: one launch per elimination-tree level and direction, one
-// workgroup per supernode; diagonal blocks are applied through their stored
-// inverses (L^{-T}), so every step is a matrix-vector product.  The forward sweep is multifrontal: a supernode's
-// front vector receives its children's update vectors (no atomics), solves
-// with its diagonal block and leaves its own update vector for the parent.
-// Vectors are FP64; the factor may be FP32 (widened on load).
+// Level-scheduled and task-parallel: per elimination-tree level and
+// direction a few phases, each one launch of independent matrix-vector
+// tasks (a task = up to 256 rows or 32 columns of one factor block), so the
+// large blocks near the top of the tree spread over all compute units.
+// Diagonal blocks are applied through stored inverses (L^{-T}); compressed
+// diagonals use one leaf step plus one low-rank correction per tree height
+// (precomputed W = L_B^{-1} V, X = L_A^{-T} U).  The forward sweep is
+// multifrontal: a supernode's front vector receives its children's update
+// vectors (no atomics).  Vectors are FP64; the factor may be FP32 (widened
+// on load).
 #include "solve.h"
 
 namespace rsc {
@@ -46,254 +50,136 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
   k_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(w, x, ldx, perm, n, k);
 }
 
-// ---- workgroup helpers ----------------------------------------------------
-// Dense diagonal triangles are stored inverted and transposed (Mt = L^{-T},
-// upper triangular, leading dimension ld), so both triangular solves are
-// matrix-vector products without per-column barriers.
+// ---- vector gathers of a level -------------------------------------------
+
+__global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict__ nodes,
+                                                      const int* __restrict__ lnodes,
+                                                      const int* __restrict__ children,
+                                                      const int* __restrict__ relmap,
+                                                      VecBufs b, int k) {
+  const VecNode S = nodes[lnodes[blockIdx.x]];
+  const int c = S.ncols, R = S.R, tid = threadIdx.x;
+  for (int col = 0; col < k; ++col) {
+    double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
+    const double* wc = b.w + (int64_t)col * b.n;
+    for (int i = tid; i < c; i += 256) f[i] = wc[S.first + i];
+    for (int i = tid; i < R; i += 256) f[c + i] = 0.0;
+    __syncthreads();
+    for (int q = 0; q < S.nchild; ++q) {
+      const VecNode C = nodes[children[S.child_off + q]];
+      const int* rm = relmap + C.relmap_off;
+      const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
+      for (int a = tid; a < C.R; a += 256) f[rm[a]] += fc[a];
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ nodes,
+                                                    const int* __restrict__ lnodes,
+                                                    const int* __restrict__ rows,
+                                                    VecBufs b, int k) {
+  const VecNode S = nodes[lnodes[blockIdx.x]];
+  const int c = S.ncols, R = S.R, tid = threadIdx.x;
+  const int* rw = rows + S.rows_off;
+  for (int col = 0; col < k; ++col) {
+    double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
+    const double* wc = b.w + (int64_t)col * b.n;
+    for (int i = tid; i < c; i += 256) f[i] = wc[S.first + i];
+    for (int a = tid; a < R; a += 256) f[c + a] = wc[rw[a]];
+  }
+}
+
+void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
+                         const int* children, const int* relmap, VecBufs b, int k,
+                         hipStream_t s) {
+  if (count <= 0) return;
+  k_fwd_assemble<<<count, 256, 0, s>>>(nodes, level_nodes, children, relmap, b, k);
+}
+
+void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
+                       const int* rows, VecBufs b, int k, hipStream_t s) {
+  if (count <= 0) return;
+  k_bwd_gather<<<count, 256, 0, s>>>(nodes, level_nodes, rows, b, k);
+}
+
+// ---- matrix-vector tasks ---------------------------------------------------
 
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
 }
 
-// Column-reduction GEMV: out[q] = sum_{i<m} W[i + q*ld] * src[i] for q < r.
-// Each wave owns 8 columns at a time (8 independent load streams per lane),
-// lanes stride the contiguous rows, then 8 wavefront reductions.
-template <typename T>
-__device__ void wg_colreduce(const T* __restrict__ W, int64_t ld, int m, int r,
-                             const double* src, double* out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int q0 = wave * 8; q0 < r; q0 += 32) {
-    double acc[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = 0;
-    const int nq = min(8, r - q0);
-    for (int i = lane; i < m; i += 64) {
-      const double xi = src[i];
-      const T* base = W + i + (int64_t)q0 * ld;
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (u < nq) acc[u] += (double)base[(int64_t)u * ld] * xi;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      double v = wave_sum(acc[u]);
-      if (lane == 0 && u < nq) out[q0 + u] = v;
-    }
-  }
+__device__ inline double* vsel(const VecBufs& b, int sel, int col) {
+  if (sel == 0) return b.w + (int64_t)col * b.n;
+  if (sel == 1) return b.fv + (int64_t)col * b.fvs;
+  return b.tv + (int64_t)col * b.tvs;
 }
 
-// Row-form GEMV: dst[i] (+|-)= sum_{q<r} W[i + q*ld] * t[q] for i < m, one
-// thread per row (coalesced), inner loop unrolled for loads in flight.
-template <typename T, bool SUB>
-__device__ void wg_rowgemv(const T* __restrict__ W, int64_t ld, int m, int r,
-                           const double* t, double* dst) {
-  for (int i = threadIdx.x; i < m; i += 256) {
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    const T* row = W + i;
-    int q = 0;
-    for (; q + 4 <= r; q += 4) {
-      a0 += (double)row[(int64_t)q * ld] * t[q];
-      a1 += (double)row[(int64_t)(q + 1) * ld] * t[q + 1];
-      a2 += (double)row[(int64_t)(q + 2) * ld] * t[q + 2];
-      a3 += (double)row[(int64_t)(q + 3) * ld] * t[q + 3];
-    }
-    for (; q < r; ++q) a0 += (double)row[(int64_t)q * ld] * t[q];
-    const double acc = (a0 + a1) + (a2 + a3);
-    if (SUB) dst[i] -= acc;
-    else dst[i] = acc;
-  }
-}
-
-// out = L^{-1} f with Mt = L^{-T}: out[i] = sum_k Mt[k,i] f[k]
 template <typename T>
-__device__ void wg_apply_linv(const T* __restrict__ Mt, int ld, int b, const double* f,
-                              double* out) {
-  wg_colreduce<T>(Mt, ld, b, b, f, out);
-}
-
-// out = L^{-T} z with Mt = L^{-T}: out[i] = sum_k Mt[i,k] z[k] (the strictly
-// lower part of Mt holds exact zeros)
-template <typename T>
-__device__ void wg_apply_ltinv(const T* __restrict__ Mt, int ld, int b, const double* z,
-                               double* out) {
-  wg_rowgemv<T, false>(Mt, ld, b, b, z, out);
-}
-
-// t[q] = sum_i W[i,q] src[i] for q < r (W m x r)
-template <typename T>
-__device__ void wg_tmatvec(const T* __restrict__ W, int m, int r, const double* src,
-                           double* t) {
-  wg_colreduce<T>(W, m, m, r, src, t);
-}
-
-// dst[i] -= sum_q W[i,q] t[q] for i < m (W m x r, ld ldw)
-template <typename T>
-__device__ void wg_matvec_sub(const T* __restrict__ W, int m, int r, int64_t ldw,
-                              const double* t, double* dst) {
-  wg_rowgemv<T, true>(W, ldw, m, r, t, dst);
-}
-
-// ---- forward sweep ---------------------------------------------------------
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_forward(
-    const T* __restrict__ F, const SolveNode* __restrict__ nodes,
-    const int* __restrict__ lnodes, const SolveHNode* __restrict__ htree,
-    const int* __restrict__ hops, const int* __restrict__ children,
-    const int* __restrict__ relmap, double* __restrict__ w, int n, int k,
-    double* __restrict__ fv, int64_t fvs) {
-  __shared__ double tsh[MAXR];
-  const int j = lnodes[blockIdx.x];
-  const SolveNode S = nodes[j];
-  const int c = S.ncols, R = S.R, tid = threadIdx.x;
+__global__ __launch_bounds__(256) void k_gemv_tasks(const T* __restrict__ F,
+                                                    const GvTask* __restrict__ tasks,
+                                                    VecBufs b, int k) {
+  const GvTask t = tasks[blockIdx.x];
+  const T* __restrict__ M = F + t.mat;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int col = 0; col < k; ++col) {
-    double* f = fv + S.fv_off + (int64_t)col * fvs;
-    double* tmp = f + c + R;
-    double* wc = w + (int64_t)col * n;
-    for (int i = tid; i < c; i += 256) f[i] = wc[S.first + i];
-    for (int i = tid; i < R; i += 256) f[c + i] = 0.0;
-    __syncthreads();
-    for (int q = 0; q < S.nchild; ++q) {
-      const SolveNode C = nodes[children[S.child_off + q]];
-      const int* rm = relmap + C.relmap_off;
-      const double* fc = fv + C.fv_off + (int64_t)col * fvs + C.ncols;
-      for (int a = tid; a < C.R; a += 256) f[rm[a]] += fc[a];
-      __syncthreads();
-    }
-    if (!S.compress) {
-      const T* P = F + S.panel;
-      const int ld = c + R;
-      wg_apply_linv<T>(P, ld, c, f, tmp);
-      __syncthreads();
-      // f_R -= L21 y
-      wg_rowgemv<T, true>(P + c, ld, R, c, tmp, f + c);
-      for (int i = tid; i < c; i += 256) wc[S.first + i] = tmp[i];
-      __syncthreads();
-    } else {
-      const SolveHNode* H = htree + S.htree;
-      for (int o = 0; o < S.nhops; ++o) {
-        const SolveHNode h = H[hops[S.hops_off + o]];
-        if (h.left < 0) {
-          const int b = h.r1 - h.r0;
-          wg_apply_linv<T>(F + h.leaf, b, b, f + h.r0, tmp);
-          __syncthreads();
-          for (int i = tid; i < b; i += 256) f[h.r0 + i] = tmp[i];
-        } else {
-          const SolveHNode L = H[h.left], Rt = H[h.right];
-          wg_tmatvec<T>(F + h.U, L.r1 - L.r0, h.rank, f + L.r0, tsh);
-          __syncthreads();
-          wg_matvec_sub<T>(F + h.V, Rt.r1 - Rt.r0, h.rank, Rt.r1 - Rt.r0, tsh, f + Rt.r0);
+    const double* x = vsel(b, t.xsel, col) + t.xoff;
+    double* y = vsel(b, t.ysel, col) + t.yoff;
+    if (t.kind == 0) {
+      for (int i = tid; i < t.rows; i += 256) {
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        const T* row = M + i;
+        int j = 0;
+        for (; j + 4 <= t.cols; j += 4) {
+          a0 += (double)row[(int64_t)j * t.ld] * x[j];
+          a1 += (double)row[(int64_t)(j + 1) * t.ld] * x[j + 1];
+          a2 += (double)row[(int64_t)(j + 2) * t.ld] * x[j + 2];
+          a3 += (double)row[(int64_t)(j + 3) * t.ld] * x[j + 3];
         }
-        __syncthreads();
+        for (; j < t.cols; ++j) a0 += (double)row[(int64_t)j * t.ld] * x[j];
+        const double acc = (a0 + a1) + (a2 + a3);
+        if (t.op) y[i] -= acc;
+        else y[i] = acc;
       }
-      if (R > 0) {
-        wg_tmatvec<T>(F + S.U, c, S.rank, f, tsh);
-        __syncthreads();
-        wg_matvec_sub<T>(F + S.V, R, S.rank, R, tsh, f + c);
-      }
-      for (int i = tid; i < c; i += 256) wc[S.first + i] = f[i];
-      __syncthreads();
-    }
-  }
-}
-
-// ---- backward sweep --------------------------------------------------------
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_backward(
-    const T* __restrict__ F, const SolveNode* __restrict__ nodes,
-    const int* __restrict__ lnodes, const SolveHNode* __restrict__ htree,
-    const int* __restrict__ hops, const int* __restrict__ rows,
-    double* __restrict__ w, int n, int k, double* __restrict__ fv, int64_t fvs) {
-  __shared__ double tsh[MAXR];
-  const int j = lnodes[blockIdx.x];
-  const SolveNode S = nodes[j];
-  const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int* rw = rows + S.rows_off;
-  for (int col = 0; col < k; ++col) {
-    double* z = fv + S.fv_off + (int64_t)col * fvs;
-    double* xr = z + c;  // gathered ancestor solution
-    double* tmp = z + c + R;
-    double* wc = w + (int64_t)col * n;
-    for (int i = tid; i < c; i += 256) z[i] = wc[S.first + i];
-    for (int a = tid; a < R; a += 256) xr[a] = wc[rw[a]];
-    __syncthreads();
-    if (!S.compress) {
-      const T* P = F + S.panel;
-      const int ld = c + R;
-      // z -= L21^T x_R
-      wg_colreduce<T>(P + c, ld, R, c, xr, tmp);
-      __syncthreads();
-      for (int i = tid; i < c; i += 256) z[i] -= tmp[i];
-      __syncthreads();
-      wg_apply_ltinv<T>(P, ld, c, z, tmp);
-      __syncthreads();
-      for (int i = tid; i < c; i += 256) wc[S.first + i] = tmp[i];
-      __syncthreads();
     } else {
-      if (R > 0) {
-        wg_tmatvec<T>(F + S.V, R, S.rank, xr, tsh);
-        __syncthreads();
-        wg_matvec_sub<T>(F + S.U, c, S.rank, c, tsh, z);
-        __syncthreads();
-      }
-      const SolveHNode* H = htree + S.htree;
-      for (int o = S.nhops - 1; o >= 0; --o) {
-        const SolveHNode h = H[hops[S.hops_off + o]];
-        if (h.left < 0) {
-          const int b = h.r1 - h.r0;
-          wg_apply_ltinv<T>(F + h.leaf, b, b, z + h.r0, tmp);
-          __syncthreads();
-          for (int i = tid; i < b; i += 256) z[h.r0 + i] = tmp[i];
-        } else {
-          const SolveHNode L = H[h.left], Rt = H[h.right];
-          wg_tmatvec<T>(F + h.V, Rt.r1 - Rt.r0, h.rank, z + Rt.r0, tsh);
-          __syncthreads();
-          wg_matvec_sub<T>(F + h.U, L.r1 - L.r0, h.rank, L.r1 - L.r0, tsh, z + L.r0);
+      const int q0 = wave * 8;
+      const int nq = min(8, t.cols - q0);
+      if (nq > 0) {
+        double acc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = 0;
+        for (int i = lane; i < t.rows; i += 64) {
+          const double xi = x[i];
+          const T* base = M + i + (int64_t)q0 * t.ld;
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (u < nq) acc[u] += (double)base[(int64_t)u * t.ld] * xi;
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const double v = wave_sum(acc[u]);
+          if (lane == 0 && u < nq) {
+            if (t.op) y[q0 + u] -= v;
+            else y[q0 + u] = v;
+          }
+        }
       }
-      for (int i = tid; i < c; i += 256) wc[S.first + i] = z[i];
-      __syncthreads();
     }
   }
 }
 
 template <typename T>
-void launch_forward(const T* factor, const SolveNode* nodes, const int* level_nodes,
-                    int count, const SolveHNode* htree, const int* hops,
-                    const int* children, const int* relmap, double* w, int n, int k,
-                    double* fv, int64_t fv_stride, hipStream_t s) {
+void launch_gemv_tasks(const T* factor, const GvTask* tasks, int count, VecBufs b,
+                       int k, hipStream_t s) {
   if (count <= 0) return;
-  k_forward<T><<<count, 256, 0, s>>>(factor, nodes, level_nodes, htree, hops,
-                                     children, relmap, w, n, k, fv, fv_stride);
+  k_gemv_tasks<T><<<count, 256, 0, s>>>(factor, tasks, b, k);
 }
 
-template <typename T>
-void launch_backward(const T* factor, const SolveNode* nodes, const int* level_nodes,
-                     int count, const SolveHNode* htree, const int* hops,
-                     const int* rows, double* w, int n, int k, double* fv,
-                     int64_t fv_stride, hipStream_t s) {
-  if (count <= 0) return;
-  k_backward<T><<<count, 256, 0, s>>>(factor, nodes, level_nodes, htree, hops, rows,
-                                      w, n, k, fv, fv_stride);
-}
-
-template void launch_forward<double>(const double*, const SolveNode*, const int*, int,
-                                     const SolveHNode*, const int*, const int*,
-                                     const int*, double*, int, int, double*, int64_t,
-                                     hipStream_t);
-template void launch_forward<float>(const float*, const SolveNode*, const int*, int,
-                                    const SolveHNode*, const int*, const int*,
-                                    const int*, double*, int, int, double*, int64_t,
-                                    hipStream_t);
-template void launch_backward<double>(const double*, const SolveNode*, const int*, int,
-                                      const SolveHNode*, const int*, const int*,
-                                      double*, int, int, double*, int64_t, hipStream_t);
-template void launch_backward<float>(const float*, const SolveNode*, const int*, int,
-                                     const SolveHNode*, const int*, const int*,
-                                     double*, int, int, double*, int64_t, hipStream_t);
+template void launch_gemv_tasks<double>(const double*, const GvTask*, int, VecBufs, int,
+                                        hipStream_t);
+template void launch_gemv_tasks<float>(const float*, const GvTask*, int, VecBufs, int,
+                                       hipStream_t);
 
 // ---- SpMV ------------------------------------------------------------------
 // LPR lanes cooperate on one row; the row's partial sums are combined with

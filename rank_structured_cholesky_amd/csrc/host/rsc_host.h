@@ -100,6 +100,9 @@ struct HNode {
   int rank = 0;            // off-diagonal rank (internal nodes)
   // block rows = right child's range, cols = left child's range
   int64_t off_V = -1, off_U = -1;  // factor storage (elements)
+  // solve helpers: W = L_right^{-1} V (rows x r), X = L_left^{-T} U (cols x r)
+  int64_t off_W = -1, off_X = -1;
+  int height = 0;                  // 0 for leaves
   int64_t off_leaf = -1;           // factor storage of a leaf (b x b)
   int64_t rng_off = -1;            // offset of the sketch in the RNG stream
 };

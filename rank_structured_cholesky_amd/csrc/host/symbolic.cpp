@@ -127,6 +127,8 @@ void build_hodlr(Supernode& s, const std::vector<int>& leaf_off, int explicit_th
     cur.swap(next);
   }
   int root = cur[0];
+  for (auto& h : s.hnodes)
+    if (!h.leaf()) h.height = std::max(s.hnodes[h.left].height, s.hnodes[h.right].height) + 1;
   bool use_explicit = nleaf > 1 && explicit_thr > 0;
   std::function<void(int)> trav = [&](int id) {
     HNode& h = s.hnodes[id];
@@ -202,6 +204,11 @@ void replan_ranks(Plan& plan, double diag_const) {
         foff += (int64_t)rows * h.rank;
         h.off_U = foff;
         foff += (int64_t)cols * h.rank;
+        h.off_W = foff;
+        foff += (int64_t)rows * h.rank;
+        h.off_X = foff;
+        foff += (int64_t)cols * h.rank;
+        // forward reads U and W, backward V and X
         apply += 2.0 * (double)h.rank * (rows + cols);
       }
     }

@@ -105,11 +105,12 @@ class Engine final : public FactorEngine {
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
   DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_;
   struct SolvePhase {
-    int kind;  // 0 forward assemble, 1 backward gather, 2 task list
+    int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks
     int first, count;
   };
   std::vector<std::vector<SolvePhase>> fwd_, bwd_;
-  int64_t tv_stride_ = 0, ntasks_ = 0;
+  int64_t tv_stride_ = 0, ntasks_ = 0, part_stride_ = 1;
+  DBuf dpart_;
   std::vector<int64_t> fv_off_;
   int64_t fvoff(int j) const { return fv_off_[j]; }
   void build_solve_tasks(const std::vector<int64_t>& tv_node,
@@ -118,6 +119,7 @@ class Engine final : public FactorEngine {
   int fv_k_ = 0;
   std::vector<std::vector<int>> levnode_off_;  // per level: offset in dlevnodes
   std::vector<int> lev_off_;
+  std::vector<int> lev_front_;  // largest front (ncols + R) per level
   bool program_ready_ = false;
   int pivot_replacements_ = 0;
   // program
@@ -1067,6 +1069,12 @@ void Engine<T>::setup_static() {
     levnodes.insert(levnodes.end(), lv.begin(), lv.end());
   }
   lev_off_.push_back((int)levnodes.size());
+  lev_front_.clear();
+  for (auto& lv : P.levels) {
+    int mf = 1;
+    for (int j : lv) mf = std::max(mf, P.nodes[j].ncols + P.nodes[j].R());
+    lev_front_.push_back(mf);
+  }
   dlevnodes_.upload(levnodes, st_);
   std::vector<int> status = {0, INT_MAX, 0, INT_MAX};
   const char* dbg = std::getenv("RSC_DEBUG_CHECK");
@@ -1282,11 +1290,12 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   if (k > fv_k_) {
     dfv_.alloc((size_t)fv_stride_ * k * sizeof(double));
     dtv_.alloc((size_t)tv_stride_ * k * sizeof(double));
+    dpart_.alloc((size_t)part_stride_ * k * sizeof(double));
     dw_.alloc((size_t)n * k * sizeof(double));
     fv_k_ = k;
   }
-  dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(), (int64_t)n,
-                  fv_stride_, tv_stride_};
+  dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(),
+                  dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_};
   dev::launch_gather(B, ldb, vb.w, dperm_.as<int>(), n, k, s);
   const int nlev = (int)plan_.levels.size();
   auto run = [&](const std::vector<SolvePhase>& ph, int lv) {
@@ -1295,13 +1304,15 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
         dev::launch_fwd_assemble(dvnodes_.as<dev::VecNode>(),
                                  dlevnodes_.as<int>() + lev_off_[lv],
                                  lev_off_[lv + 1] - lev_off_[lv], dchildren_.as<int>(),
-                                 drelmap_.as<int>(), vb, k, s);
+                                 drelmap_.as<int>(), vb, k, lev_front_[lv], s);
       else if (p.kind == 1)
         dev::launch_bwd_gather(dvnodes_.as<dev::VecNode>(),
                                dlevnodes_.as<int>() + lev_off_[lv],
-                               lev_off_[lv + 1] - lev_off_[lv], drows_.as<int>(), vb, k, s);
+                               lev_off_[lv + 1] - lev_off_[lv], drows_.as<int>(), vb, k,
+                               lev_front_[lv], s);
       else
-        dev::launch_gemv_tasks<T>(F(), dgv_.as<dev::GvTask>() + p.first, p.count, vb, k, s);
+        dev::launch_gemv_tasks<T>(F(), dgv_.as<dev::GvTask>() + p.first, p.count, vb, k,
+                                  p.kind == 3, s);
     }
   };
   for (int lv = 0; lv < nlev; ++lv) run(fwd_[lv], lv);
@@ -1318,42 +1329,92 @@ void Engine<T>::build_solve_tasks(const std::vector<int64_t>& tv_node,
   fwd_.assign(P.levels.size(), {});
   bwd_.assign(P.levels.size(), {});
   const int W = 0, FV = 1, TV = 2;
+  // COL tasks: 32 columns; more than CHUNK rows are split into row chunks
+  // writing partial sums, reduced by a second launch (`red`).
+  const int CHUNK = 512;
+  std::vector<dev::GvTask> red;
+  int64_t slot = 0;
   auto col = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
                  int ys, int64_t yo, bool upper) {
     for (int j0 = 0; j0 < cols; j0 += 32) {
+      const int rr = upper ? std::min(rows, j0 + 32) : rows;
+      if (rr <= 0) continue;
       dev::GvTask t{};
-      t.mat = mat + (int64_t)j0 * ld;
       t.ld = ld;
-      t.rows = upper ? std::min(rows, j0 + 32) : rows;
       t.cols = std::min(32, cols - j0);
       t.kind = 1;
       t.op = op;
       t.xsel = xs;
       t.ysel = ys;
-      t.xoff = xo;
       t.yoff = yo + j0;
-      if (t.rows > 0) gv.push_back(t);
+      if (rr <= CHUNK) {
+        t.mat = mat + (int64_t)j0 * ld;
+        t.rows = rr;
+        t.xoff = xo;
+        t.part = -1;
+        gv.push_back(t);
+        continue;
+      }
+      const int nsub = (rr + CHUNK - 1) / CHUNK;
+      for (int s2 = 0; s2 < nsub; ++s2) {
+        dev::GvTask u = t;
+        u.mat = mat + (int64_t)j0 * ld + (int64_t)s2 * CHUNK;
+        u.rows = std::min(CHUNK, rr - s2 * CHUNK);
+        u.xoff = xo + (int64_t)s2 * CHUNK;
+        u.part = (int)(slot + s2);
+        gv.push_back(u);
+      }
+      dev::GvTask rd{};
+      rd.kind = 2;
+      rd.part = (int)slot;
+      rd.rows = nsub;
+      rd.cols = t.cols;
+      rd.op = op;
+      rd.ysel = ys;
+      rd.yoff = yo + j0;
+      red.push_back(rd);
+      slot += nsub;
     }
   };
+  // ROW tasks: 64 rows, all columns
   auto row = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
                  int ys, int64_t yo, bool upper) {
-    for (int i0 = 0; i0 < rows; i0 += 256) {
+    for (int i0 = 0; i0 < rows; i0 += 64) {
       dev::GvTask t{};
       t.mat = mat + i0 + (upper ? (int64_t)i0 * ld : 0);
       t.ld = ld;
-      t.rows = std::min(256, rows - i0);
+      t.rows = std::min(64, rows - i0);
       t.cols = upper ? cols - i0 : cols;
       t.kind = 0;
       t.op = op;
       t.xsel = xs;
       t.ysel = ys;
+      t.part = -1;
       t.xoff = xo + (upper ? i0 : 0);
       t.yoff = yo + i0;
       gv.push_back(t);
     }
   };
+  // a phase = small tasks (kind 2 launch) + large tasks (kind 3 launch),
+  // then the partial-sum reductions of its split column tasks
+  auto is_big = [](const dev::GvTask& t) {
+    return t.kind == 0 ? t.cols > 64 : t.rows > 128;
+  };
   auto phase = [&](std::vector<SolvePhase>& out, size_t start) {
-    if (gv.size() > start) out.push_back({2, (int)start, (int)(gv.size() - start)});
+    if (gv.size() > start) {
+      std::stable_partition(gv.begin() + start, gv.end(),
+                            [&](const dev::GvTask& t) { return !is_big(t); });
+      size_t mid = start;
+      while (mid < gv.size() && !is_big(gv[mid])) ++mid;
+      if (mid > start) out.push_back({2, (int)start, (int)(mid - start)});
+      if (gv.size() > mid) out.push_back({3, (int)mid, (int)(gv.size() - mid)});
+    }
+    if (!red.empty()) {
+      const size_t r0 = gv.size();
+      gv.insert(gv.end(), red.begin(), red.end());
+      red.clear();
+      out.push_back({2, (int)r0, (int)(gv.size() - r0)});
+    }
   };
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
     const std::vector<int>& nodes = P.levels[lv];
@@ -1485,6 +1546,7 @@ void Engine<T>::build_solve_tasks(const std::vector<int64_t>& tv_node,
   }
   dgv_.upload(gv, st_);
   ntasks_ = (int64_t)gv.size();
+  part_stride_ = std::max<int64_t>(slot * 32, 1);
 }
 
 }  // namespace

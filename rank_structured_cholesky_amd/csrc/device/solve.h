@@ -22,13 +22,16 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
 // j < cols (waves reduce over rows).  M is the factor slice at `mat`
 // (leading dimension ld); x / y live in w (stride n), the front vectors fv
 // (stride fvs) or the rank-sized buffers tv (stride tvs): sel 0 / 1 / 2.
+// A COL task over many rows is split into row chunks that write partial sums
+// (part >= 0: slot of 32 partials in the partial buffer); a reduce task then
+// adds a group's partials in fixed order and applies op to y.
 struct GvTask {
   int64_t mat;
   int ld, rows, cols;
-  int kind;  // 0 ROW, 1 COL
+  int kind;  // 0 ROW, 1 COL, 2 REDUCE (part = first slot, rows = slot count)
   int op;    // 0 set, 1 subtract
   int xsel, ysel;
-  int pad;
+  int part;
   int64_t xoff, yoff;
 };
 
@@ -45,19 +48,20 @@ struct VecBufs {
   double* w;
   double* fv;
   double* tv;
-  int64_t n, fvs, tvs;
+  double* part;
+  int64_t n, fvs, tvs, ps;
 };
 
 // forward: fv_c = w[cols], fv_R = 0, plus children's fv_R via relmap
 void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
                          const int* children, const int* relmap, VecBufs b, int k,
-                         hipStream_t s);
+                         int max_front, hipStream_t s);
 // backward: fv_c = w[cols], fv_R[a] = w[rows[a]]
 void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
-                       const int* rows, VecBufs b, int k, hipStream_t s);
+                       const int* rows, VecBufs b, int k, int max_front, hipStream_t s);
 template <typename T>
 void launch_gemv_tasks(const T* factor, const GvTask* tasks, int count, VecBufs b,
-                       int k, hipStream_t s);
+                       int k, bool big, hipStream_t s);
 
 // CSR (= CSC of the symmetric matrix) y = A x for k columns.
 void launch_spmv(int n, const int* p, const int* idx, const double* val,

@@ -491,8 +491,6 @@ namespace {
 
 struct SepTree {
   const std::vector<double>& pts;  // 3 per point
-  std::vector<int> order;
-  std::vector<int> leaves;
 
   explicit SepTree(const std::vector<double>& p) : pts(p) {}
 
@@ -514,11 +512,16 @@ struct SepTree {
     return maxAxis;
   }
 
-  // returns the height; appends leaves in left-to-right order
-  int build(std::vector<int>& data, int maxLeaf) {
-    if ((int)data.size() <= maxLeaf) {
-      for (int v : data) order.push_back(v);
-      leaves.push_back((int)data.size());
+  // Builds the subtree of `data` into (ord, lv) and returns its height.
+  // min_height > 0 forces splits below maxLeaf until that height is reached;
+  // it is used only when the reference's one-level "full tree" override
+  // leaves sibling heights unequal (SeparatorTree.cpp:322-356 asserts equality
+  // there, and DiagonalBlock.cpp:106-121 needs a power-of-two leaf count).
+  int build(std::vector<int> data, int maxLeaf, int min_height, std::vector<int>& ord,
+            std::vector<int>& lv) {
+    if ((int)data.size() <= maxLeaf && min_height <= 0) {
+      ord.insert(ord.end(), data.begin(), data.end());
+      lv.push_back((int)data.size());
       return 0;
     }
     int axis = longest_axis(data);
@@ -530,19 +533,31 @@ struct SepTree {
     std::vector<int> L(data.begin(), data.begin() + leftSize);
     std::vector<int> R(data.begin() + leftSize, data.end());
     if ((int)L.size() == (int)data.size() || R.empty()) {
-      for (int v : data) order.push_back(v);
-      leaves.push_back((int)data.size());
+      ord.insert(ord.end(), data.begin(), data.end());
+      lv.push_back((int)data.size());
       return 0;
     }
-    int lh, rh;
-    if ((int)R.size() > maxLeaf && (int)L.size() <= maxLeaf)
-      lh = build(L, (int)L.size() - 1);
-    else
-      lh = build(L, maxLeaf);
-    if ((int)L.size() > maxLeaf && (int)R.size() <= maxLeaf)
-      rh = build(R, (int)R.size() - 1);
-    else
-      rh = build(R, maxLeaf);
+    const int sub = std::max(min_height - 1, 0);
+    int lmax = ((int)R.size() > maxLeaf && (int)L.size() <= maxLeaf) ? (int)L.size() - 1
+                                                                      : maxLeaf;
+    int rmax = ((int)L.size() > maxLeaf && (int)R.size() <= maxLeaf) ? (int)R.size() - 1
+                                                                      : maxLeaf;
+    std::vector<int> lo, ll, ro, rl;
+    int lh = build(L, lmax, sub, lo, ll);
+    int rh = build(R, rmax, sub, ro, rl);
+    if (lh < rh) {
+      lo.clear();
+      ll.clear();
+      lh = build(L, lmax, rh, lo, ll);
+    } else if (rh < lh) {
+      ro.clear();
+      rl.clear();
+      rh = build(R, rmax, lh, ro, rl);
+    }
+    ord.insert(ord.end(), lo.begin(), lo.end());
+    ord.insert(ord.end(), ro.begin(), ro.end());
+    lv.insert(lv.end(), ll.begin(), ll.end());
+    lv.insert(lv.end(), rl.begin(), rl.end());
     return 1 + std::max(lh, rh);
   }
 };
@@ -554,9 +569,9 @@ void separator_tree_order(const std::vector<double>& pts3, int max_leaf,
   SepTree t(pts3);
   std::vector<int> data(pts3.size() / 3);
   std::iota(data.begin(), data.end(), 0);
-  t.build(data, max_leaf);
-  order = t.order;
-  leaf_sizes = t.leaves;
+  order.clear();
+  leaf_sizes.clear();
+  t.build(std::move(data), max_leaf, 0, order, leaf_sizes);
 }
 
 std::vector<std::vector<int>> reorder_nested_separators(

@@ -1,0 +1,322 @@
+"""Host ordering and symbolic analysis (no GPU): nested dissection validity,
+elimination tree / column counts against brute-force symbolic elimination,
+BuildRelaxedSupernodes against an independent restatement, separator tree
+invariants, supernode row structures against brute force, rank heuristic."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import rank_structured_cholesky_amd as R
+from rank_structured_cholesky_amd import _lib
+from helpers import symbolic
+
+
+def nd(A, xyz=None, nd_small=200):
+    n = A._nrow
+    perm = np.empty(n, np.int32)
+    cp = np.empty(n, np.int32)
+    cm = np.empty(n, np.int32)
+    nc = C.c_int()
+    pos = None if xyz is None else _lib.as_f64(np.asarray(xyz).reshape(-1, 3))
+    _lib.check(_lib.lib().rsc_nested_dissection(n, _lib.iptr(A._p), _lib.iptr(A._i),
+                                                _lib.dptr(pos), nd_small, _lib.iptr(perm),
+                                                C.byref(nc), _lib.iptr(cp), _lib.iptr(cm)))
+    return perm, cp[: nc.value].copy(), cm
+
+
+def etree_counts(A, perm):
+    n = A._nrow
+    par = np.empty(n, np.int32)
+    cc = np.empty(n, np.int32)
+    _lib.check(_lib.lib().rsc_etree_colcounts(n, _lib.iptr(A._p), _lib.iptr(A._i),
+                                              _lib.iptr(_lib.as_i32(perm)), _lib.iptr(par),
+                                              _lib.iptr(cc)))
+    return par, cc
+
+
+def brute_structure(A, perm):
+    """Boolean symbolic Cholesky of P A P^T: column patterns of L."""
+    n = A._nrow
+    S = A.to_scipy().toarray() != 0
+    S = S[np.ix_(perm, perm)]
+    cols = []
+    for k in range(n):
+        cols.append(set(np.nonzero(S[k + 1:, k])[0] + k + 1))
+    for k in range(n):
+        if cols[k]:
+            p = min(cols[k])
+            cols[p] |= {i for i in cols[k] if i > p}
+    return cols
+
+
+CASES = [
+    ("p2d8", lambda: R.poisson2d(8), 10),
+    ("p2d12", lambda: R.poisson2d(12), 20),
+    ("p3d5", lambda: R.poisson3d(5, 1.0, 3), 20),
+    ("el2", lambda: R.elasticity3d(2, 1.0, 0.3, 3), 30),
+]
+
+
+@pytest.mark.parametrize("name,build,small", CASES)
+@pytest.mark.parametrize("use_xyz", [True, False])
+def test_nested_dissection_is_valid(name, build, small, use_xyz):
+    A, xyz = build()
+    perm, cp, cm = nd(A, xyz if use_xyz else None, small)
+    n = A._nrow
+    assert sorted(perm.tolist()) == list(range(n))
+    comp_new = cm[perm]
+    assert np.all(np.diff(comp_new) >= 0), "components contiguous and in order"
+    assert np.all(cp < np.arange(len(cp))) or np.all((cp == -1) | (cp > np.arange(len(cp))))
+    # separator property: an edge joins two components only if one is an
+    # ancestor of the other in the component tree
+    anc = [set() for _ in cp]
+    for c in range(len(cp)):
+        a = cp[c]
+        while a >= 0:
+            anc[c].add(a)
+            a = cp[a]
+    S = A.to_scipy().tocoo()
+    for u, v in zip(S.row, S.col):
+        cu, cv = cm[u], cm[v]
+        assert cu == cv or cu in anc[cv] or cv in anc[cu], (u, v, cu, cv)
+
+
+@pytest.mark.parametrize("name,build,small", CASES)
+def test_etree_and_column_counts_match_brute_force(name, build, small):
+    A, xyz = build()
+    perm, _, _ = nd(A, xyz, small)
+    par, cc = etree_counts(A, perm)
+    cols = brute_structure(A, perm)
+    n = A._nrow
+    exp_par = np.array([min(c) if c else -1 for c in cols])
+    assert np.array_equal(par, exp_par)
+    # identity must be a postorder of the etree for ND orderings
+    size = np.ones(n, int)
+    for j in range(n):
+        if par[j] >= 0:
+            size[par[j]] += size[j]
+    first = np.arange(n) - size + 1
+    for j in range(n):
+        if par[j] >= 0:
+            assert first[par[j]] <= first[j]
+    exp_cc = np.array([len(c) + 1 for c in cols])
+    assert np.array_equal(cc, exp_cc)
+
+
+def relaxed_supernodes_restated(parent, cc, seps, sep_parent, max_block):
+    """Independent restatement of NestedDissection::BuildRelaxedSupernodes
+    (NestedDissection.cpp:1266-1667) from its documented semantics."""
+    n = len(parent)
+    nchild = np.zeros(n, int)
+    for p in parent:
+        if p >= 0:
+            nchild[p] += 1
+    comp = [False] * len(seps)
+    for s, (size, first, last, nch) in enumerate(seps):
+        if size > max_block and nch > 0:
+            a = s
+            while a >= 0:
+                comp[a] = True
+                a = sep_parent[a]
+    starts, flags = [], []
+    col = 0
+    new_segment = True
+    for s, (size, first, last, nch) in enumerate(seps):
+        if comp[s]:
+            starts.append(first)
+            flags.append(True)
+            new_segment = True
+            continue
+        if new_segment:
+            starts.append(first)
+            flags.append(False)
+            col = first + 1
+            new_segment = False
+        while col <= last:
+            if (parent[col - 1] != col or cc[col - 1] != cc[col] + 1 or nchild[col] > 1):
+                starts.append(col)
+                flags.append(False)
+            col += 1
+    nf = len(starts)
+    starts.append(n)
+    smap = np.empty(n, int)
+    for s in range(nf):
+        smap[starts[s]:starts[s + 1]] = s
+    spar = [smap[parent[starts[s + 1] - 1]] if parent[starts[s + 1] - 1] >= 0 else -1
+            for s in range(nf)]
+    merged = [-1] * nf
+    nscol = [starts[s + 1] - starts[s] for s in range(nf)]
+    zeros = [0] * nf
+    snz = [cc[starts[s]] for s in range(nf)]
+    for s in range(nf - 2, -1, -1):
+        if spar[s] == -1 or flags[s] or flags[s + 1]:
+            continue
+        ss = spar[s]
+        while merged[ss] != -1:
+            ss = merged[ss]
+        sp = ss
+        ss = spar[s]
+        while merged[ss] != -1:
+            nxt = merged[ss]
+            merged[ss] = sp
+            ss = nxt
+        if sp != s + 1:
+            continue
+        n0, n1 = nscol[s], nscol[s + 1]
+        ns = n0 + n1
+        tot = zeros[s + 1]
+        if ns <= 4:
+            merge = True
+        else:
+            newz = n0 * (snz[s + 1] + n0 - snz[s])
+            if newz == 0:
+                merge = True
+            else:
+                xtot = tot + newz
+                xsize = ns * (ns + 1) / 2 + ns * (snz[s + 1] - n1)
+                z = xtot / xsize
+                tot += newz
+                merge = (ns <= 16 and z < 0.8) or (ns <= 48 and z < 0.1) or z < 0.05
+        if merge:
+            zeros[s] = tot
+            merged[s + 1] = s
+            snz[s] = n0 + snz[s + 1]
+            nscol[s] += nscol[s + 1]
+    out = [(starts[s], flags[s]) for s in range(nf) if merged[s] == -1]
+    return out
+
+
+@pytest.mark.parametrize("name,build,mdb", [
+    ("p2d32", lambda: R.poisson2d(32), 15),
+    ("p2d64", lambda: R.poisson2d(64), 31),
+    ("p3d12", lambda: R.poisson3d(12, 1.0, 5), 40),
+    ("p3d16", lambda: R.poisson3d(16, 1.0, 5), 63),
+])
+def test_relaxed_supernodes_match_restatement(name, build, mdb):
+    A, xyz = build()
+    perm, cp, cm = nd(A, xyz, 200)
+    par, cc = etree_counts(A, perm)
+    comp_new = cm[perm]
+    seps = []
+    for c in range(len(cp)):
+        idx = np.nonzero(comp_new == c)[0]
+        seps.append((len(idx), idx[0], idx[-1], int(np.sum(cp == c))))
+    exp = relaxed_supernodes_restated(par, cc, seps, cp, mdb)
+    sym = symbolic(A, xyz, ordering=(perm, cp, cm), max_dense_block=mdb,
+                   max_diagonal_block=10 ** 6)
+    got = list(zip(sym["first"][:-1].tolist(), (sym["compressed"] == 1).tolist()))
+    assert got == [(int(a), bool(b)) for a, b in exp]
+    # compressed supernodes are whole separators
+    for (a, f), (b, _) in zip(got, got[1:] + [(A._nrow, False)]):
+        if f:
+            cset = set(comp_new[a:b])
+            assert len(cset) == 1 and seps[cset.pop()][0] == b - a
+
+
+def test_separator_tree_invariants_and_exact_order():
+    rng = np.random.default_rng(11)
+    L = _lib.lib()
+    for npts, leaf in [(1000, 100), (257, 64), (4096, 256), (400, 400), (3000, 40)]:
+        pts = rng.random((npts, 3))
+        order = np.empty(npts, np.int32)
+        sizes = np.empty(npts, np.int32)
+        nl = C.c_int()
+        _lib.check(L.rsc_separator_tree(npts, _lib.dptr(_lib.as_f64(pts)), leaf,
+                                        _lib.iptr(order), C.byref(nl), _lib.iptr(sizes)))
+        sizes = sizes[: nl.value]
+        assert sorted(order.tolist()) == list(range(npts))
+        assert sizes.sum() == npts and sizes.max() <= leaf
+        assert nl.value & (nl.value - 1) == 0, "power-of-two leaves (full tree)"
+
+        # restatement (distinct coordinates: any correct sort gives this order)
+        def build(idx, maxleaf, out, leaves):
+            if len(idx) <= maxleaf:
+                out.extend(idx)
+                leaves.append(len(idx))
+                return 0
+            P = pts[idx]
+            ext = P.max(0) - P.min(0)
+            ax = int(np.argmax(ext)) if ext.max() > 0 else 0
+            for d in range(3):  # strict '>' picks the lowest axis on ties
+                if ext[d] == ext.max():
+                    ax = d
+                    break
+            idx = [idx[i] for i in np.argsort(P[:, ax], kind="stable")]
+            h = len(idx) // 2
+            Lh, Rh = idx[:h], idx[h:]
+            lm = len(Lh) - 1 if (len(Rh) > maxleaf and len(Lh) <= maxleaf) else maxleaf
+            a = build(Lh, lm, out, leaves)
+            rm = len(Rh) - 1 if (len(Lh) > maxleaf and len(Rh) <= maxleaf) else maxleaf
+            b = build(Rh, rm, out, leaves)
+            balanced[0] &= a == b
+            return 1 + max(a, b)
+        out, leaves, balanced = [], [], [True]
+        build(list(range(npts)), leaf, out, leaves)
+        if balanced[0]:
+            # inside the reference's domain (its full-tree assertion holds):
+            # identical order and leaves
+            assert order.tolist() == out
+            assert sizes.tolist() == leaves
+        else:
+            # the one-level override left unequal heights (the reference
+            # asserts here and would drop blocks when pairing); the deeper
+            # split keeps the tree full
+            assert len(leaves) != nl.value
+
+
+def test_grid_separator_blocks():
+    A, xyz = R.poisson3d(32, 1.0, 1)
+    sym = symbolic(A, xyz, max_dense_block=255, max_diagonal_block=400)
+    first = sym["first"]
+    for j, b in enumerate(sym["diag_blocks"]):
+        size = first[j + 1] - first[j]
+        if size > 400:
+            assert len(b) - 1 >= 2 and ((len(b) - 1) & (len(b) - 2)) == 0
+            assert b[0] == 0 and b[-1] == size and np.all(np.diff(b) <= 400)
+        else:
+            assert len(b) == 0
+
+
+@pytest.mark.parametrize("name,build,mdb", [
+    ("p2d16", lambda: R.poisson2d(16), 7),
+    ("p3d8", lambda: R.poisson3d(8, 1.0, 2), 20),
+    ("el2", lambda: R.elasticity3d(2, 1.0, 0.3, 2), 30),
+])
+def test_row_structure_matches_brute_force(name, build, mdb):
+    A, xyz = build()
+    sym = symbolic(A, xyz, max_dense_block=mdb)
+    cols = brute_structure(A, sym["perm"])
+    first, rp, rows = sym["first"], sym["rows_ptr"], sym["rows"]
+    for j in range(len(first) - 1):
+        a, b = first[j], first[j + 1]
+        exp = set()
+        for k in range(a, b):
+            exp |= {i for i in cols[k] if i >= b}
+        assert sorted(exp) == rows[rp[j]:rp[j + 1]].tolist(), j
+        if rp[j + 1] > rp[j]:
+            assert sym["parent"][j] == np.searchsorted(first, rows[rp[j]], side="right") - 1
+
+
+def test_block_rank_table():
+    # rank = clamp(floor(sqrt(m) log2(sqrt(m)) c) + 5, >= 16, <= min(rows, cols))
+    # (FactorManager.cpp:7283-7338); values at c = 0.4 and 0.5
+    L = _lib.lib()
+    m = [128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+    r04 = [20, 30, 45, 69, 104, 158, 240, 363]
+    r05 = [24, 37, 55, 85, 129, 197, 299, 453]
+    for mm, a, b in zip(m, r04, r05):
+        assert L.rsc_block_rank(mm, 10 ** 6, 0.4, 5, 16, 0) == a
+        assert L.rsc_block_rank(10 ** 6, mm, 0.5, 5, 16, 0) == b
+    assert L.rsc_block_rank(8, 100, 0.4, 5, 16, 0) == 8
+    assert L.rsc_block_rank(100, 100, 0.4, 5, 16, 0) == 18  # 10*log2(10)*0.4 = 13.3
+    assert L.rsc_block_rank(36, 36, 0.4, 5, 16, 0) == 16  # 6.2 + 5 < 16
+    assert L.rsc_block_rank(4096, 4096, 0.4, 5, 16, 64) == 64
+
+
+def test_symbolic_is_deterministic():
+    A, xyz = R.poisson3d(20, 1.0, 9)
+    a = symbolic(A, xyz, max_dense_block=63)
+    b = symbolic(A, xyz, max_dense_block=63)
+    for k in ("perm", "first", "compressed", "rows", "rank"):
+        assert np.array_equal(a[k], b[k])

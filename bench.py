@@ -123,30 +123,47 @@ def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps):
     return out
 
 
-def cpu_leg(A, xyz, B, cfg, eps):
-    """Oracle (FP64 CPU restatement of the reference) on a bounded sample."""
+def cpu_leg(A, xyz, B, cfg, eps, full):
+    """Oracle (FP64 CPU restatement of the reference, one thread) on a bounded
+    sample of the same workload family, scaled to the full workload by the
+    ratio of per-solve algorithmic bytes (the solve is memory-bound on the
+    CPU as on the GPU)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import oracle_for, symbolic
-    N = cfg.get("N", 0)
-    sample_N = min(N, 40) if N else 0
     import rank_structured_cholesky_amd as R
-    if sample_N:
-        As, xs = R.poisson3d(sample_N, cfg["sigma"], 1234)
-        bs = R.uniform_rhs(As._nrow, 5678)
-        desc = f"poisson3d {sample_N}^3 (n={As._nrow}), same method/options"
+    N = cfg.get("N", 0)
+    if N:
+        sN = min(N, 40)
+        As, xs = R.poisson3d(sN, cfg["sigma"], 1234)
+        desc = f"poisson3d {sN}^3 (n={As._nrow}), same options"
     else:
-        As, xs, bs = A, xyz, B[:, 0].copy()
-        desc = f"poisson2d n={A._nrow}"
+        sN = min(cfg["N2"], 128)
+        As, xs = R.poisson2d(sN)
+        desc = f"poisson2d {sN}^2 (n={As._nrow}), same options"
+    bs = R.uniform_rhs(As._nrow, 5678)
     t0 = time.perf_counter()
     sym = symbolic(As, xs, max_dense_block=cfg["max_dense_block"], max_rank=cfg["max_rank"])
     O = oracle_for(As, sym, max_rank=cfg["max_rank"])
     tf = time.perf_counter() - t0
+    reps = 3
     t1 = time.perf_counter()
-    _, info = O.pcg(bs, eps, 10000)
-    ts = time.perf_counter() - t1
-    return dict(value=1.0 / ts, factor_s=tf, solve_s=ts, iterations=info["iterations"],
-                cores=1, sample=desc + "; factor %.1fs, one PCG solve %.2fs (%d it)" % (
-                    tf, ts, info["iterations"]))
+    for _ in range(reps):
+        _, info = O.pcg(bs, eps, 10000)
+    ts = (time.perf_counter() - t1) / reps
+
+    def spmv_bytes(M):
+        return 12.0 * M._nzmax + 4.0 * (M._nrow + 1) + 16.0 * M._nrow
+
+    def solve_bytes(iters, factor_bytes, Mx):
+        return iters * (2.0 * factor_bytes + spmv_bytes(Mx) + 80.0 * Mx._nrow) + spmv_bytes(Mx)
+
+    w_sample = solve_bytes(info["iterations"], 8.0 * sym["factor_elems"], As)
+    w_full = solve_bytes(max(full["iters"]), float(full["info"]["factor_bytes"]), A)
+    value = (1.0 / ts) * (w_sample / w_full)
+    return dict(value=value, cores=1,
+                sample=desc + "; factor %.1f s, one PCG solve %.3f s (%d it, mean of %d); "
+                "scaled by per-solve algorithmic bytes %.3g / %.3g" % (
+                    tf, ts, info["iterations"], reps, w_sample, w_full))
 
 
 def main():
@@ -165,7 +182,7 @@ def main():
     mixed = None
     if not a.no_mixed:
         mixed = gpu_leg(A, xyz, B, cfg, 1, a.steps, a.warmup, a.eps)
-    cpu = None if a.no_cpu else cpu_leg(A, xyz, B, cfg, a.eps)
+    cpu = None if a.no_cpu else cpu_leg(A, xyz, B, cfg, a.eps, g64)
     steps = a.steps
     ms = g64["total"] / steps * 1e3
     value = steps * g64["k"] / g64["total"]

@@ -84,7 +84,10 @@ typedef struct {
   int nd_small;                 /* leaf size of the nested dissection -> 200 */
   /* MI355X build options */
   int factor_precision;         /* RSC_FP64 or RSC_FP32 */
-  int sketch_precision;         /* RSC_FP64, RSC_FP32 or RSC_BF16 */
+  int sketch_precision;         /* sketches run in the factor precision:
+                                   RSC_FP64 (default: same as the factor) or
+                                   RSC_FP32 with an FP32 factor; RSC_BF16 is
+                                   reserved and returns RSC_UNSUPPORTED_MODE */
   int verbose;                  /* 0 = silent */
 } rsc_options;
 

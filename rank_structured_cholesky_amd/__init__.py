@@ -280,8 +280,12 @@ class PCG_SOLVER:
         """Solve A x = b from the initial guess in x (one or more columns)."""
         if not isinstance(M, FactorPreconditioner) or M.handle is None:
             raise TypeError("M must be an initialized FactorPreconditioner")
-        if M._matrix._nrow != self._A._nrow:
-            raise ValueError("preconditioner and matrix sizes differ")
+        Mm, A = M._matrix, self._A
+        if Mm is not A and not (Mm._nrow == A._nrow and np.array_equal(Mm._p, A._p)
+                                and np.array_equal(Mm._i, A._i)
+                                and np.array_equal(Mm._x, A._x)):
+            # the device recurrence runs on the factor's copy of A
+            raise ValueError("the preconditioner was initialized on a different matrix")
         b = np.asarray(b, dtype=np.float64)
         k = 1 if b.ndim == 1 else b.shape[1]
         B = np.asfortranarray(b.reshape(b.shape[0], k))

@@ -86,9 +86,13 @@ rsc_status check_options(const rsc_options& o) {
                 "only CHOLESKY / IN_PLACE / DECOMPOSE_TRANSPOSE / DECOMPOSE_FACTOR is built");
   if (o.factor_precision != RSC_FP64 && o.factor_precision != RSC_FP32)
     return fail(RSC_UNSUPPORTED_MODE, "factor_precision must be RSC_FP64 or RSC_FP32");
-  if (o.sketch_precision != RSC_FP64 && o.sketch_precision != RSC_FP32 &&
-      o.sketch_precision != RSC_BF16)
+  // sketch products run in the factor's precision
+  if (o.sketch_precision == RSC_BF16)
+    return fail(RSC_UNSUPPORTED_MODE, "BF16 sketches are not built");
+  if (o.sketch_precision != RSC_FP64 && o.sketch_precision != RSC_FP32)
     return fail(RSC_UNSUPPORTED_MODE, "bad sketch_precision");
+  if (o.sketch_precision == RSC_FP32 && o.factor_precision != RSC_FP32)
+    return fail(RSC_UNSUPPORTED_MODE, "FP32 sketches need an FP32 factor");
   if (o.max_dense_block < 0 || o.max_diagonal_block < 1 || o.power_iterations < 0 ||
       o.min_rank < 1 || o.max_attempts < 1)
     return fail(RSC_BAD_INPUT, "bad option value");

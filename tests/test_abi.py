@@ -58,7 +58,8 @@ def test_bad_input_and_unsupported_modes():
     for field, val in [("compression_type", R.EXTENDED_VARIABLE),
                        ("extended_factor_type", R.LDL),
                        ("decomposition_block", R.DECOMPOSE_STANDARD),
-                       ("decomposition_type", R.DECOMPOSE_SCHUR)]:
+                       ("decomposition_type", R.DECOMPOSE_SCHUR),
+                       ("sketch_precision", R.RSC_BF16), ("sketch_precision", R.RSC_FP32)]:
         o2 = _lib.default_options(**{field: val})
         st = L.rsc_symbolic_create(A._nrow, _lib.iptr(A._p), _lib.iptr(A._i), None,
                                    C.byref(o2), C.byref(h))

@@ -46,6 +46,21 @@ CASES = {
     "p3d20_hodlr": (lambda: R.poisson3d(20, 1.0, 1234),
                     dict(max_dense_block=63, max_diagonal_block=50,
                          explicit_block_threshold=120), dict(explicit_threshold=120), 1e-8),
+    "el6_c30": (lambda: R.elasticity3d(6, 1.0, 0.3, 2),
+                dict(max_dense_block=30, max_diagonal_block=40), {}, 1e-8),
+}
+
+# Every rank at min(rows, cols): each range finder captures its block
+# exactly and the factor is the exact Cholesky factor of P A P^T.
+FULL = dict(rank_constant=1e4, diagonal_rank_constant=1e4)
+FULL_CASES = {
+    "p3d12_hodlr": (lambda: R.poisson3d(12, 1.0, 3),
+                    dict(max_dense_block=20, max_diagonal_block=40)),
+    "p3d12_explicit": (lambda: R.poisson3d(12, 1.0, 3),
+                       dict(max_dense_block=20, max_diagonal_block=40,
+                            explicit_block_threshold=80)),
+    "el6": (lambda: R.elasticity3d(6, 1.0, 0.3, 2),
+            dict(max_dense_block=30, max_diagonal_block=40)),
 }
 
 
@@ -134,3 +149,17 @@ def test_fp32_factor_apply_close_to_oracle(name):
     x = np.zeros_like(b)
     res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M)
     assert res["true_relres"] < 1e-7
+
+
+@pytest.mark.parametrize("name", list(FULL_CASES))
+def test_full_rank_compression_is_exact(name):
+    import scipy.sparse.linalg as spla
+    build, sopts = FULL_CASES[name]
+    A, xyz = build()
+    M = _factor(A, xyz, **dict(sopts, **FULL))
+    assert M.info()["ncompressed"] > 0
+    b = R.uniform_rhs(A._nrow, 99)
+    x = np.empty_like(b)
+    M.solve(x, b)
+    e = spla.spsolve(A.to_scipy().tocsc(), b)
+    assert relerr(x, e) < 1e-9, relerr(x, e)

@@ -17,7 +17,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host/ordering.cpp", "host/generators.cpp", "host/symbolic.cpp"]
+HOST_SOURCES = ["host/ordering.cpp", "host/generators.cpp", "host/symbolic.cpp",
+                "host/solve_tasks.cpp"]
 HIP_SOURCES = ["device/kernels.hip", "device/factor.hip", "device/solve.hip",
                "device/pcg.hip", "capi.cpp"]
 

@@ -323,6 +323,7 @@ rsc_status rsc_apply_host(rsc_factor* f, int nrhs, const double* B, int ldb,
       RSC_HIP(hipMemcpyAsync(X + (size_t)j * ldx, dX.as<double>() + (size_t)j * n,
                              n * sizeof(double), hipMemcpyDeviceToHost, f->stream));
     RSC_HIP(hipStreamSynchronize(f->stream));
+    f->eng->check_apply(f->stream);
     return RSC_OK;
   });
 }

@@ -85,6 +85,8 @@ class FactorEngine {
   virtual void numeric(const double* Ax_host) = 0;          // may throw IndefiniteError
   virtual void apply(int k, const double* B, int ldb, double* X, int ldx,
                      hipStream_t s) = 0;                      // device pointers
+  // Synchronizes s and throws DeviceError if an apply on it failed.
+  virtual void check_apply(hipStream_t s) = 0;
   virtual int64_t factor_bytes() const = 0;
   virtual int64_t workspace_bytes() const = 0;
   virtual int pivot_replacements() const = 0;

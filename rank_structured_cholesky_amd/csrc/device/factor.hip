@@ -87,6 +87,7 @@ class Engine final : public FactorEngine {
   void numeric(const double* Ax) override;
   void apply(int k, const double* B, int ldb, double* X, int ldx,
              hipStream_t s) override;
+  void check_apply(hipStream_t s) override;
   int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
   int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
   int pivot_replacements() const override { return pivot_replacements_; }
@@ -104,17 +105,15 @@ class Engine final : public FactorEngine {
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
   DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_;
-  struct SolvePhase {
-    int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks
-    int first, count;
-  };
   std::vector<std::vector<SolvePhase>> fwd_, bwd_;
   int64_t tv_stride_ = 0, ntasks_ = 0, part_stride_ = 1;
   DBuf dpart_;
   std::vector<int64_t> fv_off_;
-  int64_t fvoff(int j) const { return fv_off_[j]; }
-  void build_solve_tasks(const std::vector<int64_t>& tv_node,
-                         const std::vector<std::vector<int64_t>>& tv_h);
+  // dataflow apply (one persistent launch per application)
+  DBuf ddf_, ddfgv_, dgwoff_, dgw_, dneed_, dcnt_, dapst_;
+  int df_ntasks_ = 0, df_ngroups_ = 0, df_grid_ = 1;
+  unsigned long long df_timeout_ = 0;
+  bool use_dataflow_ = false;
   int64_t fv_stride_ = 0;
   int fv_k_ = 0;
   std::vector<std::vector<int>> levnode_off_;  // per level: offset in dlevnodes
@@ -1008,60 +1007,51 @@ void Engine<T>::setup_static() {
   dlowp_.upload(L.p, st_);
   da_dst_.upload(a_dst, st_);
   da_src_.upload(L.src, st_);
-  // children, relmaps, rows
-  std::vector<int> children, relmap, rows;
-  std::vector<int> child_off(ns), relmap_off(ns);
-  std::vector<int64_t> rows_off(ns);
-  for (int j = 0; j < ns; ++j) {
-    const Supernode& s = P.nodes[j];
-    child_off[j] = (int)children.size();
-    children.insert(children.end(), s.children.begin(), s.children.end());
-    relmap_off[j] = (int)relmap.size();
-    relmap.insert(relmap.end(), s.relmap.begin(), s.relmap.end());
-    if (s.relmap.empty()) relmap.resize(relmap.size() + s.rows.size(), 0);
-    rows_off[j] = (int64_t)rows.size();
-    rows.insert(rows.end(), s.rows.begin(), s.rows.end());
-  }
-  dchildren_.upload(children, st_);
-  drelmap_.upload(relmap, st_);
-  drows_.upload(rows, st_);
+  // solve: vector layout, level-scheduled and dataflow task lists
+  SolveLayout SL;
+  build_solve_layout(P, SL);
+  dchildren_.upload(SL.children, st_);
+  drelmap_.upload(SL.relmap, st_);
+  drows_.upload(SL.rows, st_);
   std::vector<int> permv(P.perm);
   dperm_.upload(permv, st_);
-  // solve: per-supernode vector descriptors and per-level task phases
-  std::vector<dev::VecNode> vn(ns);
-  std::vector<int64_t> tv_node(ns, -1);
-  fv_off_.assign(ns, 0);
-  std::vector<std::vector<int64_t>> tv_h(ns);
-  int64_t fvo = 0, tvo = 0;
-  for (int j = 0; j < ns; ++j) {
-    const Supernode& s = P.nodes[j];
-    dev::VecNode& d = vn[j];
-    d.first = s.first;
-    d.ncols = s.ncols;
-    d.R = s.R();
-    d.nchild = (int)s.children.size();
-    d.child_off = child_off[j];
-    d.pad = 0;
-    d.fv_off = fvo;
-    fv_off_[j] = fvo;
-    d.rows_off = rows_off[j];
-    d.relmap_off = relmap_off[j];
-    fvo += (int64_t)s.ncols + s.R();
-    if (s.compress) {
-      tv_node[j] = tvo;
-      tvo += s.rank;
-      tv_h[j].assign(s.hnodes.size(), -1);
-      for (size_t q = 0; q < s.hnodes.size(); ++q)
-        if (!s.hnodes[q].leaf()) {
-          tv_h[j][q] = tvo;
-          tvo += s.hnodes[q].rank;
-        }
-    }
+  fv_off_ = SL.fv_off;
+  fv_stride_ = SL.fv_stride;
+  tv_stride_ = SL.tv_stride;
+  dvnodes_.upload(SL.vn, st_);
+  {
+    LevelTasks LT;
+    build_level_tasks(P, SL, LT);
+    fwd_.swap(LT.fwd);
+    bwd_.swap(LT.bwd);
+    dgv_.upload(LT.gv, st_);
+    ntasks_ = (int64_t)LT.gv.size();
+    DataflowTasks DT;
+    build_dataflow_tasks(P, SL, DT);
+    df_ntasks_ = (int)DT.tasks.size();
+    df_ngroups_ = (int)DT.need.size();
+    ddf_.upload(DT.tasks, st_);
+    ddfgv_.upload(DT.gv, st_);
+    dgwoff_.upload(DT.gwoff, st_);
+    dgw_.upload(DT.gw, st_);
+    dneed_.upload(DT.need, st_);
+    dcnt_.alloc((size_t)(df_ngroups_ + 1) * sizeof(unsigned));
+    part_stride_ = std::max<int64_t>(std::max(LT.slots, DT.slots) * 32, 1);
   }
-  fv_stride_ = std::max<int64_t>(fvo, 1);
-  tv_stride_ = std::max<int64_t>(tvo, 1);
-  dvnodes_.upload(vn, st_);
-  build_solve_tasks(tv_node, tv_h);
+  {
+    const char* df = std::getenv("RSC_APPLY_DATAFLOW");
+    use_dataflow_ = df && df[0] == '1';
+    int dev = 0, cus = 1, khz = 0;
+    RSC_HIP(hipGetDevice(&dev));
+    RSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0)
+      khz = 100000;
+    df_timeout_ = (unsigned long long)khz * 1000ull * 4ull;  // 4 s
+    df_grid_ = std::max(1, std::min(df_ntasks_, cus * dev::dataflow_blocks_per_cu<T>()));
+    std::vector<int> z = {0, 0};
+    dapst_.upload(z, st_);
+  }
   std::vector<int> levnodes;
   lev_off_.clear();
   for (auto& lv : P.levels) {
@@ -1296,6 +1286,31 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   }
   dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(),
                   dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_};
+  if (use_dataflow_) {
+    dev::DfArgs a{};
+    a.tasks = ddf_.as<dev::DfTask>();
+    a.ntasks = df_ntasks_;
+    a.ngroups = df_ngroups_;
+    a.gv = ddfgv_.as<dev::GvTask>();
+    a.gwait_off = dgwoff_.as<int>();
+    a.gwait = dgw_.as<int>();
+    a.need = dneed_.as<unsigned>();
+    a.cnt = dcnt_.as<unsigned>();
+    a.status = dapst_.as<int>();
+    a.timeout_ticks = df_timeout_;
+    a.nodes = dvnodes_.as<dev::VecNode>();
+    a.children = dchildren_.as<int>();
+    a.relmap = drelmap_.as<int>();
+    a.rows = drows_.as<int>();
+    a.perm = dperm_.as<int>();
+    a.B = B;
+    a.ldb = ldb;
+    a.X = X;
+    a.ldx = ldx;
+    dev::launch_df_reset(dcnt_.as<unsigned>(), df_ngroups_ + 1, s);
+    dev::launch_apply_dataflow<T>(F(), a, vb, k, df_grid_, s);
+    return;
+  }
   dev::launch_gather(B, ldb, vb.w, dperm_.as<int>(), n, k, s);
   const int nlev = (int)plan_.levels.size();
   auto run = [&](const std::vector<SolvePhase>& ph, int lv) {
@@ -1320,233 +1335,18 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   dev::launch_scatter(vb.w, X, ldx, dperm_.as<int>(), n, k, s);
 }
 
-// Task phases of the level-scheduled solves (see solve.hip).
 template <typename T>
-void Engine<T>::build_solve_tasks(const std::vector<int64_t>& tv_node,
-                                  const std::vector<std::vector<int64_t>>& tv_h) {
-  const Plan& P = plan_;
-  std::vector<dev::GvTask> gv;
-  fwd_.assign(P.levels.size(), {});
-  bwd_.assign(P.levels.size(), {});
-  const int W = 0, FV = 1, TV = 2;
-  // COL tasks: 32 columns; more than CHUNK rows are split into row chunks
-  // writing partial sums, reduced by a second launch (`red`).
-  const int CHUNK = 512;
-  std::vector<dev::GvTask> red;
-  int64_t slot = 0;
-  auto col = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
-                 int ys, int64_t yo, bool upper) {
-    for (int j0 = 0; j0 < cols; j0 += 32) {
-      const int rr = upper ? std::min(rows, j0 + 32) : rows;
-      if (rr <= 0) continue;
-      dev::GvTask t{};
-      t.ld = ld;
-      t.cols = std::min(32, cols - j0);
-      t.kind = 1;
-      t.op = op;
-      t.xsel = xs;
-      t.ysel = ys;
-      t.yoff = yo + j0;
-      if (rr <= CHUNK) {
-        t.mat = mat + (int64_t)j0 * ld;
-        t.rows = rr;
-        t.xoff = xo;
-        t.part = -1;
-        gv.push_back(t);
-        continue;
-      }
-      const int nsub = (rr + CHUNK - 1) / CHUNK;
-      for (int s2 = 0; s2 < nsub; ++s2) {
-        dev::GvTask u = t;
-        u.mat = mat + (int64_t)j0 * ld + (int64_t)s2 * CHUNK;
-        u.rows = std::min(CHUNK, rr - s2 * CHUNK);
-        u.xoff = xo + (int64_t)s2 * CHUNK;
-        u.part = (int)(slot + s2);
-        gv.push_back(u);
-      }
-      dev::GvTask rd{};
-      rd.kind = 2;
-      rd.part = (int)slot;
-      rd.rows = nsub;
-      rd.cols = t.cols;
-      rd.op = op;
-      rd.ysel = ys;
-      rd.yoff = yo + j0;
-      red.push_back(rd);
-      slot += nsub;
-    }
-  };
-  // ROW tasks: 64 rows, all columns
-  auto row = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
-                 int ys, int64_t yo, bool upper) {
-    for (int i0 = 0; i0 < rows; i0 += 64) {
-      dev::GvTask t{};
-      t.mat = mat + i0 + (upper ? (int64_t)i0 * ld : 0);
-      t.ld = ld;
-      t.rows = std::min(64, rows - i0);
-      t.cols = upper ? cols - i0 : cols;
-      t.kind = 0;
-      t.op = op;
-      t.xsel = xs;
-      t.ysel = ys;
-      t.part = -1;
-      t.xoff = xo + (upper ? i0 : 0);
-      t.yoff = yo + i0;
-      gv.push_back(t);
-    }
-  };
-  // a phase = small tasks (kind 2 launch) + large tasks (kind 3 launch),
-  // then the partial-sum reductions of its split column tasks
-  auto is_big = [](const dev::GvTask& t) {
-    return t.kind == 0 ? t.cols > 64 : t.rows > 128;
-  };
-  auto phase = [&](std::vector<SolvePhase>& out, size_t start) {
-    if (gv.size() > start) {
-      std::stable_partition(gv.begin() + start, gv.end(),
-                            [&](const dev::GvTask& t) { return !is_big(t); });
-      size_t mid = start;
-      while (mid < gv.size() && !is_big(gv[mid])) ++mid;
-      if (mid > start) out.push_back({2, (int)start, (int)(mid - start)});
-      if (gv.size() > mid) out.push_back({3, (int)mid, (int)(gv.size() - mid)});
-    }
-    if (!red.empty()) {
-      const size_t r0 = gv.size();
-      gv.insert(gv.end(), red.begin(), red.end());
-      red.clear();
-      out.push_back({2, (int)r0, (int)(gv.size() - r0)});
-    }
-  };
-  for (size_t lv = 0; lv < P.levels.size(); ++lv) {
-    const std::vector<int>& nodes = P.levels[lv];
-    int maxh = 0;
-    for (int j : nodes)
-      for (auto& h : P.nodes[j].hnodes) maxh = std::max(maxh, h.height);
-    // ---- forward
-    std::vector<SolvePhase>& F = fwd_[lv];
-    F.push_back({0, 0, 0});
-    size_t st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      const int64_t fo = 0;
-      (void)fo;
-      if (!s.compress) {
-        col(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
-      } else {
-        for (auto& h : s.hnodes)
-          if (h.leaf())
-            col(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
-                s.first + h.r0, true);
-      }
-    }
-    phase(F, st);
-    for (int hh = 1; hh <= maxh; ++hh) {
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& A = s.hnodes[h.left];
-          col(h.off_U, A.r1 - A.r0, A.r1 - A.r0, h.rank, 0, W, s.first + A.r0, TV, tv_h[j][q],
-              false);
-        }
-      }
-      phase(F, st);
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& B = s.hnodes[h.right];
-          row(h.off_W, B.r1 - B.r0, B.r1 - B.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + B.r0,
-              false);
-        }
-      }
-      phase(F, st);
-    }
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.compress && s.R() > 0)
-        col(s.off_U, s.ncols, s.ncols, s.rank, 0, W, s.first, TV, tv_node[j], false);
-    }
-    phase(F, st);
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.R() == 0) continue;
-      if (!s.compress)
-        row(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, W, s.first, FV,
-            fvoff(j) + s.ncols, false);
-      else
-        row(s.off_V, s.R(), s.R(), s.rank, 1, TV, tv_node[j], FV, fvoff(j) + s.ncols, false);
-    }
-    phase(F, st);
-    // ---- backward
-    std::vector<SolvePhase>& Bk = bwd_[lv];
-    Bk.push_back({1, 0, 0});
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.R() == 0) continue;
-      if (!s.compress)
-        col(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, FV, fvoff(j) + s.ncols,
-            FV, fvoff(j), false);
-      else
-        col(s.off_V, s.R(), s.R(), s.rank, 0, FV, fvoff(j) + s.ncols, TV, tv_node[j], false);
-    }
-    phase(Bk, st);
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.compress && s.R() > 0)
-        row(s.off_U, s.ncols, s.ncols, s.rank, 1, TV, tv_node[j], FV, fvoff(j), false);
-    }
-    phase(Bk, st);
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (!s.compress) {
-        row(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
-      } else {
-        for (auto& h : s.hnodes)
-          if (h.leaf())
-            row(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
-                s.first + h.r0, true);
-      }
-    }
-    phase(Bk, st);
-    for (int hh = 1; hh <= maxh; ++hh) {
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& B = s.hnodes[h.right];
-          col(h.off_V, B.r1 - B.r0, B.r1 - B.r0, h.rank, 0, W, s.first + B.r0, TV, tv_h[j][q],
-              false);
-        }
-      }
-      phase(Bk, st);
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& A = s.hnodes[h.left];
-          row(h.off_X, A.r1 - A.r0, A.r1 - A.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + A.r0,
-              false);
-        }
-      }
-      phase(Bk, st);
-    }
+void Engine<T>::check_apply(hipStream_t s) {
+  if (!use_dataflow_) return;
+  int st[2] = {0, 0};
+  RSC_HIP(hipMemcpyAsync(st, dapst_.p, sizeof(st), hipMemcpyDeviceToHost, s));
+  RSC_HIP(hipStreamSynchronize(s));
+  if (st[0] != 0 || st[1] != 0) {
+    std::vector<int> z = {0, 0};
+    dapst_.upload(z, s);
+    RSC_HIP(hipStreamSynchronize(s));
+    throw DeviceError("preconditioner apply: a dependency wait timed out");
   }
-  dgv_.upload(gv, st_);
-  ntasks_ = (int64_t)gv.size();
-  part_stride_ = std::max<int64_t>(slot * 32, 1);
 }
 
 }  // namespace

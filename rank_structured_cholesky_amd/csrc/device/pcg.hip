@@ -244,6 +244,7 @@ void pcg_solve_device(FactorEngine& M, const DevCsr& A, int k, const double* B,
                          hipMemcpyDeviceToHost, s));
   RSC_HIP(hipStreamSynchronize(s));
   RSC_HIP(hipGetLastError());
+  M.check_apply(s);
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (res)
     for (int c = 0; c < k; ++c) {

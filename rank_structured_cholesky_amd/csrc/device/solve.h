@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "factor.h"
+#include "../host/solve_tasks.h"
 
 namespace rsc {
 namespace dev {
@@ -16,33 +17,6 @@ void launch_gather(const double* b, int ldb, double* w, const int* perm, int n,
 // x[perm[i] + j*ldx] = w[i + j*n]
 void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
                     int k, hipStream_t s);
-
-// One matrix-vector task of a solve phase.  ROW: y[i] (op)= sum_j M[i,j] x[j]
-// for i < rows (one thread per row).  COL: y[j] (op)= sum_i M[i,j] x[i] for
-// j < cols (waves reduce over rows).  M is the factor slice at `mat`
-// (leading dimension ld); x / y live in w (stride n), the front vectors fv
-// (stride fvs) or the rank-sized buffers tv (stride tvs): sel 0 / 1 / 2.
-// A COL task over many rows is split into row chunks that write partial sums
-// (part >= 0: slot of 32 partials in the partial buffer); a reduce task then
-// adds a group's partials in fixed order and applies op to y.
-struct GvTask {
-  int64_t mat;
-  int ld, rows, cols;
-  int kind;  // 0 ROW, 1 COL, 2 REDUCE (part = first slot, rows = slot count)
-  int op;    // 0 set, 1 subtract
-  int xsel, ysel;
-  int part;
-  int64_t xoff, yoff;
-};
-
-// Per-supernode vector gathers of a solve level.
-struct VecNode {
-  int first, ncols, R, nchild;
-  int child_off, pad;
-  int64_t fv_off;    // front vector offset (per RHS column)
-  int64_t rows_off;  // row list offset
-  int64_t relmap_off;
-};
 
 struct VecBufs {
   double* w;
@@ -62,6 +36,37 @@ void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
 template <typename T>
 void launch_gemv_tasks(const T* factor, const GvTask* tasks, int count, VecBufs b,
                        int k, bool big, hipStream_t s);
+
+// ---- dataflow apply (tasks: DfTask) --------------------------------------
+struct DfArgs {
+  const DfTask* tasks;
+  int ntasks, ngroups;
+  const GvTask* gv;
+  const int* gwait_off;  // ngroups + 1
+  const int* gwait;      // groups waited on, per group
+  const unsigned* need;  // tasks per group
+  unsigned* cnt;         // ngroups completion counters, then the ticket counter
+  int* status;           // [0] = 1 after a wait timed out, [1] = abort request
+  unsigned long long timeout_ticks;  // wall-clock ticks before a wait gives up
+  const VecNode* nodes;
+  const int* children;
+  const int* relmap;
+  const int* rows;
+  const int* perm;
+  const double* B;
+  int64_t ldb;
+  double* X;
+  int64_t ldx;
+};
+
+// Zero the counters (ngroups + 1 entries) before a launch.
+void launch_df_reset(unsigned* cnt, int count, hipStream_t s);
+template <typename T>
+void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, int grid,
+                           hipStream_t s);
+// Workgroups of the dataflow kernel resident per compute unit.
+template <typename T>
+int dataflow_blocks_per_cu();
 
 // CSR (= CSC of the symmetric matrix) y = A x for k columns.
 void launch_spmv(int n, const int* p, const int* idx, const double* val,

@@ -17,6 +17,8 @@
 // on load).
 #include "solve.h"
 
+#include <algorithm>
+
 namespace rsc {
 namespace dev {
 
@@ -58,22 +60,74 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
 // in the chunk: deterministic, no atomics.
 constexpr int VCH = 1024;
 
-__global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict__ nodes,
-                                                      const int* __restrict__ lnodes,
-                                                      const int* __restrict__ children,
-                                                      const int* __restrict__ relmap,
-                                                      VecBufs b, int k) {
-  const VecNode S = nodes[lnodes[blockIdx.x]];
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ inline double* vsel(const VecBufs& b, int sel, int col) {
+  if (sel == 0) return b.w + (int64_t)col * b.n;
+  if (sel == 1) return b.fv + (int64_t)col * b.fvs;
+  return b.tv + (int64_t)col * b.tvs;
+}
+
+// Task descriptors are the same for every lane of a workgroup; copying them
+// through readfirstlane puts them in scalar registers, so the compiler sees
+// uniform control flow around the workgroup barriers of the task bodies.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uni64(int64_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffff));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline GvTask uniform_task(const GvTask& t) {
+  GvTask u;
+  u.mat = uni64(t.mat);
+  u.ld = uni(t.ld);
+  u.rows = uni(t.rows);
+  u.cols = uni(t.cols);
+  u.kind = uni(t.kind);
+  u.op = uni(t.op);
+  u.xsel = uni(t.xsel);
+  u.ysel = uni(t.ysel);
+  u.part = uni(t.part);
+  u.xoff = uni64(t.xoff);
+  u.yoff = uni64(t.yoff);
+  return u;
+}
+__device__ inline VecNode uniform_node(const VecNode& n) {
+  VecNode u;
+  u.first = uni(n.first);
+  u.ncols = uni(n.ncols);
+  u.R = uni(n.R);
+  u.nchild = uni(n.nchild);
+  u.child_off = uni(n.child_off);
+  u.pad = 0;
+  u.fv_off = uni64(n.fv_off);
+  u.rows_off = uni64(n.rows_off);
+  u.relmap_off = uni64(n.relmap_off);
+  return u;
+}
+
+// Forward assembly of one 1024-entry chunk of node S's front vector:
+// fv_c = source[perm[cols]] (perm may be null: source indexed directly),
+// fv_R = 0, then the children's update vectors added through their relmaps.
+__device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
+                               const int* __restrict__ children,
+                               const int* __restrict__ relmap, const VecBufs& b, int k,
+                               const double* __restrict__ src, int64_t lds,
+                               const int* __restrict__ perm) {
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int p0 = blockIdx.y * VCH, p1 = min(p0 + VCH, c + R);
+  const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
   if (p0 >= c + R) return;
   for (int col = 0; col < k; ++col) {
     double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
-    const double* wc = b.w + (int64_t)col * b.n;
-    for (int i = p0 + tid; i < p1; i += 256) f[i] = i < c ? wc[S.first + i] : 0.0;
+    const double* sc = src + (int64_t)col * lds;
+    for (int i = p0 + tid; i < p1; i += 256)
+      f[i] = i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0;
     __syncthreads();
     for (int q = 0; q < S.nchild; ++q) {
-      const VecNode C = nodes[children[S.child_off + q]];
+      const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
       const int* rm = relmap + C.relmap_off;
       const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
       // one predicated pass over the child's rows (no dependent searches)
@@ -86,13 +140,11 @@ __global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict_
   }
 }
 
-__global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ nodes,
-                                                    const int* __restrict__ lnodes,
-                                                    const int* __restrict__ rows,
-                                                    VecBufs b, int k) {
-  const VecNode S = nodes[lnodes[blockIdx.x]];
+// Backward gather of one chunk: fv_c = w[cols], fv_R[a] = w[rows[a]].
+__device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
+                             const VecBufs& b, int k) {
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int p0 = blockIdx.y * VCH, p1 = min(p0 + VCH, c + R);
+  const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
   if (p0 >= c + R) return;
   const int* rw = rows + S.rows_off;
   for (int col = 0; col < k; ++col) {
@@ -100,6 +152,23 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ 
     const double* wc = b.w + (int64_t)col * b.n;
     for (int i = p0 + tid; i < p1; i += 256) f[i] = i < c ? wc[S.first + i] : wc[rw[i - c]];
   }
+}
+
+__global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict__ nodes,
+                                                      const int* __restrict__ lnodes,
+                                                      const int* __restrict__ children,
+                                                      const int* __restrict__ relmap,
+                                                      VecBufs b, int k) {
+  const VecNode S = nodes[lnodes[blockIdx.x]];
+  assemble_chunk(S, blockIdx.y, nodes, children, relmap, b, k, b.w, b.n, nullptr);
+}
+
+__global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ nodes,
+                                                    const int* __restrict__ lnodes,
+                                                    const int* __restrict__ rows,
+                                                    VecBufs b, int k) {
+  const VecNode S = nodes[lnodes[blockIdx.x]];
+  gather_chunk(S, blockIdx.y, rows, b, k);
 }
 
 void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
@@ -119,17 +188,6 @@ void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
 }
 
 // ---- matrix-vector tasks ---------------------------------------------------
-
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-__device__ inline double* vsel(const VecBufs& b, int sel, int col) {
-  if (sel == 0) return b.w + (int64_t)col * b.n;
-  if (sel == 1) return b.fv + (int64_t)col * b.fvs;
-  return b.tv + (int64_t)col * b.tvs;
-}
 
 // Small tasks: plain loops, few registers (many workgroups per CU).
 template <typename T>
@@ -187,11 +245,8 @@ __global__ __launch_bounds__(256) void k_gemv_small(const T* __restrict__ F,
 
 // Large tasks: every load of a block issued before its first use.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_tasks(const T* __restrict__ F,
-                                                    const GvTask* __restrict__ tasks,
-                                                    VecBufs b, int k) {
-  __shared__ double red[4][64];
-  const GvTask t = tasks[blockIdx.x];
+__device__ void gemv_task(const T* __restrict__ F, const GvTask& t, const VecBufs& b, int k,
+                          double (*red)[64]) {
   const T* __restrict__ M = F + t.mat;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int col = 0; col < k; ++col) {
@@ -282,6 +337,15 @@ __global__ __launch_bounds__(256) void k_gemv_tasks(const T* __restrict__ F,
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_tasks(const T* __restrict__ F,
+                                                    const GvTask* __restrict__ tasks,
+                                                    VecBufs b, int k) {
+  __shared__ double red[4][64];
+  const GvTask t = tasks[blockIdx.x];
+  gemv_task<T>(F, t, b, k, red);
+}
+
+template <typename T>
 void launch_gemv_tasks(const T* factor, const GvTask* tasks, int count, VecBufs b,
                        int k, bool big, hipStream_t s) {
   if (count <= 0) return;
@@ -293,6 +357,215 @@ template void launch_gemv_tasks<double>(const double*, const GvTask*, int, VecBu
                                         bool, hipStream_t);
 template void launch_gemv_tasks<float>(const float*, const GvTask*, int, VecBufs, int,
                                        bool, hipStream_t);
+
+// ---- dataflow apply -------------------------------------------------------
+
+// Matrix-vector task with few registers (about 64 VGPRs, so several
+// workgroups stay resident per compute unit while others wait on their
+// dependencies).  ROW: lane = row, wave g takes columns g, g+4, ..., eight
+// per batch; COL: wave w owns columns 8w..8w+7, lane l rows l + 64m, two
+// row blocks per batch.
+template <typename T>
+__device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBufs& b, int k,
+                          double (*red)[64]) {
+  const GvTask t = uniform_task(t0);
+  const T* __restrict__ M = F + t.mat;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int col = 0; col < k; ++col) {
+    double* y = vsel(b, t.ysel, col) + t.yoff;
+    if (t.kind == 2) {
+      const double* pp = b.part + (int64_t)col * b.ps + (int64_t)t.part * 32;
+      if (tid < t.cols) {
+        double acc = 0;
+        for (int s2 = 0; s2 < t.rows; ++s2) acc += pp[(int64_t)s2 * 32 + tid];
+        if (t.op) y[tid] -= acc;
+        else y[tid] = acc;
+      }
+      continue;
+    }
+    const double* x = vsel(b, t.xsel, col) + t.xoff;
+    if (t.kind == 0) {
+      const bool act = lane < t.rows;
+      const T* rp = M + (act ? lane : 0);
+      const int64_t ld4 = 4 * (int64_t)t.ld;
+      double acc0 = 0, acc1 = 0;
+      int j = wave;
+      for (; j + 28 < t.cols; j += 32) {
+        const T* p = rp + (int64_t)j * t.ld;
+        double m[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          m[u] = (double)p[u * ld4];
+          xv[u] = x[j + 4 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          acc0 += m[u] * xv[u];
+          acc1 += m[u + 1] * xv[u + 1];
+        }
+      }
+      for (; j < t.cols; j += 4) acc0 += (double)rp[(int64_t)j * t.ld] * x[j];
+      red[wave][lane] = act ? acc0 + acc1 : 0.0;
+      __syncthreads();
+      if (wave == 0 && act) {
+        const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        if (t.op) y[lane] -= v;
+        else y[lane] = v;
+      }
+      __syncthreads();
+    } else {
+      const int q0 = wave * 8;
+      const int nq = min(8, t.cols - q0);
+      double acc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] = 0;
+      if (nq > 0) {
+        const T* cbase = M + (int64_t)q0 * t.ld;
+        for (int i = lane; i < t.rows; i += 128) {
+          const int i2 = i + 64;
+          const bool ok2 = i2 < t.rows;
+          const double x1 = x[i], x2 = ok2 ? x[i2] : 0.0;
+          double m1[8], m2[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
+            m1[u] = (double)cp[i];
+            m2[u] = ok2 ? (double)cp[i2] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc[u] += m1[u] * x1 + m2[u] * x2;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double v = wave_sum(acc[u]);
+        if (lane == 0 && u < nq) {
+          if (t.part >= 0) b.part[(int64_t)col * b.ps + (int64_t)t.part * 32 + q0 + u] = v;
+          else if (t.op) y[q0 + u] -= v;
+          else y[q0 + u] = v;
+        }
+      }
+    }
+  }
+}
+
+constexpr int DF_SCATTER_CHUNK = 4096;
+
+// Thread 0 waits until group g has completed need tasks.  Returns false when
+// the wait gave up (timeout here or in another workgroup): the kernel then
+// runs to its end without further waiting and the host reports the failure.
+__device__ bool df_wait(const DfArgs& a, int g) {
+  const unsigned need = a.need[g];
+  if (__hip_atomic_load(a.cnt + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need)
+    return true;
+  const unsigned long long t0 = wall_clock64();
+  while (__hip_atomic_load(a.cnt + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+    if (__hip_atomic_load(a.status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return false;
+    if (wall_clock64() - t0 > a.timeout_ticks) {
+      __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
+// One tid == 0 region per loop iteration, with a barrier on both sides: the
+// completion signal and the next ticket are taken together, so the compiler
+// cannot merge a tid == 0 region of one iteration with one of the next (a
+// merged region lets the other lanes run the loop without thread 0).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArgs a,
+                                                  VecBufs b, int k) {
+  __shared__ double red[4][64];
+  __shared__ int s_ticket;
+  __shared__ int s_ok;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_ok = 1;
+    s_ticket = (int)__hip_atomic_fetch_add(a.cnt + a.ngroups, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  for (;;) {
+    const int ticket = uni(s_ticket);
+    if (ticket >= a.ntasks) break;
+    const DfTask& tk = a.tasks[ticket];
+    const int type = uni(tk.type), ta = uni(tk.a), tb = uni(tk.b);
+    const int group = uni(tk.group), group2 = uni(tk.group2);
+    const int w0 = uni(a.gwait_off[group]), w1 = uni(a.gwait_off[group + 1]);
+    if (w1 > w0) {
+      if (tid == 0) {
+        bool ok = s_ok != 0;
+        for (int q = w0; ok && q < w1; ++q) ok = df_wait(a, a.gwait[q]);
+        s_ok = ok ? 1 : 0;
+      }
+      __syncthreads();
+    }
+    // every wave: later loads observe what the producers released
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (type == 0) {
+      assemble_chunk(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b, k, a.B,
+                     a.ldb, a.perm);
+    } else if (type == 1) {
+      gather_chunk(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
+    } else if (type == 2) {
+      gemv_lite<T>(F, a.gv[ta], b, k, red);
+    } else {
+      const int i0 = ta * DF_SCATTER_CHUNK;
+      const int i1 = min(i0 + DF_SCATTER_CHUNK, (int)b.n);
+      for (int col = 0; col < k; ++col)
+        for (int i = i0 + tid; i < i1; i += 256)
+          a.X[a.perm[i] + col * a.ldx] = b.w[i + col * b.n];
+    }
+    // every wave: its stores are written back before the completion count
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_fetch_add(a.cnt + group, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (group2 >= 0)
+        __hip_atomic_fetch_add(a.cnt + group2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ticket = (int)__hip_atomic_fetch_add(a.cnt + a.ngroups, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_df_reset(unsigned* cnt, int count) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    cnt[i] = 0u;
+}
+
+void launch_df_reset(unsigned* cnt, int count, hipStream_t s) {
+  const int blocks = std::min(1024, (count + 255) / 256);
+  k_df_reset<<<blocks, 256, 0, s>>>(cnt, count);
+}
+
+template <typename T>
+void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, int grid,
+                           hipStream_t s) {
+  if (a.ntasks <= 0) return;
+  k_apply_df<T><<<grid, 256, 0, s>>>(factor, a, b, k);
+}
+
+template <typename T>
+int dataflow_blocks_per_cu() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_apply_df<T>, 256, 0) != hipSuccess ||
+      nb < 1)
+    nb = 1;
+  return nb;
+}
+
+template void launch_apply_dataflow<double>(const double*, const DfArgs&, VecBufs, int, int,
+                                            hipStream_t);
+template void launch_apply_dataflow<float>(const float*, const DfArgs&, VecBufs, int, int,
+                                           hipStream_t);
+template int dataflow_blocks_per_cu<double>();
+template int dataflow_blocks_per_cu<float>();
 
 // ---- SpMV ------------------------------------------------------------------
 // LPR lanes cooperate on one row; the row's partial sums are combined with

@@ -1,0 +1,99 @@
+// Task lists of the preconditioner apply M^{-1} = P^T L^{-T} L^{-1} P, built
+// on the host from the symbolic plan (FactorManager::forwardSolve /
+// backwardSolve, FactorManager.cpp:6245-6341; Supernode::forwardSolve /
+// backwardSolve, Supernode.cpp:4571-5250).  The structs in namespace dev are
+// read by the device kernels as they are laid out here.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "rsc_host.h"
+
+namespace rsc {
+namespace dev {
+
+// One matrix-vector task of a solve phase.  ROW: y[i] (op)= sum_j M[i,j] x[j]
+// for i < rows (one thread per row).  COL: y[j] (op)= sum_i M[i,j] x[i] for
+// j < cols (waves reduce over rows).  M is the factor slice at `mat`
+// (leading dimension ld); x / y live in w (stride n), the front vectors fv
+// (stride fvs) or the rank-sized buffers tv (stride tvs): sel 0 / 1 / 2.
+// A COL task over many rows is split into row chunks that write partial sums
+// (part >= 0: slot of 32 partials in the partial buffer); a reduce task then
+// adds a group's partials in fixed order and applies op to y.
+struct GvTask {
+  int64_t mat;
+  int ld, rows, cols;
+  int kind;  // 0 ROW, 1 COL, 2 REDUCE (part = first slot, rows = slot count)
+  int op;    // 0 set, 1 subtract
+  int xsel, ysel;
+  int part;
+  int64_t xoff, yoff;
+};
+
+// Per-supernode vector gathers of a solve level.
+struct VecNode {
+  int first, ncols, R, nchild;
+  int child_off, pad;
+  int64_t fv_off;    // front vector offset (per RHS column)
+  int64_t rows_off;  // row list offset
+  int64_t relmap_off;
+};
+
+
+// The whole application M^{-1} B as ONE persistent launch.  Tasks are listed
+// in a topological order and handed out through a ticket counter; every task
+// belongs to a group (the tasks of one stage of one supernode: its
+// assembly, one matrix-vector phase, one reduction) and waits until each
+// group on its wait list has completed all of its tasks.  A workgroup only
+// ever waits on groups whose tasks hold smaller tickets, which are already
+// running or done, so the launch drains whatever the occupancy.
+struct DfTask {
+  int type;    // 0 forward assembly chunk, 1 backward gather chunk, 2 matrix-vector
+               // task, 3 scatter chunk
+  int a, b;    // type 0/1: node, chunk; type 2: GvTask index; type 3: chunk
+  int group;   // group signalled on completion
+  int group2;  // second group signalled (-1: none)
+  int pad[3];
+};
+
+}  // namespace dev
+
+constexpr int kSolveChunk = 1024;    // front-vector entries per assembly/gather task
+constexpr int kScatterChunk = 4096;  // vector entries per scatter task
+
+struct SolvePhase {
+  int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks
+  int first, count;
+};
+
+// Vector layout of the solves: per supernode its front vector (ncols + R
+// entries per right-hand side) and, for compressed nodes, rank-sized vectors
+// for the off-diagonal pair and every HODLR node.
+struct SolveLayout {
+  std::vector<dev::VecNode> vn;
+  std::vector<int> children, relmap, rows;
+  std::vector<int64_t> fv_off, tv_node;
+  std::vector<std::vector<int64_t>> tv_h;
+  int64_t fv_stride = 1, tv_stride = 1;
+};
+
+struct LevelTasks {
+  std::vector<dev::GvTask> gv;
+  std::vector<std::vector<SolvePhase>> fwd, bwd;  // per level
+  int64_t slots = 0;                              // partial-sum slots (32 each)
+};
+
+struct DataflowTasks {
+  std::vector<dev::DfTask> tasks;  // topological order
+  std::vector<dev::GvTask> gv;
+  std::vector<unsigned> need;      // tasks per group
+  std::vector<int> gwoff, gw;      // groups waited on, per group
+  int64_t slots = 0;
+};
+
+void build_solve_layout(const Plan& P, SolveLayout& L);
+void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
+void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out);
+
+}  // namespace rsc

@@ -1,0 +1,338 @@
+// CPU emulation of the preconditioner apply's task lists (host code of the
+// library only; no device needed).  For a synthetic system it builds the
+// symbolic plan, the vector layout, the level-scheduled task list and the
+// dataflow task list, fills the factor storage with seeded random values and
+// then runs
+//   (1) the level-scheduled list in launch order,
+//   (2) the dataflow list sequentially in ticket order, checking that every
+//       group a task waits on is complete when the task starts,
+//   (3) the dataflow list on several threads that take tickets and wait on
+//       completion counters, the way the persistent kernel does,
+// with every buffer access bounds-checked.  All three must give bitwise the
+// same result: each task's result depends only on its inputs, so any missing
+// dependency or out-of-range index shows up.
+//
+//   emulate_apply <problem> <N> <max_dense_block> <max_diagonal_block> <nrhs> <threads>
+//   problem: p3d (7-point log-normal Poisson), p2d, el (elasticity)
+// Prints "ok ..." and exits 0 on success.
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../rank_structured_cholesky_amd/csrc/host/rsc_host.h"
+#include "../../rank_structured_cholesky_amd/csrc/host/solve_tasks.h"
+
+using namespace rsc;
+
+namespace {
+
+[[noreturn]] void fail(const std::string& m) {
+  std::fprintf(stderr, "FAIL: %s\n", m.c_str());
+  std::fflush(stderr);
+  std::exit(1);
+}
+
+struct Buf {
+  std::vector<double> v;
+  const char* name;
+  double& at(int64_t i) {
+    if (i < 0 || i >= (int64_t)v.size())
+      fail(std::string("out of range in ") + name + ": " + std::to_string(i) + " of " +
+           std::to_string(v.size()));
+    return v[i];
+  }
+};
+
+template <typename V>
+auto& chk(V& vec, int64_t i, const char* what) {
+  if (i < 0 || i >= (int64_t)vec.size())
+    fail(std::string("out of range in ") + what + ": " + std::to_string(i) + " of " +
+         std::to_string(vec.size()));
+  return vec[i];
+}
+
+struct State {
+  const Plan* P;
+  const SolveLayout* L;
+  Buf F{{}, "factor"}, B{{}, "B"}, X{{}, "X"}, w{{}, "w"}, fv{{}, "fv"}, tv{{}, "tv"},
+      part{{}, "part"};
+  int n = 0, k = 1;
+  int64_t ps = 1;
+  std::vector<int> perm;
+  void reset_work() {
+    std::fill(w.v.begin(), w.v.end(), std::nan(""));
+    std::fill(fv.v.begin(), fv.v.end(), std::nan(""));
+    std::fill(tv.v.begin(), tv.v.end(), std::nan(""));
+    std::fill(part.v.begin(), part.v.end(), std::nan(""));
+    std::fill(X.v.begin(), X.v.end(), std::nan(""));
+  }
+  Buf& vec(int sel) { return sel == 0 ? w : sel == 1 ? fv : tv; }
+  int64_t stride(int sel) const {
+    return sel == 0 ? n : sel == 1 ? L->fv_stride : L->tv_stride;
+  }
+};
+
+// forward assembly chunk; from_w: source is the permuted vector w (level
+// path) instead of B through the permutation (dataflow path)
+void assemble(State& S, int j, int chunk, bool from_w) {
+  const dev::VecNode& N = chk(S.L->vn, j, "vn");
+  const int c = N.ncols, R = N.R;
+  const int p0 = chunk * kSolveChunk, p1 = std::min(p0 + kSolveChunk, c + R);
+  if (p0 >= c + R) return;
+  for (int col = 0; col < S.k; ++col) {
+    const int64_t fb = N.fv_off + (int64_t)col * S.L->fv_stride;
+    for (int i = p0; i < p1; ++i) {
+      double v = 0.0;
+      if (i < c) {
+        const int g = N.first + i;
+        v = from_w ? S.w.at(g + (int64_t)col * S.n)
+                   : S.B.at(chk(S.perm, g, "perm") + (int64_t)col * S.n);
+      }
+      S.fv.at(fb + i) = v;
+    }
+    for (int q = 0; q < N.nchild; ++q) {
+      const int cj = chk(S.L->children, N.child_off + q, "children");
+      const dev::VecNode& C = chk(S.L->vn, cj, "vn");
+      for (int a = 0; a < C.R; ++a) {
+        const int p = chk(S.L->relmap, C.relmap_off + a, "relmap");
+        if (p < 0 || p >= c + R) fail("relmap entry outside the parent front");
+        if (p >= p0 && p < p1)
+          S.fv.at(fb + p) += S.fv.at(C.fv_off + (int64_t)col * S.L->fv_stride + C.ncols + a);
+      }
+    }
+  }
+}
+
+void gather(State& S, int j, int chunk) {
+  const dev::VecNode& N = chk(S.L->vn, j, "vn");
+  const int c = N.ncols, R = N.R;
+  const int p0 = chunk * kSolveChunk, p1 = std::min(p0 + kSolveChunk, c + R);
+  if (p0 >= c + R) return;
+  for (int col = 0; col < S.k; ++col)
+    for (int i = p0; i < p1; ++i) {
+      const int src = i < c ? N.first + i : chk(S.L->rows, N.rows_off + i - c, "rows");
+      S.fv.at(N.fv_off + (int64_t)col * S.L->fv_stride + i) =
+          S.w.at(src + (int64_t)col * S.n);
+    }
+}
+
+void gemv(State& S, const dev::GvTask& t) {
+  for (int col = 0; col < S.k; ++col) {
+    Buf& Y = S.vec(t.ysel);
+    const int64_t yb = (int64_t)col * S.stride(t.ysel) + t.yoff;
+    if (t.kind == 2) {
+      for (int j = 0; j < t.cols; ++j) {
+        double acc = 0;
+        for (int s2 = 0; s2 < t.rows; ++s2)
+          acc += S.part.at((int64_t)col * S.ps + ((int64_t)t.part + s2) * 32 + j);
+        if (t.op) Y.at(yb + j) -= acc;
+        else Y.at(yb + j) = acc;
+      }
+      continue;
+    }
+    Buf& Xv = S.vec(t.xsel);
+    const int64_t xb = (int64_t)col * S.stride(t.xsel) + t.xoff;
+    if (t.kind == 0) {
+      if (t.rows > 64) fail("ROW task with more than 64 rows");
+      for (int i = 0; i < t.rows; ++i) {
+        double acc = 0;
+        for (int j = 0; j < t.cols; ++j)
+          acc += S.F.at(t.mat + i + (int64_t)j * t.ld) * Xv.at(xb + j);
+        if (t.op) Y.at(yb + i) -= acc;
+        else Y.at(yb + i) = acc;
+      }
+    } else {
+      if (t.cols > 32) fail("COL task with more than 32 columns");
+      for (int j = 0; j < t.cols; ++j) {
+        double acc = 0;
+        for (int i = 0; i < t.rows; ++i)
+          acc += S.F.at(t.mat + i + (int64_t)j * t.ld) * Xv.at(xb + i);
+        if (t.part >= 0) S.part.at((int64_t)col * S.ps + (int64_t)t.part * 32 + j) = acc;
+        else if (t.op) Y.at(yb + j) -= acc;
+        else Y.at(yb + j) = acc;
+      }
+    }
+  }
+}
+
+void scatter(State& S, int chunk) {
+  const int i0 = chunk * kScatterChunk, i1 = std::min(i0 + kScatterChunk, S.n);
+  for (int col = 0; col < S.k; ++col)
+    for (int i = i0; i < i1; ++i)
+      S.X.at(chk(S.perm, i, "perm") + (int64_t)col * S.n) = S.w.at(i + (int64_t)col * S.n);
+}
+
+void run_df_task(State& S, const DataflowTasks& D, const dev::DfTask& t) {
+  switch (t.type) {
+    case 0: assemble(S, t.a, t.b, false); break;
+    case 1: gather(S, t.a, t.b); break;
+    case 2: gemv(S, chk(D.gv, t.a, "df gv")); break;
+    case 3: scatter(S, t.a); break;
+    default: fail("bad task type");
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 7) {
+    std::fprintf(stderr, "usage: %s p3d|p2d|el N mdb mdiag nrhs threads\n", argv[0]);
+    return 2;
+  }
+  const std::string prob = argv[1];
+  const int N = std::atoi(argv[2]), mdb = std::atoi(argv[3]), mdiag = std::atoi(argv[4]);
+  const int k = std::atoi(argv[5]), nthreads = std::atoi(argv[6]);
+  Csc A;
+  std::vector<double> xyz;
+  if (prob == "p2d") {
+    A = poisson2d(N);
+    xyz = grid_positions(N, N, 1);
+  } else if (prob == "el") {
+    A = elasticity3d(N, 1.0, 0.3, 7);
+  } else {
+    A = poisson3d_lognormal(N, 1.0, 1234);
+    xyz = grid_positions(N, N, N);
+  }
+  rsc_options opt;
+  rsc_options_default(&opt);
+  opt.max_dense_block = mdb;
+  opt.max_diagonal_block = mdiag;
+  opt.explicit_block_threshold = 2 * mdiag;
+  Plan P;
+  build_plan(A, xyz.empty() ? nullptr : xyz.data(), opt, nullptr, P);
+  SolveLayout L;
+  build_solve_layout(P, L);
+  LevelTasks LT;
+  build_level_tasks(P, L, LT);
+  DataflowTasks D;
+  build_dataflow_tasks(P, L, D);
+
+  State S;
+  S.P = &P;
+  S.L = &L;
+  S.n = P.n;
+  S.k = k;
+  S.perm = P.perm;
+  S.ps = std::max<int64_t>(std::max(LT.slots, D.slots) * 32, 1);
+  std::mt19937_64 g(99);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  S.F.v.resize(std::max<int64_t>(P.factor_elems, 1));
+  for (double& x : S.F.v) x = U(g) * 0.1;
+  S.B.v.resize((size_t)S.n * k);
+  for (double& x : S.B.v) x = U(g);
+  S.X.v.resize((size_t)S.n * k);
+  S.w.v.resize((size_t)S.n * k);
+  S.fv.v.resize((size_t)L.fv_stride * k);
+  S.tv.v.resize((size_t)L.tv_stride * k);
+  S.part.v.resize((size_t)S.ps * k);
+
+  // (1) level-scheduled order
+  S.reset_work();
+  for (int col = 0; col < k; ++col)
+    for (int i = 0; i < S.n; ++i)
+      S.w.at(i + (int64_t)col * S.n) = S.B.at(S.perm[i] + (int64_t)col * S.n);
+  auto run_level = [&](const std::vector<SolvePhase>& ph, int lv) {
+    for (const SolvePhase& p : ph) {
+      if (p.kind == 0 || p.kind == 1) {
+        for (int j : P.levels[lv]) {
+          const int nch = (P.nodes[j].ncols + P.nodes[j].R() + kSolveChunk - 1) / kSolveChunk;
+          for (int c = 0; c < nch; ++c) {
+            if (p.kind == 0) assemble(S, j, c, true);
+            else gather(S, j, c);
+          }
+        }
+      } else {
+        for (int q = p.first; q < p.first + p.count; ++q) gemv(S, chk(LT.gv, q, "level gv"));
+      }
+    }
+  };
+  const int nlev = (int)P.levels.size();
+  for (int lv = 0; lv < nlev; ++lv) run_level(LT.fwd[lv], lv);
+  for (int lv = nlev - 1; lv >= 0; --lv) run_level(LT.bwd[lv], lv);
+  for (int c = 0; c * kScatterChunk < S.n; ++c) scatter(S, c);
+  const std::vector<double> x_level = S.X.v;
+  for (double v : x_level)
+    if (!std::isfinite(v)) fail("level path left a non-finite output entry");
+
+  // (2) dataflow list, sequential, with the wait check
+  const int ng = (int)D.need.size();
+  if ((int)D.gwoff.size() != ng + 1) fail("group wait offsets size");
+  S.reset_work();
+  {
+    std::vector<unsigned> cnt(ng, 0u);
+    for (size_t i = 0; i < D.tasks.size(); ++i) {
+      const dev::DfTask& t = D.tasks[i];
+      if (t.group < 0 || t.group >= ng) fail("task group out of range");
+      for (int q = D.gwoff[t.group]; q < D.gwoff[t.group + 1]; ++q) {
+        const int wg = chk(D.gw, q, "gw");
+        if (wg < 0 || wg >= ng) fail("waited group out of range");
+        if (cnt[wg] != D.need[wg]) {
+          int first = -1, last = -1, nsig = 0;
+          for (size_t u = 0; u < D.tasks.size(); ++u)
+            if (D.tasks[u].group == wg || D.tasks[u].group2 == wg) {
+              if (first < 0) first = (int)u;
+              last = (int)u;
+              ++nsig;
+            }
+          fail("task " + std::to_string(i) + " (type " + std::to_string(t.type) + ", group " +
+               std::to_string(t.group) + ") starts before group " + std::to_string(wg) +
+               " completed: count " + std::to_string(cnt[wg]) + " need " +
+               std::to_string(D.need[wg]) + ", signalled by " + std::to_string(nsig) +
+               " tasks at " + std::to_string(first) + ".." + std::to_string(last));
+        }
+      }
+      run_df_task(S, D, t);
+      cnt[t.group] += 1u;
+      if (t.group2 >= 0) chk(cnt, t.group2, "group2") += 1u;
+    }
+    for (int q = 0; q < ng; ++q)
+      if (cnt[q] != D.need[q]) fail("group " + std::to_string(q) + " count mismatch");
+  }
+  if (S.X.v != x_level) fail("sequential dataflow result differs from the level path");
+
+  // (3) dataflow list on threads: tickets + completion counters + waits
+  S.reset_work();
+  {
+    std::vector<std::atomic<unsigned>> cnt(ng);
+    for (auto& c : cnt) c.store(0u);
+    std::atomic<int> ticket{0};
+    std::atomic<bool> stuck{false};
+    auto worker = [&]() {
+      for (;;) {
+        const int i = ticket.fetch_add(1);
+        if (i >= (int)D.tasks.size()) return;
+        const dev::DfTask& t = D.tasks[i];
+        for (int q = D.gwoff[t.group]; q < D.gwoff[t.group + 1]; ++q) {
+          const int wg = D.gw[q];
+          const auto t0 = std::chrono::steady_clock::now();
+          while (cnt[wg].load(std::memory_order_acquire) < D.need[wg]) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+              stuck = true;
+              return;
+            }
+            std::this_thread::yield();
+          }
+        }
+        run_df_task(S, D, t);
+        cnt[t.group].fetch_add(1u, std::memory_order_release);
+        if (t.group2 >= 0) cnt[t.group2].fetch_add(1u, std::memory_order_release);
+      }
+    };
+    std::vector<std::thread> th;
+    for (int q = 0; q < nthreads; ++q) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    if (stuck) fail("threaded dataflow run stalled");
+  }
+  if (S.X.v != x_level) fail("threaded dataflow result differs from the level path");
+  std::printf("ok n=%d nodes=%d levels=%d level_tasks=%zu df_tasks=%zu groups=%d\n", S.n,
+              (int)P.nodes.size(), nlev, LT.gv.size(), D.tasks.size(), ng);
+  return 0;
+}

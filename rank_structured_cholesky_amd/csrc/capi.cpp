@@ -28,6 +28,7 @@ struct rsc_factor {
   hipStream_t stream = nullptr;
   std::unique_ptr<rsc::FactorEngine> eng;
   rsc::DevCsr dA;
+  rsc::PcgWork pcgw;
   rsc_factor_info info{};
 };
 
@@ -291,7 +292,7 @@ rsc_status rsc_block_pcg_solve(rsc_factor* f, int k, const double* B, int ldb,
     return fail(RSC_BAD_INPUT, "bad PCG arguments");
   return guard([&]() -> rsc_status {
     hipStream_t s = stream ? (hipStream_t)stream : f->stream;
-    rsc::pcg_solve_device(*f->eng, f->dA, k, B, ldb, X, ldx, eps, max_iters, res, s);
+    rsc::pcg_solve_device(*f->eng, f->dA, f->pcgw, k, B, ldb, X, ldx, eps, max_iters, res, s);
     if (res)
       for (int c = 0; c < k; ++c)
         if (!(res[c].recurrence_relres <= eps)) return RSC_NOT_CONVERGED;

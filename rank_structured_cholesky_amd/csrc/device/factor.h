@@ -83,8 +83,10 @@ class FactorEngine {
  public:
   virtual ~FactorEngine() = default;
   virtual void numeric(const double* Ax_host) = 0;          // may throw IndefiniteError
-  virtual void apply(int k, const double* B, int ldb, double* X, int ldx,
-                     hipStream_t s) = 0;                      // device pointers
+  // X = M^{-1} B (device pointers).  live: optional device flag; when it
+  // reads 0 at run time the apply does nothing (converged PCG columns).
+  virtual void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
+                     const int* live = nullptr) = 0;
   // Synchronizes s and throws DeviceError if an apply on it failed.
   virtual void check_apply(hipStream_t s) = 0;
   virtual int64_t factor_bytes() const = 0;

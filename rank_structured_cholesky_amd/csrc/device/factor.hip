@@ -85,8 +85,8 @@ class Engine final : public FactorEngine {
   ~Engine() override { (void)hipStreamSynchronize(st_); }
 
   void numeric(const double* Ax) override;
-  void apply(int k, const double* B, int ldb, double* X, int ldx,
-             hipStream_t s) override;
+  void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
+             const int* live) override;
   void check_apply(hipStream_t s) override;
   int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
   int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
@@ -1275,7 +1275,7 @@ void Engine<T>::numeric(const double* Ax) {
 
 template <typename T>
 void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
-                      hipStream_t s) {
+                      hipStream_t s, const int* live) {
   const int n = plan_.n;
   if (k > fv_k_) {
     dfv_.alloc((size_t)fv_stride_ * k * sizeof(double));
@@ -1285,7 +1285,8 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     fv_k_ = k;
   }
   dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(),
-                  dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_};
+                  dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_,
+                  live};
   if (use_dataflow_) {
     dev::DfArgs a{};
     a.tasks = ddf_.as<dev::DfTask>();
@@ -1311,7 +1312,7 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     dev::launch_apply_dataflow<T>(F(), a, vb, k, df_grid_, s);
     return;
   }
-  dev::launch_gather(B, ldb, vb.w, dperm_.as<int>(), n, k, s);
+  dev::launch_gather(B, ldb, vb.w, dperm_.as<int>(), n, k, live, s);
   const int nlev = (int)plan_.levels.size();
   auto run = [&](const std::vector<SolvePhase>& ph, int lv) {
     for (const SolvePhase& p : ph) {
@@ -1332,7 +1333,7 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   };
   for (int lv = 0; lv < nlev; ++lv) run(fwd_[lv], lv);
   for (int lv = nlev - 1; lv >= 0; --lv) run(bwd_[lv], lv);
-  dev::launch_scatter(vb.w, X, ldx, dperm_.as<int>(), n, k, s);
+  dev::launch_scatter(vb.w, X, ldx, dperm_.as<int>(), n, k, live, s);
 }
 
 template <typename T>

@@ -142,8 +142,17 @@ __global__ void k_commit(int k, PcgCol* __restrict__ st, int* __restrict__ flags
   if (threadIdx.x == 0) *any = cnt;
 }
 
-__global__ void k_flags(int k, const PcgCol* __restrict__ st, int* __restrict__ flags) {
-  for (int c = threadIdx.x; c < k; c += blockDim.x) flags[c] = st[c].active;
+__global__ void k_flags(int k, const PcgCol* __restrict__ st, int* __restrict__ flags,
+                        int* __restrict__ any) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    flags[c] = st[c].active;
+    if (st[c].active) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *any = cnt;
 }
 
 }  // namespace dev
@@ -167,80 +176,93 @@ void DevCsr::spmv(const double* X, int ldx, double* Y, int ldy, int k,
                    active, s, lanes);
 }
 
-void pcg_solve_device(FactorEngine& M, const DevCsr& A, int k, const double* B,
+PcgWork::~PcgWork() {
+  if (host_any) (void)hipHostFree(host_any);
+  for (hipEvent_t& e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void PcgWork::ensure(int n_, int k_) {
+  using namespace dev;
+  if (!host_any) {
+    RSC_HIP(hipHostMalloc((void**)&host_any, 2 * sizeof(int), hipHostMallocDefault));
+    for (hipEvent_t& e : ev) RSC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (n_ == n && k_ <= k) return;
+  const size_t vb = (size_t)n_ * k_ * sizeof(double);
+  r.alloc(vb);
+  d.alloc(vb);
+  q.alloc(vb);
+  sv.alloc(vb);
+  part1.alloc((size_t)k_ * RMAX * sizeof(double));
+  part2.alloc((size_t)k_ * RMAX * sizeof(double));
+  stb.alloc((size_t)k_ * sizeof(PcgCol));
+  flags.alloc((size_t)k_ * sizeof(int));
+  anyb.alloc(sizeof(int));
+  n = n_;
+  k = k_;
+}
+
+// The host checks convergence one iteration behind: iteration i+1 is queued
+// before the flag of iteration i is read, so the device never idles on the
+// check.  A queued iteration whose columns have all converged does no work
+// (every kernel of it returns on the flags).
+void pcg_solve_device(FactorEngine& M, const DevCsr& A, PcgWork& w, int k, const double* B,
                       int ldb, double* X, int ldx, double eps, int max_iters,
                       rsc_pcg_result* res, hipStream_t s) {
   using namespace dev;
   auto t0 = std::chrono::steady_clock::now();
   const int n = A.n;
-  DBuf r, d, q, sv, part1, part2, stb, flags, anyb;
-  const size_t vb = (size_t)n * k * sizeof(double);
-  r.alloc(vb);
-  d.alloc(vb);
-  q.alloc(vb);
-  sv.alloc(vb);
-  part1.alloc((size_t)k * RMAX * sizeof(double));
-  part2.alloc((size_t)k * RMAX * sizeof(double));
-  stb.alloc((size_t)k * sizeof(PcgCol));
-  flags.alloc((size_t)k * sizeof(int));
-  anyb.alloc(sizeof(int));
-  RSC_HIP(hipMemsetAsync(stb.p, 0, (size_t)k * sizeof(PcgCol), s));
-  PcgCol* st = stb.as<PcgCol>();
-  double *R = r.as<double>(), *D = d.as<double>(), *Q = q.as<double>(),
-         *S = sv.as<double>();
+  w.ensure(n, k);
+  RSC_HIP(hipMemsetAsync(w.stb.p, 0, (size_t)k * sizeof(PcgCol), s));
+  PcgCol* st = w.stb.as<PcgCol>();
+  double *R = w.r.as<double>(), *D = w.d.as<double>(), *Q = w.q.as<double>(),
+         *S = w.sv.as<double>();
+  int* any = w.anyb.as<int>();
   const int nb = (int)std::min<int64_t>(RMAX, ((int64_t)n + RB * 4 - 1) / (RB * 4));
   const dim3 dgrid(nb, k);
   const int ublocks = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
   const dim3 ugrid(ublocks, k);
   const unsigned eb = (unsigned)(((int64_t)n * k + 255) / 256);
   // ||b||
-  k_dot<<<dgrid, RB, 0, s>>>(n, B, ldb, B, ldb, part1.as<double>());
-  k_scalars<<<k, RB, 0, s>>>(0, part1.as<double>(), nullptr, nb, st, eps, max_iters);
+  k_dot<<<dgrid, RB, 0, s>>>(n, B, ldb, B, ldb, w.part1.as<double>());
+  k_scalars<<<k, RB, 0, s>>>(0, w.part1.as<double>(), nullptr, nb, st, eps, max_iters);
   // r = b - A x ; d = M^{-1} r ; delta = r.d
   A.spmv(X, ldx, R, n, k, nullptr, s);
   k_residual<<<eb, 256, 0, s>>>(n, k, B, ldb, R);
   M.apply(k, R, n, D, n, s);
-  k_dot<<<dgrid, RB, 0, s>>>(n, R, n, D, n, part1.as<double>());
-  k_scalars<<<k, RB, 0, s>>>(1, part1.as<double>(), nullptr, nb, st, eps, max_iters);
-  k_flags<<<1, 256, 0, s>>>(k, st, flags.as<int>());
-  int* any_host = nullptr;
-  RSC_HIP(hipHostMalloc((void**)&any_host, sizeof(int), hipHostMallocDefault));
-  *any_host = 1;
-  const int poll = 4;
+  k_dot<<<dgrid, RB, 0, s>>>(n, R, n, D, n, w.part1.as<double>());
+  k_scalars<<<k, RB, 0, s>>>(1, w.part1.as<double>(), nullptr, nb, st, eps, max_iters);
+  k_flags<<<1, 256, 0, s>>>(k, st, w.flags.as<int>(), any);
   int it = 0;
-  try {
-    while (it < max_iters) {
-      A.spmv(D, n, Q, n, k, flags.as<int>(), s);
-      k_dot<<<dgrid, RB, 0, s>>>(n, D, n, Q, n, part1.as<double>());
-      k_scalars<<<k, RB, 0, s>>>(2, part1.as<double>(), nullptr, nb, st, eps, max_iters);
-      k_update_xr<<<ugrid, 256, 0, s>>>(n, k, st, X, ldx, R, D, Q);
-      M.apply(k, R, n, S, n, s);
-      k_dot<<<dgrid, RB, 0, s>>>(n, R, n, S, n, part1.as<double>());
-      k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, part2.as<double>());
-      k_scalars<<<k, RB, 0, s>>>(3, part1.as<double>(), part2.as<double>(), nb, st, eps,
-                                 max_iters);
-      k_update_d<<<ugrid, 256, 0, s>>>(n, k, st, D, S);
-      k_commit<<<1, 256, 0, s>>>(k, st, flags.as<int>(), anyb.as<int>());
-      ++it;
-      if (it % poll == 0 || it == max_iters) {
-        RSC_HIP(hipMemcpyAsync(any_host, anyb.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        RSC_HIP(hipStreamSynchronize(s));
-        if (*any_host == 0) break;
-      }
+  while (it < max_iters) {
+    A.spmv(D, n, Q, n, k, w.flags.as<int>(), s);
+    k_dot<<<dgrid, RB, 0, s>>>(n, D, n, Q, n, w.part1.as<double>());
+    k_scalars<<<k, RB, 0, s>>>(2, w.part1.as<double>(), nullptr, nb, st, eps, max_iters);
+    k_update_xr<<<ugrid, 256, 0, s>>>(n, k, st, X, ldx, R, D, Q);
+    M.apply(k, R, n, S, n, s, any);
+    k_dot<<<dgrid, RB, 0, s>>>(n, R, n, S, n, w.part1.as<double>());
+    k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, w.part2.as<double>());
+    k_scalars<<<k, RB, 0, s>>>(3, w.part1.as<double>(), w.part2.as<double>(), nb, st, eps,
+                               max_iters);
+    k_update_d<<<ugrid, 256, 0, s>>>(n, k, st, D, S);
+    k_commit<<<1, 256, 0, s>>>(k, st, w.flags.as<int>(), any);
+    RSC_HIP(hipMemcpyAsync(w.host_any + (it & 1), any, sizeof(int), hipMemcpyDeviceToHost, s));
+    RSC_HIP(hipEventRecord(w.ev[it & 1], s));
+    ++it;
+    if (it >= 2) {
+      RSC_HIP(hipEventSynchronize(w.ev[it & 1]));  // iteration it - 2 (0-based)
+      if (w.host_any[it & 1] == 0) break;
     }
-  } catch (...) {
-    (void)hipHostFree(any_host);
-    throw;
   }
-  (void)hipHostFree(any_host);
   // true residual
   A.spmv(X, ldx, R, n, k, nullptr, s);
   k_residual<<<eb, 256, 0, s>>>(n, k, B, ldb, R);
-  k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, part1.as<double>());
+  k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, w.part1.as<double>());
   std::vector<PcgCol> hst(k);
   std::vector<double> hp((size_t)k * RMAX);
-  RSC_HIP(hipMemcpyAsync(hst.data(), stb.p, k * sizeof(PcgCol), hipMemcpyDeviceToHost, s));
-  RSC_HIP(hipMemcpyAsync(hp.data(), part1.p, hp.size() * sizeof(double),
+  RSC_HIP(hipMemcpyAsync(hst.data(), w.stb.p, k * sizeof(PcgCol), hipMemcpyDeviceToHost, s));
+  RSC_HIP(hipMemcpyAsync(hp.data(), w.part1.p, hp.size() * sizeof(double),
                          hipMemcpyDeviceToHost, s));
   RSC_HIP(hipStreamSynchronize(s));
   RSC_HIP(hipGetLastError());

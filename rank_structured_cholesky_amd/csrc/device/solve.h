@@ -13,10 +13,10 @@ namespace dev {
 
 // w[i + j*n] = b[perm[i] + j*ldb]
 void launch_gather(const double* b, int ldb, double* w, const int* perm, int n,
-                   int k, hipStream_t s);
+                   int k, const int* live, hipStream_t s);
 // x[perm[i] + j*ldx] = w[i + j*n]
 void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
-                    int k, hipStream_t s);
+                    int k, const int* live, hipStream_t s);
 
 struct VecBufs {
   double* w;
@@ -24,6 +24,7 @@ struct VecBufs {
   double* tv;
   double* part;
   int64_t n, fvs, tvs, ps;
+  const int* live;  // when non-null and *live == 0 every kernel of the apply returns
 };
 
 // forward: fv_c = w[cols], fv_R = 0, plus children's fv_R via relmap

@@ -25,7 +25,9 @@ namespace dev {
 constexpr int MAXR = 1024;  // largest rank handled in LDS by the solves
 
 __global__ void k_gather(const double* __restrict__ b, int ldb, double* __restrict__ w,
-                         const int* __restrict__ perm, int n, int k) {
+                         const int* __restrict__ perm, int n, int k,
+                         const int* __restrict__ live) {
+  if (live && *live == 0) return;
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * k) return;
   int i = (int)(t % n), j = (int)(t / n);
@@ -33,7 +35,9 @@ __global__ void k_gather(const double* __restrict__ b, int ldb, double* __restri
 }
 
 __global__ void k_scatter(const double* __restrict__ w, double* __restrict__ x, int ldx,
-                          const int* __restrict__ perm, int n, int k) {
+                          const int* __restrict__ perm, int n, int k,
+                          const int* __restrict__ live) {
+  if (live && *live == 0) return;
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * k) return;
   int i = (int)(t % n), j = (int)(t / n);
@@ -41,15 +45,15 @@ __global__ void k_scatter(const double* __restrict__ w, double* __restrict__ x, 
 }
 
 void launch_gather(const double* b, int ldb, double* w, const int* perm, int n,
-                   int k, hipStream_t s) {
+                   int k, const int* live, hipStream_t s) {
   int64_t tot = (int64_t)n * k;
-  k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(b, ldb, w, perm, n, k);
+  k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(b, ldb, w, perm, n, k, live);
 }
 
 void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
-                    int k, hipStream_t s) {
+                    int k, const int* live, hipStream_t s) {
   int64_t tot = (int64_t)n * k;
-  k_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(w, x, ldx, perm, n, k);
+  k_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(w, x, ldx, perm, n, k, live);
 }
 
 // ---- vector gathers of a level -------------------------------------------
@@ -109,9 +113,25 @@ __device__ inline VecNode uniform_node(const VecNode& n) {
   return u;
 }
 
+// Vector accesses of the task bodies.  COH = true (dataflow launch): data
+// that other workgroups of the same launch produce or consume goes through
+// device-scope loads and stores, which bypass the per-XCD caches, so no
+// cache-wide writeback or invalidate is needed between tasks.
+template <bool COH>
+__device__ inline double vld(const double* p) {
+  if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool COH>
+__device__ inline void vst(double* p, double v) {
+  if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
 // Forward assembly of one 1024-entry chunk of node S's front vector:
 // fv_c = source[perm[cols]] (perm may be null: source indexed directly),
 // fv_R = 0, then the children's update vectors added through their relmaps.
+template <bool COH>
 __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
                                const int* __restrict__ children,
                                const int* __restrict__ relmap, const VecBufs& b, int k,
@@ -124,7 +144,7 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
     double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
     const double* sc = src + (int64_t)col * lds;
     for (int i = p0 + tid; i < p1; i += 256)
-      f[i] = i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0;
+      vst<COH>(f + i, i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0);
     __syncthreads();
     for (int q = 0; q < S.nchild; ++q) {
       const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
@@ -133,7 +153,7 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
       // one predicated pass over the child's rows (no dependent searches)
       for (int a = tid; a < C.R; a += 256) {
         const int p = rm[a];
-        if (p >= p0 && p < p1) f[p] += fc[a];
+        if (p >= p0 && p < p1) vst<COH>(f + p, vld<COH>(f + p) + vld<COH>(fc + a));
       }
       __syncthreads();
     }
@@ -141,6 +161,7 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
 }
 
 // Backward gather of one chunk: fv_c = w[cols], fv_R[a] = w[rows[a]].
+template <bool COH>
 __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
                              const VecBufs& b, int k) {
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
@@ -150,7 +171,8 @@ __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict_
   for (int col = 0; col < k; ++col) {
     double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
     const double* wc = b.w + (int64_t)col * b.n;
-    for (int i = p0 + tid; i < p1; i += 256) f[i] = i < c ? wc[S.first + i] : wc[rw[i - c]];
+    for (int i = p0 + tid; i < p1; i += 256)
+      vst<COH>(f + i, vld<COH>(wc + (i < c ? S.first + i : rw[i - c])));
   }
 }
 
@@ -159,16 +181,18 @@ __global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict_
                                                       const int* __restrict__ children,
                                                       const int* __restrict__ relmap,
                                                       VecBufs b, int k) {
+  if (b.live && *b.live == 0) return;
   const VecNode S = nodes[lnodes[blockIdx.x]];
-  assemble_chunk(S, blockIdx.y, nodes, children, relmap, b, k, b.w, b.n, nullptr);
+  assemble_chunk<false>(S, blockIdx.y, nodes, children, relmap, b, k, b.w, b.n, nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ nodes,
                                                     const int* __restrict__ lnodes,
                                                     const int* __restrict__ rows,
                                                     VecBufs b, int k) {
+  if (b.live && *b.live == 0) return;
   const VecNode S = nodes[lnodes[blockIdx.x]];
-  gather_chunk(S, blockIdx.y, rows, b, k);
+  gather_chunk<false>(S, blockIdx.y, rows, b, k);
 }
 
 void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
@@ -195,6 +219,7 @@ __global__ __launch_bounds__(256) void k_gemv_small(const T* __restrict__ F,
                                                     const GvTask* __restrict__ tasks,
                                                     VecBufs b, int k) {
   __shared__ double red[4][64];
+  if (b.live && *b.live == 0) return;
   const GvTask t = tasks[blockIdx.x];
   const T* __restrict__ M = F + t.mat;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -341,6 +366,7 @@ __global__ __launch_bounds__(256) void k_gemv_tasks(const T* __restrict__ F,
                                                     const GvTask* __restrict__ tasks,
                                                     VecBufs b, int k) {
   __shared__ double red[4][64];
+  if (b.live && *b.live == 0) return;
   const GvTask t = tasks[blockIdx.x];
   gemv_task<T>(F, t, b, k, red);
 }
@@ -365,7 +391,7 @@ template void launch_gemv_tasks<float>(const float*, const GvTask*, int, VecBufs
 // dependencies).  ROW: lane = row, wave g takes columns g, g+4, ..., eight
 // per batch; COL: wave w owns columns 8w..8w+7, lane l rows l + 64m, two
 // row blocks per batch.
-template <typename T>
+template <typename T, bool COH>
 __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBufs& b, int k,
                           double (*red)[64]) {
   const GvTask t = uniform_task(t0);
@@ -377,9 +403,8 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
       const double* pp = b.part + (int64_t)col * b.ps + (int64_t)t.part * 32;
       if (tid < t.cols) {
         double acc = 0;
-        for (int s2 = 0; s2 < t.rows; ++s2) acc += pp[(int64_t)s2 * 32 + tid];
-        if (t.op) y[tid] -= acc;
-        else y[tid] = acc;
+        for (int s2 = 0; s2 < t.rows; ++s2) acc += vld<COH>(pp + (int64_t)s2 * 32 + tid);
+        vst<COH>(y + tid, t.op ? vld<COH>(y + tid) - acc : acc);
       }
       continue;
     }
@@ -396,7 +421,7 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           m[u] = (double)p[u * ld4];
-          xv[u] = x[j + 4 * u];
+          xv[u] = vld<COH>(x + j + 4 * u);
         }
 #pragma unroll
         for (int u = 0; u < 8; u += 2) {
@@ -404,13 +429,12 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
           acc1 += m[u + 1] * xv[u + 1];
         }
       }
-      for (; j < t.cols; j += 4) acc0 += (double)rp[(int64_t)j * t.ld] * x[j];
+      for (; j < t.cols; j += 4) acc0 += (double)rp[(int64_t)j * t.ld] * vld<COH>(x + j);
       red[wave][lane] = act ? acc0 + acc1 : 0.0;
       __syncthreads();
       if (wave == 0 && act) {
         const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-        if (t.op) y[lane] -= v;
-        else y[lane] = v;
+        vst<COH>(y + lane, t.op ? vld<COH>(y + lane) - v : v);
       }
       __syncthreads();
     } else {
@@ -424,7 +448,7 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
         for (int i = lane; i < t.rows; i += 128) {
           const int i2 = i + 64;
           const bool ok2 = i2 < t.rows;
-          const double x1 = x[i], x2 = ok2 ? x[i2] : 0.0;
+          const double x1 = vld<COH>(x + i), x2 = ok2 ? vld<COH>(x + i2) : 0.0;
           double m1[8], m2[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
@@ -440,9 +464,8 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
       for (int u = 0; u < 8; ++u) {
         const double v = wave_sum(acc[u]);
         if (lane == 0 && u < nq) {
-          if (t.part >= 0) b.part[(int64_t)col * b.ps + (int64_t)t.part * 32 + q0 + u] = v;
-          else if (t.op) y[q0 + u] -= v;
-          else y[q0 + u] = v;
+          if (t.part >= 0) vst<COH>(b.part + (int64_t)col * b.ps + (int64_t)t.part * 32 + q0 + u, v);
+          else vst<COH>(y + q0 + u, t.op ? vld<COH>(y + q0 + u) - v : v);
         }
       }
     }
@@ -459,6 +482,7 @@ __device__ bool df_wait(const DfArgs& a, int g) {
   if (__hip_atomic_load(a.cnt + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need)
     return true;
   const unsigned long long t0 = wall_clock64();
+  int backoff = 1;
   while (__hip_atomic_load(a.cnt + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
     if (__hip_atomic_load(a.status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       return false;
@@ -467,7 +491,8 @@ __device__ bool df_wait(const DfArgs& a, int g) {
       __hip_atomic_store(a.status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
-    __builtin_amdgcn_s_sleep(2);
+    for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(4);
+    if (backoff < 16) backoff *= 2;
   }
   return true;
 }
@@ -483,6 +508,7 @@ __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArg
   __shared__ int s_ticket;
   __shared__ int s_ok;
   const int tid = threadIdx.x;
+  if (b.live && uni(*b.live) == 0) return;
   if (tid == 0) {
     s_ok = 1;
     s_ticket = (int)__hip_atomic_fetch_add(a.cnt + a.ngroups, 1u, __ATOMIC_RELAXED,
@@ -501,27 +527,27 @@ __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArg
         bool ok = s_ok != 0;
         for (int q = w0; ok && q < w1; ++q) ok = df_wait(a, a.gwait[q]);
         s_ok = ok ? 1 : 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       }
       __syncthreads();
     }
-    // every wave: later loads observe what the producers released
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (type == 0) {
-      assemble_chunk(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b, k, a.B,
-                     a.ldb, a.perm);
+      assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b, k,
+                           a.B, a.ldb, a.perm);
     } else if (type == 1) {
-      gather_chunk(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
+      gather_chunk<true>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
-      gemv_lite<T>(F, a.gv[ta], b, k, red);
+      gemv_lite<T, true>(F, a.gv[ta], b, k, red);
     } else {
       const int i0 = ta * DF_SCATTER_CHUNK;
       const int i1 = min(i0 + DF_SCATTER_CHUNK, (int)b.n);
       for (int col = 0; col < k; ++col)
         for (int i = i0 + tid; i < i1; i += 256)
-          a.X[a.perm[i] + col * a.ldx] = b.w[i + col * b.n];
+          a.X[a.perm[i] + col * a.ldx] = vld<true>(b.w + i + col * b.n);
     }
-    // every wave: its stores are written back before the completion count
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every thread: its device-scope stores have completed before the count
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (tid == 0) {
       __hip_atomic_fetch_add(a.cnt + group, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

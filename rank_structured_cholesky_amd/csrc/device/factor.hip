@@ -75,6 +75,13 @@ struct SketchJob {
   int node;
 };
 
+// Tree node q lies in the subtree rooted at sub (q == sub included).
+inline bool inside_subtree(const Supernode& s, int q, int sub) {
+  for (int a = q; a >= 0; a = s.hnodes[a].parent)
+    if (a == sub) return true;
+  return false;
+}
+
 template <typename T>
 class Engine final : public FactorEngine {
  public:
@@ -104,7 +111,7 @@ class Engine final : public FactorEngine {
   DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
-  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_;
+  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_, dz_;
   std::vector<std::vector<SolvePhase>> fwd_, bwd_;
   int64_t tv_stride_ = 0, ntasks_ = 0, part_stride_ = 1;
   DBuf dpart_;
@@ -141,6 +148,18 @@ class Engine final : public FactorEngine {
   int64_t esize() const { return cur_f64_ ? 8 : (int64_t)sizeof(T); }
   std::vector<dev::ConvJob> vjobs_;
   int64_t qr64_elems_ = 0;  // FP64 QR scratch (in T elements)
+  int64_t zp_elems_ = 0;    // Zp = L21 Lc^{-1} staging of one level (T elements)
+  // Solve-helper targets of a compressed diagonal: matrices S (rows of
+  // subtree `sub`, r columns) that receive D = L_sub^{-1} S (W, from V) or
+  // D = L_sub^{-T} S (X, from U).  Internal tree nodes give one W and one X
+  // target; the off-diagonal pair gives X_off = L_D^{-T} U (subtree = root).
+  struct HTarget {
+    int sub;
+    bool isW;
+    int64_t src, dst;  // factor offsets
+    int rank;
+  };
+  std::vector<HTarget> helper_targets(int j) const;
   void push(const Step& st0) {
     Step st = st0;
     st.f64 = cur_f64_;
@@ -166,6 +185,7 @@ class Engine final : public FactorEngine {
   int64_t rng_base() const { return scratch_base() + scratch_elems_; }
   int64_t inv_base() const { return rng_base() + plan_.rng_count; }
   int64_t qr64_base() const { return inv_base() + inv_elems_; }
+  int64_t zp_base() const { return qr64_base() + qr64_elems_; }
 
   void setup_static();
   void build_program();
@@ -632,14 +652,16 @@ void Engine<T>::layout_scratch() {
     {
       int maxh = 0;
       for (auto& h : s.hnodes) maxh = std::max(maxh, h.height);
+      const std::vector<HTarget> tg = helper_targets((int)j);
       int64_t best = 0;
-      for (int hh = 1; hh < maxh; ++hh) {
+      for (int hh = 1; hh <= maxh; ++hh) {
         int64_t sum = 0;
-        for (auto& hm : s.hnodes) {
-          if (hm.leaf() || hm.height <= hh) continue;
-          for (auto& hn : s.hnodes)
-            if (!hn.leaf() && hn.height == hh) sum += (int64_t)hn.rank * hm.rank;
-        }
+        for (const HTarget& t : tg)
+          for (size_t q = 0; q < s.hnodes.size(); ++q) {
+            const HNode& hn = s.hnodes[q];
+            if (hn.leaf() || hn.height != hh) continue;
+            if (inside_subtree(s, (int)q, t.sub)) sum += (int64_t)hn.rank * t.rank;
+          }
         best = std::max(best, sum);
       }
       wx_scratch_[j] = off;
@@ -655,6 +677,14 @@ void Engine<T>::layout_scratch() {
     }
   }
   scratch_elems_ = off;
+  // Zp staging: per level, the uncompressed panels' R x c blocks
+  zp_elems_ = 0;
+  for (auto& lv : plan_.levels) {
+    int64_t sum = 0;
+    for (int j : lv)
+      if (!plan_.nodes[j].compress) sum += (int64_t)plan_.nodes[j].R() * plan_.nodes[j].ncols;
+    zp_elems_ = std::max(zp_elems_, sum);
+  }
   // FP64 QR scratch (FP32 factors): largest concurrent batch of QRs, each
   // max(rows, cols) x r + r x r doubles, counted in T elements
   qr64_elems_ = 0;
@@ -783,72 +813,71 @@ void Engine<T>::emit_compressed(int j) {
   emit_solve_helpers(j);
 }
 
-// Solve helpers of a compressed diagonal: for every internal tree node M
-// with children (A, B): W_M = L_B^{-1} V_M and X_M = L_A^{-T} U_M.  With
-// them the tree solves become one leaf step plus one low-rank correction per
-// tree height:  forward  y_B -= W_M (U_M^T y_A),  backward  x_A -= X_M
-// (V_M^T x_B), both bottom-up.  Computed here with the same recursions
-// applied to V_M / U_M: leaf blocks first (leaves hold L^{-T}), then the
-// corrections of every internal node N inside the subtree, by height.
+template <typename T>
+std::vector<typename Engine<T>::HTarget> Engine<T>::helper_targets(int j) const {
+  const Supernode& s = plan_.nodes[j];
+  std::vector<HTarget> tg;
+  for (int M = 0; M < (int)s.hnodes.size(); ++M) {
+    const HNode& hm = s.hnodes[M];
+    if (hm.leaf()) continue;
+    tg.push_back({hm.right, true, hm.off_V, hm.off_W, hm.rank});
+    tg.push_back({hm.left, false, hm.off_U, hm.off_X, hm.rank});
+  }
+  if (s.R() > 0 && s.rank > 0) tg.push_back({hodlr_root(s), false, s.off_U, s.off_Xoff, s.rank});
+  return tg;
+}
+
+// Solve helpers of a compressed diagonal.  For every internal tree node M
+// with children (A, B): W_M = L_B^{-1} V_M and X_M = L_A^{-T} U_M; for the
+// off-diagonal pair X_off = L_D^{-T} U.  With them the tree solves become
+// one leaf step plus one low-rank correction per tree height (forward
+// y_B -= W_M (U_M^T y_A), transposed x_A -= X_M (V_M^T x_B), bottom-up),
+// and the off-diagonal block acts on the front without a diagonal solve:
+// L21 L_D^{-1} = V X_off^T.  Computed with the same recursions applied to
+// the target matrices: leaf blocks first (leaves hold L^{-T}), then the
+// corrections of every internal node N inside the target's subtree, by
+// height.
 template <typename T>
 void Engine<T>::emit_solve_helpers(int j) {
   const Supernode& s = plan_.nodes[j];
   int maxh = 0;
   for (auto& h : s.hnodes) maxh = std::max(maxh, h.height);
-  if (maxh == 0) return;
-  auto inside = [&](int q, int sub) {  // node q inside subtree sub
-    for (int a = q; a >= 0; a = s.hnodes[a].parent)
-      if (a == sub) return true;
-    return false;
-  };
+  const std::vector<HTarget> tg = helper_targets(j);
+  if (tg.empty()) return;
+  auto sub_r0 = [&](const HTarget& t) { return s.hnodes[t.sub].r0; };
+  auto sub_n = [&](const HTarget& t) { return s.hnodes[t.sub].r1 - s.hnodes[t.sub].r0; };
   std::vector<dev::GemmJob> tn, nn;
-  for (int M = 0; M < (int)s.hnodes.size(); ++M) {
-    const HNode& hm = s.hnodes[M];
-    if (hm.leaf()) continue;
-    const HNode& A = s.hnodes[hm.left];
-    const HNode& B = s.hnodes[hm.right];
-    const int cr = B.r1 - B.r0, cl = A.r1 - A.r0, r = hm.rank;
+  for (const HTarget& t : tg) {
+    const int ld = sub_n(t), r0 = sub_r0(t);
     for (int q = 0; q < (int)s.hnodes.size(); ++q) {
       const HNode& lf = s.hnodes[q];
-      if (!lf.leaf()) continue;
+      if (!lf.leaf() || !inside_subtree(s, q, t.sub)) continue;
       const int b = lf.r1 - lf.r0;
       dev::GemmJob g;
       g.m = b;
-      g.n = r;
+      g.n = t.rank;
       g.k = b;
       g.lda = b;
       g.alpha = 1.0;
       g.beta = 0.0;
-      if (inside(q, hm.right)) {  // W_M rows of this leaf = Mt^T V_M rows
-        g.A = fp(lf.off_leaf);
-        g.B = fp(hm.off_V + (lf.r0 - B.r0));
-        g.C = fp(hm.off_W + (lf.r0 - B.r0));
-        g.ldb = g.ldc = cr;
-        tn.push_back(g);
-      } else if (inside(q, hm.left)) {  // X_M rows of this leaf = Mt U_M rows
-        g.A = fp(lf.off_leaf);
-        g.B = fp(hm.off_U + (lf.r0 - A.r0));
-        g.C = fp(hm.off_X + (lf.r0 - A.r0));
-        g.ldb = g.ldc = cl;
-        nn.push_back(g);
-      }
+      g.A = fp(lf.off_leaf);
+      g.B = fp(t.src + (lf.r0 - r0));
+      g.C = fp(t.dst + (lf.r0 - r0));
+      g.ldb = g.ldc = ld;
+      (t.isW ? tn : nn).push_back(g);
     }
   }
   emit_gemm(OP_GEMM_TN, tn, false);
   emit_gemm(OP_GEMM_NN, nn, false);
   const int64_t tbase = scratch_base() + wx_scratch_[j];
-  for (int h = 1; h < maxh; ++h) {
+  for (int h = 1; h <= maxh; ++h) {
     std::vector<dev::GemmJob> t1, t2, u1, u2;
     int64_t toff = tbase;
-    for (int M = 0; M < (int)s.hnodes.size(); ++M) {
-      const HNode& hm = s.hnodes[M];
-      if (hm.leaf() || hm.height <= h) continue;
-      const HNode& A = s.hnodes[hm.left];
-      const HNode& B = s.hnodes[hm.right];
-      const int cr = B.r1 - B.r0, cl = A.r1 - A.r0, rm = hm.rank;
+    for (const HTarget& t : tg) {
+      const int ld = sub_n(t), r0 = sub_r0(t), rm = t.rank;
       for (int N = 0; N < (int)s.hnodes.size(); ++N) {
         const HNode& hn = s.hnodes[N];
-        if (hn.leaf() || hn.height != h) continue;
+        if (hn.leaf() || hn.height != h || !inside_subtree(s, N, t.sub)) continue;
         const HNode& NA = s.hnodes[hn.left];
         const HNode& NB = s.hnodes[hn.right];
         const int ncl = NA.r1 - NA.r0, ncr = NB.r1 - NB.r0, rn = hn.rank;
@@ -857,55 +886,47 @@ void Engine<T>::emit_solve_helpers(int j) {
         dev::GemmJob g;
         g.alpha = 1.0;
         g.beta = 0.0;
-        if (inside(N, hm.right)) {
-          // T = U_N^T W_M[N.A rows]; W_M[N.B rows] -= W_N T
+        dev::GemmJob u;
+        u.B = Tm;
+        u.n = rm;
+        u.k = rn;
+        u.ldb = rn;
+        u.ldc = ld;
+        u.alpha = -1.0;
+        u.beta = 1.0;
+        if (t.isW) {
+          // T = U_N^T D[N.A rows]; D[N.B rows] -= W_N T
           g.A = fp(hn.off_U);
-          g.B = fp(hm.off_W + (NA.r0 - B.r0));
-          g.C = Tm;
+          g.B = fp(t.dst + (NA.r0 - r0));
           g.m = rn;
           g.n = rm;
           g.k = ncl;
           g.lda = ncl;
-          g.ldb = cr;
+          g.ldb = ld;
+          g.C = Tm;
           g.ldc = rn;
           t1.push_back(g);
-          dev::GemmJob u;
           u.A = fp(hn.off_W);
-          u.B = Tm;
-          u.C = fp(hm.off_W + (NB.r0 - B.r0));
+          u.C = fp(t.dst + (NB.r0 - r0));
           u.m = ncr;
-          u.n = rm;
-          u.k = rn;
           u.lda = ncr;
-          u.ldb = rn;
-          u.ldc = cr;
-          u.alpha = -1.0;
-          u.beta = 1.0;
           u1.push_back(u);
-        } else if (inside(N, hm.left)) {
-          // T = V_N^T X_M[N.B rows]; X_M[N.A rows] -= X_N T
+        } else {
+          // T = V_N^T D[N.B rows]; D[N.A rows] -= X_N T
           g.A = fp(hn.off_V);
-          g.B = fp(hm.off_X + (NB.r0 - A.r0));
-          g.C = Tm;
+          g.B = fp(t.dst + (NB.r0 - r0));
           g.m = rn;
           g.n = rm;
           g.k = ncr;
           g.lda = ncr;
-          g.ldb = cl;
+          g.ldb = ld;
+          g.C = Tm;
           g.ldc = rn;
           t2.push_back(g);
-          dev::GemmJob u;
           u.A = fp(hn.off_X);
-          u.B = Tm;
-          u.C = fp(hm.off_X + (NA.r0 - A.r0));
+          u.C = fp(t.dst + (NA.r0 - r0));
           u.m = ncl;
-          u.n = rm;
-          u.k = rn;
           u.lda = ncl;
-          u.ldb = rn;
-          u.ldc = cl;
-          u.alpha = -1.0;
-          u.beta = 1.0;
           u2.push_back(u);
         }
       }
@@ -1094,7 +1115,7 @@ void Engine<T>::build_program() {
   kjobs_.clear();
   vjobs_.clear();
   tasks_.clear();
-  const int64_t welems = qr64_base() + qr64_elems_;
+  const int64_t welems = zp_base() + zp_elems_;
   work_.alloc(std::max<int64_t>(welems, 1) * sizeof(T));
   // Gaussian sketches: the host MT19937 stream, converted on the device
   if (P.rng_count > 0) {
@@ -1162,6 +1183,39 @@ void Engine<T>::build_program() {
     panel_chol(mats);
     emit_gemm(OP_GEMM_NT, syrk, true);
     invert(mats);
+    // Partitioned-inverse form of the uncompressed panels: Zp = L21 Lc^{-1}
+    // replaces L21 once the Schur update has used it.  The forward solve then
+    // needs only Lc^{-1} fv_c and fv_R -= Zp fv_c, both from fv_c; the
+    // transposed solve x_c = Lc^{-T} y_c - Zp^T x_R.
+    {
+      ctx_ = "level " + std::to_string(lv) + " Zp";
+      std::vector<dev::GemmJob> zg;
+      std::vector<dev::CopyJob> zc;
+      int64_t zo = zp_base();
+      for (int j : nodes) {
+        const Supernode& s = P.nodes[j];
+        if (s.compress || s.R() == 0) continue;
+        const int c = s.ncols, R = s.R(), ld = c + R;
+        T* Z = Wb() + zo;
+        zo += (int64_t)R * c;
+        dev::GemmJob g;
+        g.A = F() + s.off_panel + c;  // L21 (R x c)
+        g.B = F() + s.off_panel;      // Lc^{-T} (upper), used transposed
+        g.C = Z;
+        g.m = R;
+        g.n = c;
+        g.k = c;
+        g.lda = g.ldb = ld;
+        g.ldc = R;
+        g.alpha = 1.0;
+        g.beta = 0.0;
+        zg.push_back(g);
+        zc.push_back({Z, F() + s.off_panel + c, R, c, R, ld});
+      }
+      if (zo - zp_base() > zp_elems_) throw std::runtime_error("Zp scratch too small");
+      emit_gemm(OP_GEMM_NT, zg, false);
+      emit_copy(zc);
+    }
     for (int j : nodes)
       if (P.nodes[j].compress) emit_compressed(j);
 
@@ -1282,11 +1336,12 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     dtv_.alloc((size_t)tv_stride_ * k * sizeof(double));
     dpart_.alloc((size_t)part_stride_ * k * sizeof(double));
     dw_.alloc((size_t)n * k * sizeof(double));
+    dz_.alloc((size_t)n * k * sizeof(double));
     fv_k_ = k;
   }
   dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(),
                   dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_,
-                  live};
+                  live, dz_.as<double>()};
   if (use_dataflow_) {
     dev::DfArgs a{};
     a.tasks = ddf_.as<dev::DfTask>();

@@ -19,19 +19,20 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
                     int k, const int* live, hipStream_t s);
 
 struct VecBufs {
-  double* w;
-  double* fv;
-  double* tv;
-  double* part;
+  double* w;        // permuted vector (y, then x), stride n per column
+  double* fv;       // front vectors
+  double* tv;       // rank-sized vectors
+  double* part;     // partial sums of split column tasks
   int64_t n, fvs, tvs, ps;
   const int* live;  // when non-null and *live == 0 every kernel of the apply returns
+  double* z;        // z = L^{-T} y per supernode, stride n per column
 };
 
 // forward: fv_c = w[cols], fv_R = 0, plus children's fv_R via relmap
 void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
                          const int* children, const int* relmap, VecBufs b, int k,
                          int max_front, hipStream_t s);
-// backward: fv_c = w[cols], fv_R[a] = w[rows[a]]
+// backward: fv_R[a] = w[rows[a]], w[cols] = z[cols]
 void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
                        const int* rows, VecBufs b, int k, int max_front, hipStream_t s);
 template <typename T>

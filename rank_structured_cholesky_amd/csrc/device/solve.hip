@@ -72,7 +72,8 @@ __device__ inline double wave_sum(double v) {
 __device__ inline double* vsel(const VecBufs& b, int sel, int col) {
   if (sel == 0) return b.w + (int64_t)col * b.n;
   if (sel == 1) return b.fv + (int64_t)col * b.fvs;
-  return b.tv + (int64_t)col * b.tvs;
+  if (sel == 2) return b.tv + (int64_t)col * b.tvs;
+  return b.z + (int64_t)col * b.n;
 }
 
 // Task descriptors are the same for every lane of a workgroup; copying them
@@ -160,7 +161,8 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
   }
 }
 
-// Backward gather of one chunk: fv_c = w[cols], fv_R[a] = w[rows[a]].
+// Backward gather of one chunk: fv_R[a] = w[rows[a]] (the solution at the
+// node's off-diagonal rows) and w[cols] = z[cols] (start of x_c).
 template <bool COH>
 __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
                              const VecBufs& b, int k) {
@@ -170,9 +172,12 @@ __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict_
   const int* rw = rows + S.rows_off;
   for (int col = 0; col < k; ++col) {
     double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
-    const double* wc = b.w + (int64_t)col * b.n;
-    for (int i = p0 + tid; i < p1; i += 256)
-      vst<COH>(f + i, vld<COH>(wc + (i < c ? S.first + i : rw[i - c])));
+    double* wc = b.w + (int64_t)col * b.n;
+    const double* zc = b.z + (int64_t)col * b.n;
+    for (int i = p0 + tid; i < p1; i += 256) {
+      if (i < c) vst<COH>(wc + S.first + i, vld<COH>(zc + S.first + i));
+      else vst<COH>(f + i, vld<COH>(wc + rw[i - c]));
+    }
   }
 }
 

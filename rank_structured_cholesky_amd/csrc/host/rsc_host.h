@@ -131,6 +131,7 @@ struct Supernode {
   // compressed
   int rank = 0;
   int64_t off_V = -1, off_U = -1;
+  int64_t off_Xoff = -1;        // L_D^{-T} U (ncols x rank), the apply's form of U
   int64_t rng_off = -1;
   std::vector<HNode> hnodes;    // HODLR tree (compressed only); [0] = root
   std::vector<HStep> hsteps;

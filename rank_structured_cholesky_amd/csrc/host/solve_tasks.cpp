@@ -59,260 +59,265 @@ void build_solve_layout(const Plan& P, SolveLayout& L) {
   L.tv_stride = std::max<int64_t>(tvo, 1);
 }
 
-// Level-scheduled form: per level and direction a few phases, each one
-// launch of independent tasks (a phase's split column tasks are followed by a
-// launch of their reductions).
-void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out) {
-  const std::vector<int64_t>& tv_node = L.tv_node;
-  const std::vector<std::vector<int64_t>>& tv_h = L.tv_h;
-  auto fvoff = [&](int j) { return L.fv_off[j]; };
-  std::vector<dev::GvTask> gv;
-  std::vector<std::vector<SolvePhase>>& fwd_ = out.fwd;
-  std::vector<std::vector<SolvePhase>>& bwd_ = out.bwd;
-  fwd_.assign(P.levels.size(), {});
-  bwd_.assign(P.levels.size(), {});
-  const int W = 0, FV = 1, TV = 2;
-  // COL tasks: 32 columns; more than CHUNK rows are split into row chunks
-  // writing partial sums, reduced by a second launch (`red`).
-  const int CHUNK = 512;
-  std::vector<dev::GvTask> red;
+// ---- per-node stages ------------------------------------------------------
+//
+// The factor is held in partitioned-inverse form: an uncompressed panel
+// stores [Lc^{-T} (upper); Zp = L21 Lc^{-1}], a compressed node its HODLR
+// diagonal (leaves L^{-T}, W/X helpers), V and X_off = L_D^{-T} U.  Per
+// supernode (b = front vector fv = [fv_c; fv_R], w = permuted vector,
+// z = L^{-T} y):
+//   A  (forward, towards the parent)
+//      A0  y_c = Lc^{-1} fv_c  (uncompressed: COL on Lc^{-T}; compressed:
+//          HODLR leaves), fv_R -= Zp fv_c (ROW) or tv = X_off^T fv_c (COL)
+//      A1  compressed: fv_R -= V tv (ROW)
+//   B  (forward, off the path to the parent) compressed: HODLR heights of
+//      y_c; then z_c = Lc^{-T} y_c (ROW on the stored upper inverse; HODLR
+//      leaves and heights with V_M / X_M)
+//   C  (backward, after the gather fv_R = x[rows], x_c = z_c)
+//      uncompressed: x_c -= Zp^T fv_R (COL); compressed: tv = V^T fv_R (COL),
+//      then x_c -= X_off tv (ROW)
+namespace {
+
+struct Mv {
+  bool col;
+  int64_t mat;
+  int ld, rows, cols, op, xs;
+  int64_t xo;
+  int ys;
+  int64_t yo;
+  bool upper;
+};
+using Stage = std::vector<Mv>;
+
+struct NodeStages {
+  std::vector<Stage> A, B, C;
+};
+
+constexpr int W = 0, FV = 1, TV = 2, ZB = 3;
+
+NodeStages node_stages(const Plan& P, const SolveLayout& L, int j) {
+  const Supernode& s = P.nodes[j];
+  const int64_t fo = L.fv_off[j];
+  const int c = s.ncols, R = s.R();
+  NodeStages ns;
+  int maxh = 0;
+  for (auto& h : s.hnodes) maxh = std::max(maxh, h.height);
+  if (!s.compress) {
+    Stage a0;
+    a0.push_back({true, s.off_panel, c + R, c, c, 0, FV, fo, W, s.first, true});
+    if (R > 0) a0.push_back({false, s.off_panel + c, c + R, R, c, 1, FV, fo, FV, fo + c, false});
+    ns.A.push_back(a0);
+    ns.B.push_back({{false, s.off_panel, c + R, c, c, 0, W, s.first, ZB, s.first, true}});
+    if (R > 0)
+      ns.C.push_back({{true, s.off_panel + c, c + R, R, c, 1, FV, fo + c, W, s.first, false}});
+    return ns;
+  }
+  const int64_t tvn = L.tv_node[j];
+  const std::vector<int64_t>& tvh = L.tv_h[j];
+  Stage a0;
+  for (auto& h : s.hnodes)
+    if (h.leaf())
+      a0.push_back({true, h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fo + h.r0,
+                    W, s.first + h.r0, true});
+  if (R > 0 && s.rank > 0) a0.push_back({true, s.off_Xoff, c, c, s.rank, 0, FV, fo, TV, tvn, false});
+  ns.A.push_back(a0);
+  if (R > 0 && s.rank > 0)
+    ns.A.push_back({{false, s.off_V, R, R, s.rank, 1, TV, tvn, FV, fo + c, false}});
+  // y_c: HODLR heights (forward)
+  for (int hh = 1; hh <= maxh; ++hh) {
+    Stage c1, c2;
+    for (size_t q = 0; q < s.hnodes.size(); ++q) {
+      const HNode& h = s.hnodes[q];
+      if (h.leaf() || h.height != hh) continue;
+      const HNode& A = s.hnodes[h.left];
+      const HNode& B = s.hnodes[h.right];
+      c1.push_back({true, h.off_U, A.r1 - A.r0, A.r1 - A.r0, h.rank, 0, W, s.first + A.r0, TV,
+                    tvh[q], false});
+      c2.push_back({false, h.off_W, B.r1 - B.r0, B.r1 - B.r0, h.rank, 1, TV, tvh[q], W,
+                    s.first + B.r0, false});
+    }
+    ns.B.push_back(c1);
+    ns.B.push_back(c2);
+  }
+  // z_c = L_D^{-T} y_c: leaves, then heights (transposed)
+  Stage zl;
+  for (auto& h : s.hnodes)
+    if (h.leaf())
+      zl.push_back({false, h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, W,
+                    s.first + h.r0, ZB, s.first + h.r0, true});
+  ns.B.push_back(zl);
+  for (int hh = 1; hh <= maxh; ++hh) {
+    Stage c1, c2;
+    for (size_t q = 0; q < s.hnodes.size(); ++q) {
+      const HNode& h = s.hnodes[q];
+      if (h.leaf() || h.height != hh) continue;
+      const HNode& A = s.hnodes[h.left];
+      const HNode& B = s.hnodes[h.right];
+      c1.push_back({true, h.off_V, B.r1 - B.r0, B.r1 - B.r0, h.rank, 0, ZB, s.first + B.r0, TV,
+                    tvh[q], false});
+      c2.push_back({false, h.off_X, A.r1 - A.r0, A.r1 - A.r0, h.rank, 1, TV, tvh[q], ZB,
+                    s.first + A.r0, false});
+    }
+    ns.B.push_back(c1);
+    ns.B.push_back(c2);
+  }
+  if (R > 0 && s.rank > 0) {
+    ns.C.push_back({{true, s.off_V, R, R, s.rank, 0, FV, fo + c, TV, tvn, false}});
+    ns.C.push_back({{false, s.off_Xoff, c, c, s.rank, 1, TV, tvn, W, s.first, false}});
+  }
+  return ns;
+}
+
+// Matrix-vector specs -> GvTasks.  COL: 32 columns per task, more than CHUNK
+// rows split into partial sums (slot per chunk) plus one reduction task;
+// ROW: 64 rows per task.
+struct TaskEmitter {
   int64_t slot = 0;
-  auto col = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
-                 int ys, int64_t yo, bool upper) {
-    for (int j0 = 0; j0 < cols; j0 += 32) {
-      const int rr = upper ? std::min(rows, j0 + 32) : rows;
-      if (rr <= 0) continue;
-      dev::GvTask t{};
-      t.ld = ld;
-      t.cols = std::min(32, cols - j0);
-      t.kind = 1;
-      t.op = op;
-      t.xsel = xs;
-      t.ysel = ys;
-      t.yoff = yo + j0;
-      if (rr <= CHUNK) {
-        t.mat = mat + (int64_t)j0 * ld;
-        t.rows = rr;
-        t.xoff = xo;
+  static constexpr int CHUNK = 512;
+  void emit(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
+    if (m.col) {
+      for (int j0 = 0; j0 < m.cols; j0 += 32) {
+        const int rr = m.upper ? std::min(m.rows, j0 + 32) : m.rows;
+        if (rr <= 0) continue;
+        dev::GvTask t{};
+        t.ld = m.ld;
+        t.cols = std::min(32, m.cols - j0);
+        t.kind = 1;
+        t.op = m.op;
+        t.xsel = m.xs;
+        t.ysel = m.ys;
+        t.yoff = m.yo + j0;
+        if (rr <= CHUNK) {
+          t.mat = m.mat + (int64_t)j0 * m.ld;
+          t.rows = rr;
+          t.xoff = m.xo;
+          t.part = -1;
+          main.push_back(t);
+          continue;
+        }
+        const int nsub = (rr + CHUNK - 1) / CHUNK;
+        for (int s2 = 0; s2 < nsub; ++s2) {
+          dev::GvTask u = t;
+          u.mat = m.mat + (int64_t)j0 * m.ld + (int64_t)s2 * CHUNK;
+          u.rows = std::min(CHUNK, rr - s2 * CHUNK);
+          u.xoff = m.xo + (int64_t)s2 * CHUNK;
+          u.part = (int)(slot + s2);
+          main.push_back(u);
+        }
+        dev::GvTask rd{};
+        rd.kind = 2;
+        rd.part = (int)slot;
+        rd.rows = nsub;
+        rd.cols = t.cols;
+        rd.op = m.op;
+        rd.ysel = m.ys;
+        rd.yoff = m.yo + j0;
+        red.push_back(rd);
+        slot += nsub;
+      }
+    } else {
+      for (int i0 = 0; i0 < m.rows; i0 += 64) {
+        dev::GvTask t{};
+        t.mat = m.mat + i0 + (m.upper ? (int64_t)i0 * m.ld : 0);
+        t.ld = m.ld;
+        t.rows = std::min(64, m.rows - i0);
+        t.cols = m.upper ? m.cols - i0 : m.cols;
+        t.kind = 0;
+        t.op = m.op;
+        t.xsel = m.xs;
+        t.ysel = m.ys;
         t.part = -1;
-        gv.push_back(t);
-        continue;
+        t.xoff = m.xo + (m.upper ? i0 : 0);
+        t.yoff = m.yo + i0;
+        main.push_back(t);
       }
-      const int nsub = (rr + CHUNK - 1) / CHUNK;
-      for (int s2 = 0; s2 < nsub; ++s2) {
-        dev::GvTask u = t;
-        u.mat = mat + (int64_t)j0 * ld + (int64_t)s2 * CHUNK;
-        u.rows = std::min(CHUNK, rr - s2 * CHUNK);
-        u.xoff = xo + (int64_t)s2 * CHUNK;
-        u.part = (int)(slot + s2);
-        gv.push_back(u);
+    }
+  }
+};
+
+}  // namespace
+
+// Level-scheduled form: per level and direction a sequence of phases, each
+// one launch of independent tasks; stage k of every node of the level runs
+// in phase k (a phase's split column tasks are followed by a launch of their
+// reductions).
+void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out) {
+  std::vector<dev::GvTask> gv;
+  out.fwd.assign(P.levels.size(), {});
+  out.bwd.assign(P.levels.size(), {});
+  TaskEmitter em;
+  auto phases = [&](std::vector<SolvePhase>& ph, const std::vector<std::vector<Stage>>& per) {
+    size_t nst = 0;
+    for (auto& v : per) nst = std::max(nst, v.size());
+    for (size_t k = 0; k < nst; ++k) {
+      std::vector<dev::GvTask> main, red;
+      for (auto& v : per)
+        if (k < v.size())
+          for (const Mv& m : v[k]) em.emit(m, main, red);
+      if (!main.empty()) {
+        ph.push_back({2, (int)gv.size(), (int)main.size()});
+        gv.insert(gv.end(), main.begin(), main.end());
       }
-      dev::GvTask rd{};
-      rd.kind = 2;
-      rd.part = (int)slot;
-      rd.rows = nsub;
-      rd.cols = t.cols;
-      rd.op = op;
-      rd.ysel = ys;
-      rd.yoff = yo + j0;
-      red.push_back(rd);
-      slot += nsub;
-    }
-  };
-  // ROW tasks: 64 rows, all columns
-  auto row = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo,
-                 int ys, int64_t yo, bool upper) {
-    for (int i0 = 0; i0 < rows; i0 += 64) {
-      dev::GvTask t{};
-      t.mat = mat + i0 + (upper ? (int64_t)i0 * ld : 0);
-      t.ld = ld;
-      t.rows = std::min(64, rows - i0);
-      t.cols = upper ? cols - i0 : cols;
-      t.kind = 0;
-      t.op = op;
-      t.xsel = xs;
-      t.ysel = ys;
-      t.part = -1;
-      t.xoff = xo + (upper ? i0 : 0);
-      t.yoff = yo + i0;
-      gv.push_back(t);
-    }
-  };
-  // a phase = small tasks (kind 2 launch) + large tasks (kind 3 launch),
-  // then the partial-sum reductions of its split column tasks
-  auto is_big = [](const dev::GvTask& t) {
-    return t.kind == 0 ? t.cols > 64 : t.rows > 128;
-  };
-  auto phase = [&](std::vector<SolvePhase>& out, size_t start) {
-    if (gv.size() > start) {
-      std::stable_partition(gv.begin() + start, gv.end(),
-                            [&](const dev::GvTask& t) { return !is_big(t); });
-      size_t mid = start;
-      while (mid < gv.size() && !is_big(gv[mid])) ++mid;
-      if (mid > start) out.push_back({2, (int)start, (int)(mid - start)});
-      if (gv.size() > mid) out.push_back({3, (int)mid, (int)(gv.size() - mid)});
-    }
-    if (!red.empty()) {
-      const size_t r0 = gv.size();
-      gv.insert(gv.end(), red.begin(), red.end());
-      red.clear();
-      out.push_back({2, (int)r0, (int)(gv.size() - r0)});
+      if (!red.empty()) {
+        ph.push_back({2, (int)gv.size(), (int)red.size()});
+        gv.insert(gv.end(), red.begin(), red.end());
+      }
     }
   };
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
-    const std::vector<int>& nodes = P.levels[lv];
-    int maxh = 0;
-    for (int j : nodes)
-      for (auto& h : P.nodes[j].hnodes) maxh = std::max(maxh, h.height);
-    // ---- forward
-    std::vector<SolvePhase>& F = fwd_[lv];
-    F.push_back({0, 0, 0});
-    size_t st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      const int64_t fo = 0;
-      (void)fo;
-      if (!s.compress) {
-        col(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
-      } else {
-        for (auto& h : s.hnodes)
-          if (h.leaf())
-            col(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
-                s.first + h.r0, true);
-      }
+    std::vector<std::vector<Stage>> A, B, C;
+    for (int j : P.levels[lv]) {
+      NodeStages ns = node_stages(P, L, j);
+      A.push_back(ns.A);
+      B.push_back(ns.B);
+      C.push_back(ns.C);
     }
-    phase(F, st);
-    for (int hh = 1; hh <= maxh; ++hh) {
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& A = s.hnodes[h.left];
-          col(h.off_U, A.r1 - A.r0, A.r1 - A.r0, h.rank, 0, W, s.first + A.r0, TV, tv_h[j][q],
-              false);
-        }
-      }
-      phase(F, st);
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& B = s.hnodes[h.right];
-          row(h.off_W, B.r1 - B.r0, B.r1 - B.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + B.r0,
-              false);
-        }
-      }
-      phase(F, st);
-    }
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.compress && s.R() > 0)
-        col(s.off_U, s.ncols, s.ncols, s.rank, 0, W, s.first, TV, tv_node[j], false);
-    }
-    phase(F, st);
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.R() == 0) continue;
-      if (!s.compress)
-        row(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, W, s.first, FV,
-            fvoff(j) + s.ncols, false);
-      else
-        row(s.off_V, s.R(), s.R(), s.rank, 1, TV, tv_node[j], FV, fvoff(j) + s.ncols, false);
-    }
-    phase(F, st);
-    // ---- backward
-    std::vector<SolvePhase>& Bk = bwd_[lv];
-    Bk.push_back({1, 0, 0});
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.R() == 0) continue;
-      if (!s.compress)
-        col(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, FV, fvoff(j) + s.ncols,
-            FV, fvoff(j), false);
-      else
-        col(s.off_V, s.R(), s.R(), s.rank, 0, FV, fvoff(j) + s.ncols, TV, tv_node[j], false);
-    }
-    phase(Bk, st);
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.compress && s.R() > 0)
-        row(s.off_U, s.ncols, s.ncols, s.rank, 1, TV, tv_node[j], FV, fvoff(j), false);
-    }
-    phase(Bk, st);
-    st = gv.size();
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (!s.compress) {
-        row(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
-      } else {
-        for (auto& h : s.hnodes)
-          if (h.leaf())
-            row(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
-                s.first + h.r0, true);
-      }
-    }
-    phase(Bk, st);
-    for (int hh = 1; hh <= maxh; ++hh) {
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& B = s.hnodes[h.right];
-          col(h.off_V, B.r1 - B.r0, B.r1 - B.r0, h.rank, 0, W, s.first + B.r0, TV, tv_h[j][q],
-              false);
-        }
-      }
-      phase(Bk, st);
-      st = gv.size();
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& A = s.hnodes[h.left];
-          row(h.off_X, A.r1 - A.r0, A.r1 - A.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + A.r0,
-              false);
-        }
-      }
-      phase(Bk, st);
-    }
+    out.fwd[lv].push_back({0, 0, 0});
+    phases(out.fwd[lv], A);
+    phases(out.fwd[lv], B);
+    out.bwd[lv].push_back({1, 0, 0});
+    phases(out.bwd[lv], C);
   }
+  // large tasks use the high-MLP kernel
+  for (auto* dir : {&out.fwd, &out.bwd})
+    for (auto& lv : *dir) {
+      std::vector<SolvePhase> sp;
+      for (const SolvePhase& p : lv) {
+        if (p.kind < 2) {
+          sp.push_back(p);
+          continue;
+        }
+        auto big = [&](const dev::GvTask& t) {
+          return t.kind == 0 ? t.cols > 64 : (t.kind == 1 && t.rows > 128);
+        };
+        std::stable_partition(gv.begin() + p.first, gv.begin() + p.first + p.count,
+                              [&](const dev::GvTask& t) { return !big(t); });
+        int mid = p.first;
+        while (mid < p.first + p.count && !big(gv[mid])) ++mid;
+        if (mid > p.first) sp.push_back({2, p.first, mid - p.first});
+        if (p.first + p.count > mid) sp.push_back({3, mid, p.first + p.count - mid});
+      }
+      lv.swap(sp);
+    }
   out.gv.swap(gv);
-  out.slots = slot;
+  out.slots = em.slot;
 }
 
-// Dataflow form of the same solves: per supernode a chain of stages
-// (forward: assembly, diagonal leaves, HODLR heights, off-diagonal; backward:
-// gather, off-diagonal, diagonal leaves, HODLR heights), each stage one
-// group of tasks waiting on the previous stage of the same node.  A
-// node's forward assembly waits on its children's last forward stage; its
-// backward gather waits on its parent's last backward stage (on its own
-// forward for a root).  The scatter waits on every node's last backward
-// stage.  Tasks are emitted level by level, so the list is in topological
-// order.
+// Dataflow form: per supernode three chains of groups (A towards the
+// parent, B for y and z, C backward).  A node's assembly waits on its
+// children's last A group; chain B starts after A0; the backward gather
+// waits on the parent's last C group (nothing for a root) and on the node's
+// own chain B.  The scatter waits on every node's last C group.  Tasks are
+// emitted level by level, so the list is in topological order.
 void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out) {
-  const std::vector<int64_t>& tv_node = L.tv_node;
-  const std::vector<std::vector<int64_t>>& tv_h = L.tv_h;
-  auto fvoff = [&](int j) { return L.fv_off[j]; };
   const int ns = (int)P.nodes.size();
-  const int W = 0, FV = 1, TV = 2;
-  const int CHUNK = 512, VCH = kSolveChunk, SCH = kScatterChunk;
-  std::vector<dev::GvTask> gv, cur, red;
+  const int VCH = kSolveChunk, SCH = kScatterChunk;
+  std::vector<dev::GvTask> gv;
   std::vector<dev::DfTask> tasks;
   std::vector<unsigned> need;
   std::vector<int> gwoff{0}, gw;
-  std::vector<int> last_f(ns, -1), last_b(ns, -1);
-  int64_t slot = 0;
+  std::vector<int> lastA(ns, -1), lastB(ns, -1), lastC(ns, -1), a0(ns, -1);
+  TaskEmitter em;
   auto new_group = [&](const std::vector<int>& waits) {
     need.push_back(0u);
     gw.insert(gw.end(), waits.begin(), waits.end());
@@ -329,72 +334,15 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     tasks.push_back(t);
     need[g] += 1u;
   };
-  auto col = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo, int ys,
-                 int64_t yo, bool upper) {
-    for (int j0 = 0; j0 < cols; j0 += 32) {
-      const int rr = upper ? std::min(rows, j0 + 32) : rows;
-      if (rr <= 0) continue;
-      dev::GvTask t{};
-      t.ld = ld;
-      t.cols = std::min(32, cols - j0);
-      t.kind = 1;
-      t.op = op;
-      t.xsel = xs;
-      t.ysel = ys;
-      t.yoff = yo + j0;
-      if (rr <= CHUNK) {
-        t.mat = mat + (int64_t)j0 * ld;
-        t.rows = rr;
-        t.xoff = xo;
-        t.part = -1;
-        cur.push_back(t);
-        continue;
-      }
-      const int nsub = (rr + CHUNK - 1) / CHUNK;
-      for (int s2 = 0; s2 < nsub; ++s2) {
-        dev::GvTask u = t;
-        u.mat = mat + (int64_t)j0 * ld + (int64_t)s2 * CHUNK;
-        u.rows = std::min(CHUNK, rr - s2 * CHUNK);
-        u.xoff = xo + (int64_t)s2 * CHUNK;
-        u.part = (int)(slot + s2);
-        cur.push_back(u);
-      }
-      dev::GvTask rd{};
-      rd.kind = 2;
-      rd.part = (int)slot;
-      rd.rows = nsub;
-      rd.cols = t.cols;
-      rd.op = op;
-      rd.ysel = ys;
-      rd.yoff = yo + j0;
-      red.push_back(rd);
-      slot += nsub;
-    }
-  };
-  auto row = [&](int64_t mat, int ld, int rows, int cols, int op, int xs, int64_t xo, int ys,
-                 int64_t yo, bool upper) {
-    for (int i0 = 0; i0 < rows; i0 += 64) {
-      dev::GvTask t{};
-      t.mat = mat + i0 + (upper ? (int64_t)i0 * ld : 0);
-      t.ld = ld;
-      t.rows = std::min(64, rows - i0);
-      t.cols = upper ? cols - i0 : cols;
-      t.kind = 0;
-      t.op = op;
-      t.xsel = xs;
-      t.ysel = ys;
-      t.part = -1;
-      t.xoff = xo + (upper ? i0 : 0);
-      t.yoff = yo + i0;
-      cur.push_back(t);
-    }
-  };
-  // close a matrix-vector stage of node j: its tasks, then its reductions
-  auto stage = [&](int j, bool fwd) {
-    int& last = fwd ? last_f[j] : last_b[j];
-    if (!cur.empty()) {
-      const int g = new_group({last});
-      for (const dev::GvTask& t : cur) {
+  // one stage: its tasks wait on `after`; returns the last group (reductions
+  // after the main tasks), or `after` when the stage is empty
+  auto stage = [&](const Stage& st, int after) {
+    std::vector<dev::GvTask> main, red;
+    for (const Mv& m : st) em.emit(m, main, red);
+    int last = after;
+    if (!main.empty()) {
+      const int g = new_group(after >= 0 ? std::vector<int>{after} : std::vector<int>{});
+      for (const dev::GvTask& t : main) {
         gv.push_back(t);
         add(2, (int)gv.size() - 1, 0, g);
       }
@@ -408,149 +356,56 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
       }
       last = g;
     }
-    cur.clear();
-    red.clear();
+    return last;
   };
-  auto chunks = [&](int j) {
-    return (P.nodes[j].ncols + P.nodes[j].R() + VCH - 1) / VCH;
-  };
-  // ---- forward, bottom-up
+  auto chunks = [&](int j) { return (P.nodes[j].ncols + P.nodes[j].R() + VCH - 1) / VCH; };
+  std::vector<NodeStages> st(ns);
+  for (int j = 0; j < ns; ++j) st[j] = node_stages(P, L, j);
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
     const std::vector<int>& nodes = P.levels[lv];
-    int maxh = 0;
-    for (int j : nodes)
-      for (auto& h : P.nodes[j].hnodes) maxh = std::max(maxh, h.height);
     for (int j : nodes) {
       std::vector<int> waits;
-      for (int c : P.nodes[j].children) waits.push_back(last_f[c]);
+      for (int c : P.nodes[j].children) waits.push_back(lastA[c]);
       const int g = new_group(waits);
       for (int c = 0; c < chunks(j); ++c) add(0, j, c, g);
-      last_f[j] = g;
+      lastA[j] = g;
     }
+    size_t nA = 0, nB = 0;
     for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (!s.compress) {
-        col(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
-      } else {
-        for (auto& h : s.hnodes)
-          if (h.leaf())
-            col(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
-                s.first + h.r0, true);
-      }
-      stage(j, true);
+      nA = std::max(nA, st[j].A.size());
+      nB = std::max(nB, st[j].B.size());
     }
-    for (int hh = 1; hh <= maxh; ++hh) {
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& A = s.hnodes[h.left];
-          col(h.off_U, A.r1 - A.r0, A.r1 - A.r0, h.rank, 0, W, s.first + A.r0, TV, tv_h[j][q],
-              false);
+    for (size_t k = 0; k < nA; ++k)
+      for (int j : nodes)
+        if (k < st[j].A.size()) {
+          lastA[j] = stage(st[j].A[k], lastA[j]);
+          if (k == 0) a0[j] = lastA[j];
         }
-        stage(j, true);
-      }
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& B = s.hnodes[h.right];
-          row(h.off_W, B.r1 - B.r0, B.r1 - B.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + B.r0,
-              false);
-        }
-        stage(j, true);
-      }
-    }
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.compress && s.R() > 0)
-        col(s.off_U, s.ncols, s.ncols, s.rank, 0, W, s.first, TV, tv_node[j], false);
-      stage(j, true);
-    }
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.R() > 0) {
-        if (!s.compress)
-          row(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, W, s.first, FV,
-              fvoff(j) + s.ncols, false);
-        else
-          row(s.off_V, s.R(), s.R(), s.rank, 1, TV, tv_node[j], FV, fvoff(j) + s.ncols, false);
-      }
-      stage(j, true);
-    }
+    for (int j : nodes) lastB[j] = a0[j] >= 0 ? a0[j] : lastA[j];
+    for (size_t k = 0; k < nB; ++k)
+      for (int j : nodes)
+        if (k < st[j].B.size()) lastB[j] = stage(st[j].B[k], lastB[j]);
   }
-  // ---- backward, top-down
   for (int lv = (int)P.levels.size() - 1; lv >= 0; --lv) {
     const std::vector<int>& nodes = P.levels[lv];
-    int maxh = 0;
-    for (int j : nodes)
-      for (auto& h : P.nodes[j].hnodes) maxh = std::max(maxh, h.height);
     for (int j : nodes) {
       const int par = P.nodes[j].parent;
-      const int g = new_group({par >= 0 ? last_b[par] : last_f[j]});
+      std::vector<int> waits = {lastB[j]};
+      if (par >= 0) waits.push_back(lastC[par]);
+      const int g = new_group(waits);
       for (int c = 0; c < chunks(j); ++c) add(1, j, c, g);
-      last_b[j] = g;
+      lastC[j] = g;
     }
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.R() > 0) {
-        if (!s.compress)
-          col(s.off_panel + s.ncols, s.ncols + s.R(), s.R(), s.ncols, 1, FV,
-              fvoff(j) + s.ncols, FV, fvoff(j), false);
-        else
-          col(s.off_V, s.R(), s.R(), s.rank, 0, FV, fvoff(j) + s.ncols, TV, tv_node[j], false);
-      }
-      stage(j, false);
-    }
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (s.compress && s.R() > 0)
-        row(s.off_U, s.ncols, s.ncols, s.rank, 1, TV, tv_node[j], FV, fvoff(j), false);
-      stage(j, false);
-    }
-    for (int j : nodes) {
-      const Supernode& s = P.nodes[j];
-      if (!s.compress) {
-        row(s.off_panel, s.ncols + s.R(), s.ncols, s.ncols, 0, FV, fvoff(j), W, s.first, true);
-      } else {
-        for (auto& h : s.hnodes)
-          if (h.leaf())
-            row(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fvoff(j) + h.r0, W,
-                s.first + h.r0, true);
-      }
-      stage(j, false);
-    }
-    for (int hh = 1; hh <= maxh; ++hh) {
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& B = s.hnodes[h.right];
-          col(h.off_V, B.r1 - B.r0, B.r1 - B.r0, h.rank, 0, W, s.first + B.r0, TV, tv_h[j][q],
-              false);
-        }
-        stage(j, false);
-      }
-      for (int j : nodes) {
-        const Supernode& s = P.nodes[j];
-        for (size_t q = 0; q < s.hnodes.size(); ++q) {
-          const HNode& h = s.hnodes[q];
-          if (h.leaf() || h.height != hh) continue;
-          const HNode& A = s.hnodes[h.left];
-          row(h.off_X, A.r1 - A.r0, A.r1 - A.r0, h.rank, 1, TV, tv_h[j][q], W, s.first + A.r0,
-              false);
-        }
-        stage(j, false);
-      }
-    }
+    size_t nC = 0;
+    for (int j : nodes) nC = std::max(nC, st[j].C.size());
+    for (size_t k = 0; k < nC; ++k)
+      for (int j : nodes)
+        if (k < st[j].C.size()) lastC[j] = stage(st[j].C[k], lastC[j]);
   }
-  // ---- scatter after every node's last backward stage
+  // scatter after every node's last backward group
   const int ready = new_group({});
   std::vector<char> final_group(need.size(), 0);
-  for (int j = 0; j < ns; ++j) final_group[last_b[j]] = 1;
+  for (int j = 0; j < ns; ++j) final_group[lastC[j]] = 1;
   for (auto& t : tasks)
     if (final_group[t.group]) {
       t.group2 = ready;
@@ -575,7 +430,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   out.need.swap(need);
   out.gwoff.swap(gwoff);
   out.gw.swap(gw);
-  out.slots = slot;
+  out.slots = em.slot;
 }
 
 }  // namespace rsc

@@ -251,6 +251,8 @@ void replan_ranks(Plan& plan, double diag_const) {
       foff += R * s.rank;
       s.off_U = foff;
       foff += c * s.rank;
+      s.off_Xoff = foff;
+      foff += c * s.rank;
       s.rng_off = rng;
       rng += R * s.rank;
       double r = s.rank;
@@ -260,7 +262,7 @@ void replan_ranks(Plan& plan, double diag_const) {
       apply += 2.0 * r * (R + c);
     } else {
       s.rank = 0;
-      s.off_V = s.off_U = -1;
+      s.off_V = s.off_U = s.off_Xoff = -1;
       s.rng_off = -1;
     }
   }

@@ -63,7 +63,7 @@ struct State {
   const Plan* P;
   const SolveLayout* L;
   Buf F{{}, "factor"}, B{{}, "B"}, X{{}, "X"}, w{{}, "w"}, fv{{}, "fv"}, tv{{}, "tv"},
-      part{{}, "part"};
+      part{{}, "part"}, z{{}, "z"};
   int n = 0, k = 1;
   int64_t ps = 1;
   std::vector<int> perm;
@@ -73,10 +73,11 @@ struct State {
     std::fill(tv.v.begin(), tv.v.end(), std::nan(""));
     std::fill(part.v.begin(), part.v.end(), std::nan(""));
     std::fill(X.v.begin(), X.v.end(), std::nan(""));
+    std::fill(z.v.begin(), z.v.end(), std::nan(""));
   }
-  Buf& vec(int sel) { return sel == 0 ? w : sel == 1 ? fv : tv; }
+  Buf& vec(int sel) { return sel == 0 ? w : sel == 1 ? fv : sel == 2 ? tv : z; }
   int64_t stride(int sel) const {
-    return sel == 0 ? n : sel == 1 ? L->fv_stride : L->tv_stride;
+    return sel == 0 || sel == 3 ? n : sel == 1 ? L->fv_stride : L->tv_stride;
   }
 };
 
@@ -118,9 +119,12 @@ void gather(State& S, int j, int chunk) {
   if (p0 >= c + R) return;
   for (int col = 0; col < S.k; ++col)
     for (int i = p0; i < p1; ++i) {
-      const int src = i < c ? N.first + i : chk(S.L->rows, N.rows_off + i - c, "rows");
-      S.fv.at(N.fv_off + (int64_t)col * S.L->fv_stride + i) =
-          S.w.at(src + (int64_t)col * S.n);
+      if (i < c) {
+        S.w.at(N.first + i + (int64_t)col * S.n) = S.z.at(N.first + i + (int64_t)col * S.n);
+      } else {
+        const int src = chk(S.L->rows, N.rows_off + i - c, "rows");
+        S.fv.at(N.fv_off + (int64_t)col * S.L->fv_stride + i) = S.w.at(src + (int64_t)col * S.n);
+      }
     }
 }
 
@@ -233,6 +237,7 @@ int main(int argc, char** argv) {
   S.fv.v.resize((size_t)L.fv_stride * k);
   S.tv.v.resize((size_t)L.tv_stride * k);
   S.part.v.resize((size_t)S.ps * k);
+  S.z.v.resize((size_t)S.n * k);
 
   // (1) level-scheduled order
   S.reset_work();

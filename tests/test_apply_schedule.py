@@ -1,0 +1,35 @@
+"""The apply's task lists on the CPU (tests/cpp/emulate_apply.cpp): the
+dataflow list run in ticket order and on threads gives bitwise the result of
+the level-scheduled list, every wait is satisfied when its task starts, and
+every buffer access is in range."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "emulate_apply")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("rsc_cpp_test_build",
+                                                  os.path.join(HERE, "cpp", "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.build()
+
+
+@pytest.mark.parametrize("args", [
+    "p3d 12 20 40 1 8",      # compressed separators, HODLR diagonals
+    "p3d 16 63 64 2 8",      # two right-hand sides
+    "p2d 40 10 16 1 8",      # 2D, many compressed levels
+    "el 5 30 40 1 8",        # elasticity (3 dof per node)
+    "p2d 48 1000000 400 1 8",  # no compression
+    "p3d 24 63 128 3 8",     # split column tasks (partial sums)
+])
+def test_dataflow_schedule_matches_level_schedule(args):
+    r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("ok")

@@ -113,11 +113,30 @@ class Engine final : public FactorEngine {
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
   DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_, dz_;
   std::vector<std::vector<SolvePhase>> fwd_, bwd_;
+  std::vector<std::vector<int>> level_nodes_;  // per level, nodes outside subtrees
   int64_t tv_stride_ = 0, ntasks_ = 0, part_stride_ = 1;
   DBuf dpart_;
   std::vector<int64_t> fv_off_;
   // dataflow apply (one persistent launch per application)
-  DBuf ddf_, ddfgv_, dgwoff_, dgw_, dneed_, dcnt_, dapst_;
+  DBuf ddf_, ddfgv_, dgwoff_, dgw_, dneed_, dcnt_, dapst_, dtrace_;
+  DBuf dstptr_, dstnodes_, dpanel_;
+  int nsub_ = 0;
+  dev::SubtreeArgs subtree_args(const double* B, int ldb) const {
+    dev::SubtreeArgs a{};
+    a.ptr = dstptr_.as<int>();
+    a.nodes = dstnodes_.as<int>();
+    a.panel = dpanel_.as<int64_t>();
+    a.vn = dvnodes_.as<dev::VecNode>();
+    a.children = dchildren_.as<int>();
+    a.relmap = drelmap_.as<int>();
+    a.rows = drows_.as<int>();
+    a.perm = dperm_.as<int>();
+    a.B = B;
+    a.ldb = ldb;
+    return a;
+  }
+  std::vector<dev::DfTask> df_host_;  // kept for the optional timing trace
+  std::string df_trace_path_;
   int df_ntasks_ = 0, df_ngroups_ = 0, df_grid_ = 1;
   unsigned long long df_timeout_ = 0;
   bool use_dataflow_ = false;
@@ -1030,7 +1049,16 @@ void Engine<T>::setup_static() {
   da_src_.upload(L.src, st_);
   // solve: vector layout, level-scheduled and dataflow task lists
   SolveLayout SL;
-  build_solve_layout(P, SL);
+  {
+    // subtree tasks: about 4096 subtrees' worth of panel storage each
+    int64_t lim = std::max<int64_t>(16384, P.factor_elems / 4096);
+    if (const char* e = std::getenv("RSC_SUBTREE_ELEMS")) lim = std::atoll(e);
+    build_solve_layout(P, SL, lim);
+  }
+  nsub_ = (int)SL.sub.root.size();
+  dstptr_.upload(SL.sub.ptr, st_);
+  dstnodes_.upload(SL.sub.nodes.empty() ? std::vector<int>{0} : SL.sub.nodes, st_);
+  dpanel_.upload(SL.panel, st_);
   dchildren_.upload(SL.children, st_);
   drelmap_.upload(SL.relmap, st_);
   drows_.upload(SL.rows, st_);
@@ -1045,6 +1073,7 @@ void Engine<T>::setup_static() {
     build_level_tasks(P, SL, LT);
     fwd_.swap(LT.fwd);
     bwd_.swap(LT.bwd);
+    level_nodes_.swap(LT.level_nodes);
     dgv_.upload(LT.gv, st_);
     ntasks_ = (int64_t)LT.gv.size();
     DataflowTasks DT;
@@ -1052,6 +1081,12 @@ void Engine<T>::setup_static() {
     df_ntasks_ = (int)DT.tasks.size();
     df_ngroups_ = (int)DT.need.size();
     ddf_.upload(DT.tasks, st_);
+    const char* tp = std::getenv("RSC_DF_TRACE");
+    if (tp && tp[0]) {
+      df_trace_path_ = tp;
+      df_host_ = DT.tasks;
+      dtrace_.alloc((size_t)4 * DT.tasks.size() * sizeof(unsigned long long));
+    }
     ddfgv_.upload(DT.gv, st_);
     dgwoff_.upload(DT.gwoff, st_);
     dgw_.upload(DT.gw, st_);
@@ -1075,17 +1110,18 @@ void Engine<T>::setup_static() {
   }
   std::vector<int> levnodes;
   lev_off_.clear();
-  for (auto& lv : P.levels) {
+  for (auto& lv : level_nodes_) {
     lev_off_.push_back((int)levnodes.size());
     levnodes.insert(levnodes.end(), lv.begin(), lv.end());
   }
   lev_off_.push_back((int)levnodes.size());
   lev_front_.clear();
-  for (auto& lv : P.levels) {
+  for (auto& lv : level_nodes_) {
     int mf = 1;
     for (int j : lv) mf = std::max(mf, P.nodes[j].ncols + P.nodes[j].R());
     lev_front_.push_back(mf);
   }
+  if (levnodes.empty()) levnodes.push_back(0);
   dlevnodes_.upload(levnodes, st_);
   std::vector<int> status = {0, INT_MAX, 0, INT_MAX};
   const char* dbg = std::getenv("RSC_DEBUG_CHECK");
@@ -1363,6 +1399,8 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     a.ldb = ldb;
     a.X = X;
     a.ldx = ldx;
+    a.trace = dtrace_.p ? dtrace_.as<unsigned long long>() : nullptr;
+    a.sub = subtree_args(B, ldb);
     dev::launch_df_reset(dcnt_.as<unsigned>(), df_ngroups_ + 1, s);
     dev::launch_apply_dataflow<T>(F(), a, vb, k, df_grid_, s);
     return;
@@ -1381,6 +1419,8 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
                                dlevnodes_.as<int>() + lev_off_[lv],
                                lev_off_[lv + 1] - lev_off_[lv], drows_.as<int>(), vb, k,
                                lev_front_[lv], s);
+      else if (p.kind == 4 || p.kind == 5)
+        dev::launch_subtrees<T>(F(), subtree_args(B, ldb), p.count, p.kind == 4, vb, k, s);
       else
         dev::launch_gemv_tasks<T>(F(), dgv_.as<dev::GvTask>() + p.first, p.count, vb, k,
                                   p.kind == 3, s);
@@ -1402,6 +1442,19 @@ void Engine<T>::check_apply(hipStream_t s) {
     dapst_.upload(z, s);
     RSC_HIP(hipStreamSynchronize(s));
     throw DeviceError("preconditioner apply: a dependency wait timed out");
+  }
+  if (dtrace_.p) {
+    // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each
+    std::vector<unsigned long long> tr(4 * df_host_.size());
+    RSC_HIP(hipMemcpy(tr.data(), dtrace_.p, tr.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+    if (FILE* fp = std::fopen(df_trace_path_.c_str(), "wb")) {
+      const int64_t nt = (int64_t)df_host_.size();
+      std::fwrite(&nt, sizeof(nt), 1, fp);
+      std::fwrite(df_host_.data(), sizeof(dev::DfTask), df_host_.size(), fp);
+      std::fwrite(tr.data(), sizeof(unsigned long long), tr.size(), fp);
+      std::fclose(fp);
+    }
   }
 }
 

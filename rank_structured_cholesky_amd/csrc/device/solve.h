@@ -40,6 +40,23 @@ void launch_gemv_tasks(const T* factor, const GvTask* tasks, int count, VecBufs 
                        int k, bool big, hipStream_t s);
 
 // ---- dataflow apply (tasks: DfTask) --------------------------------------
+// Subtree tasks (one workgroup per subtree of small uncompressed supernodes).
+struct SubtreeArgs {
+  const int* ptr;        // nodes of subtree s: nodes[ptr[s] .. ptr[s+1])
+  const int* nodes;
+  const int64_t* panel;  // factor offset of every uncompressed panel
+  const VecNode* vn;
+  const int* children;
+  const int* relmap;
+  const int* rows;
+  const int* perm;
+  const double* B;       // right-hand sides (forward input)
+  int64_t ldb;
+};
+template <typename T>
+void launch_subtrees(const T* factor, const SubtreeArgs& a, int count, bool forward,
+                     VecBufs b, int k, hipStream_t s);
+
 struct DfArgs {
   const DfTask* tasks;
   int ntasks, ngroups;
@@ -59,6 +76,8 @@ struct DfArgs {
   int64_t ldb;
   double* X;
   int64_t ldx;
+  unsigned long long* trace;  // optional: per ticket grab / ready / done clock, workgroup
+  SubtreeArgs sub;
 };
 
 // Zero the counters (ngroups + 1 entries) before a launch.

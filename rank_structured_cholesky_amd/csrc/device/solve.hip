@@ -477,6 +477,132 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
   }
 }
 
+// ---- subtree tasks ----------------------------------------------------------
+// One workgroup runs every supernode of a subtree in turn (children first
+// forward, parents first backward); the node's vectors live in LDS, the
+// factor streams once.  Per node (uncompressed panel [Lc^{-T}; Zp]):
+//   forward:  fv_c = b[perm], fv_R = 0 + children's fv_R;  y = Lc^{-1} fv_c;
+//             fv_R -= Zp fv_c;  z = Lc^{-T} y
+//   backward: x_R = w[rows];  x_c = z - Zp^T x_R
+// Only the subtree root's fv_R, z and x leave the workgroup; with COH they
+// are written with device-scope stores.
+struct SubtreeLds {
+  double c[kSubtreeMaxCols];
+  double y[kSubtreeMaxCols];
+  double x[kSubtreeMaxRows];
+};
+
+template <typename T, bool COH>
+__device__ void subtree_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                                const VecBufs& b, int k, SubtreeLds& L) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
+  const int rootn = uni(a.nodes[q1 - 1]);
+  for (int qi = q0; qi < q1; ++qi) {
+    const int j = uni(a.nodes[qi]);
+    const VecNode S = uniform_node(a.vn[j]);
+    const int64_t P = uni64(a.panel[j]);
+    const int c = S.ncols, R = S.R, ld = c + R;
+    const T* __restrict__ Lt = F + P;
+    for (int col = 0; col < k; ++col) {
+      double* fvR = b.fv + S.fv_off + (int64_t)col * b.fvs + c;
+      const double* bc = a.B + (int64_t)col * a.ldb;
+      for (int i = tid; i < c; i += 256) L.c[i] = bc[a.perm[S.first + i]];
+      for (int i = tid; i < R; i += 256) fvR[i] = 0.0;
+      __syncthreads();
+      for (int q = 0; q < S.nchild; ++q) {
+        const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
+        const int* rm = a.relmap + C.relmap_off;
+        const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
+        for (int t = tid; t < C.R; t += 256) {
+          const int p = rm[t];
+          if (p < c) L.c[p] += fc[t];
+          else fvR[p - c] += fc[t];
+        }
+        __syncthreads();
+      }
+      // y_i = sum_{j <= i} Lt[j, i] fv_c[j]   (one wave per column)
+      for (int i = wave; i < c; i += 4) {
+        double acc = 0;
+        const T* cp = Lt + (int64_t)i * ld;
+        for (int jj = lane; jj <= i; jj += 64) acc += (double)cp[jj] * L.c[jj];
+        acc = wave_sum(acc);
+        if (lane == 0) L.y[i] = acc;
+      }
+      // fv_R -= Zp fv_c   (one thread per row)
+      for (int r = tid; r < R; r += 256) {
+        const T* rp = Lt + c + r;
+        double acc = 0;
+        for (int jj = 0; jj < c; ++jj) acc += (double)rp[(int64_t)jj * ld] * L.c[jj];
+        fvR[r] -= acc;
+      }
+      __syncthreads();
+      // z_i = sum_{j >= i} Lt[i, j] y_j   (one thread per row)
+      double* zc = b.z + (int64_t)col * b.n + S.first;
+      for (int i = tid; i < c; i += 256) {
+        const T* rp = Lt + i;
+        double acc = 0;
+        for (int jj = i; jj < c; ++jj) acc += (double)rp[(int64_t)jj * ld] * L.y[jj];
+        vst<COH>(zc + i, acc);
+      }
+      if (COH && j == rootn)
+        for (int r = tid; r < R; r += 256) vst<true>(fvR + r, fvR[r]);
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T, bool COH>
+__device__ void subtree_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                                 const VecBufs& b, int k, SubtreeLds& L) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
+  for (int qi = q1 - 1; qi >= q0; --qi) {
+    const int j = uni(a.nodes[qi]);
+    const VecNode S = uniform_node(a.vn[j]);
+    const int64_t P = uni64(a.panel[j]);
+    const int c = S.ncols, R = S.R, ld = c + R;
+    const T* __restrict__ Zp = F + P + c;
+    const int* rw = a.rows + S.rows_off;
+    for (int col = 0; col < k; ++col) {
+      double* wc = b.w + (int64_t)col * b.n;
+      const double* zc = b.z + (int64_t)col * b.n;
+      for (int t = tid; t < R; t += 256) L.x[t] = vld<COH>(wc + rw[t]);
+      __syncthreads();
+      // x_i = z_i - sum_r Zp[r, i] x_R[r]   (one wave per column)
+      for (int i = wave; i < c; i += 4) {
+        double acc = 0;
+        const T* cp = Zp + (int64_t)i * ld;
+        for (int r = lane; r < R; r += 64) acc += (double)cp[r] * L.x[r];
+        acc = wave_sum(acc);
+        if (lane == 0) vst<COH>(wc + S.first + i, vld<COH>(zc + S.first + i) - acc);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void k_subtrees(const T* __restrict__ F, SubtreeArgs a,
+                                                  VecBufs b, int k) {
+  __shared__ SubtreeLds L;
+  if (b.live && uni(*b.live) == 0) return;
+  if (FWD) subtree_forward<T, false>(F, a, blockIdx.x, b, k, L);
+  else subtree_backward<T, false>(F, a, blockIdx.x, b, k, L);
+}
+
+template <typename T>
+void launch_subtrees(const T* factor, const SubtreeArgs& a, int count, bool forward,
+                     VecBufs b, int k, hipStream_t s) {
+  if (count <= 0) return;
+  if (forward) k_subtrees<T, true><<<count, 256, 0, s>>>(factor, a, b, k);
+  else k_subtrees<T, false><<<count, 256, 0, s>>>(factor, a, b, k);
+}
+template void launch_subtrees<double>(const double*, const SubtreeArgs&, int, bool, VecBufs,
+                                      int, hipStream_t);
+template void launch_subtrees<float>(const float*, const SubtreeArgs&, int, bool, VecBufs, int,
+                                     hipStream_t);
+
 constexpr int DF_SCATTER_CHUNK = 4096;
 
 // Thread 0 waits until group g has completed need tasks.  Returns false when
@@ -510,14 +636,17 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArgs a,
                                                   VecBufs b, int k) {
   __shared__ double red[4][64];
+  __shared__ SubtreeLds sl;
   __shared__ int s_ticket;
   __shared__ int s_ok;
+  __shared__ unsigned long long s_tgrab, s_tready;
   const int tid = threadIdx.x;
   if (b.live && uni(*b.live) == 0) return;
   if (tid == 0) {
     s_ok = 1;
     s_ticket = (int)__hip_atomic_fetch_add(a.cnt + a.ngroups, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
+    if (a.trace) s_tgrab = wall_clock64();
   }
   __syncthreads();
   for (;;) {
@@ -536,6 +665,7 @@ __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArg
       }
       __syncthreads();
     }
+    if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
       assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b, k,
                            a.B, a.ldb, a.perm);
@@ -543,6 +673,10 @@ __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArg
       gather_chunk<true>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
       gemv_lite<T, true>(F, a.gv[ta], b, k, red);
+    } else if (type == 4) {
+      subtree_forward<T, true>(F, a.sub, ta, b, k, sl);
+    } else if (type == 5) {
+      subtree_backward<T, true>(F, a.sub, ta, b, k, sl);
     } else {
       const int i0 = ta * DF_SCATTER_CHUNK;
       const int i1 = min(i0 + DF_SCATTER_CHUNK, (int)b.n);
@@ -558,8 +692,16 @@ __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArg
       __hip_atomic_fetch_add(a.cnt + group, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (group2 >= 0)
         __hip_atomic_fetch_add(a.cnt + group2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.trace) {
+        unsigned long long* tr = a.trace + 4 * (int64_t)ticket;
+        tr[0] = s_tgrab;
+        tr[1] = s_tready;
+        tr[2] = wall_clock64();
+        tr[3] = blockIdx.x;
+      }
       s_ticket = (int)__hip_atomic_fetch_add(a.cnt + a.ngroups, 1u, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
+      if (a.trace) s_tgrab = wall_clock64();
     }
     __syncthreads();
   }

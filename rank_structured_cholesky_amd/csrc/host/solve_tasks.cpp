@@ -8,7 +8,50 @@
 
 namespace rsc {
 
-void build_solve_layout(const Plan& P, SolveLayout& L) {
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S) {
+  const int ns = (int)P.nodes.size();
+  S.of.assign(ns, -1);
+  S.ptr.assign(1, 0);
+  S.nodes.clear();
+  S.root.clear();
+  if (max_elems <= 0) return;
+  // children are numbered before their parents (postordered supernodes)
+  std::vector<char> ok(ns, 0);
+  std::vector<int64_t> cost(ns, 0);
+  for (int j = 0; j < ns; ++j) {
+    const Supernode& s = P.nodes[j];
+    bool e = !s.compress && s.ncols <= kSubtreeMaxCols && s.R() <= kSubtreeMaxRows;
+    int64_t c = (int64_t)s.ncols * (s.ncols + s.R());
+    for (int ch : s.children) {
+      if (ch >= j) throw std::logic_error("supernodes not in postorder");
+      e = e && ok[ch];
+      c += cost[ch];
+    }
+    ok[j] = e && c <= max_elems;
+    cost[j] = c;
+  }
+  std::vector<int> stack;
+  for (int j = 0; j < ns; ++j) {
+    const int par = P.nodes[j].parent;
+    if (!ok[j] || (par >= 0 && ok[par])) continue;
+    const int id = (int)S.root.size();
+    S.root.push_back(j);
+    std::vector<int> mem;
+    stack.assign(1, j);
+    while (!stack.empty()) {
+      const int q = stack.back();
+      stack.pop_back();
+      mem.push_back(q);
+      for (int ch : P.nodes[q].children) stack.push_back(ch);
+    }
+    std::sort(mem.begin(), mem.end());
+    for (int q : mem) S.of[q] = id;
+    S.nodes.insert(S.nodes.end(), mem.begin(), mem.end());
+    S.ptr.push_back((int)S.nodes.size());
+  }
+}
+
+void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems) {
   const int ns = (int)P.nodes.size();
   L.children.clear();
   L.relmap.clear();
@@ -57,6 +100,10 @@ void build_solve_layout(const Plan& P, SolveLayout& L) {
   }
   L.fv_stride = std::max<int64_t>(fvo, 1);
   L.tv_stride = std::max<int64_t>(tvo, 1);
+  L.panel.assign(ns, -1);
+  for (int j = 0; j < ns; ++j)
+    if (!P.nodes[j].compress) L.panel[j] = P.nodes[j].off_panel;
+  build_subtrees(P, subtree_elems, L.sub);
 }
 
 // ---- per-node stages ------------------------------------------------------
@@ -264,26 +311,34 @@ void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out) {
       }
     }
   };
+  const int nsub = (int)L.sub.root.size();
+  out.level_nodes.assign(P.levels.size(), {});
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
     std::vector<std::vector<Stage>> A, B, C;
     for (int j : P.levels[lv]) {
+      if (L.sub.of[j] >= 0) continue;
+      out.level_nodes[lv].push_back(j);
       NodeStages ns = node_stages(P, L, j);
       A.push_back(ns.A);
       B.push_back(ns.B);
       C.push_back(ns.C);
     }
-    out.fwd[lv].push_back({0, 0, 0});
-    phases(out.fwd[lv], A);
-    phases(out.fwd[lv], B);
-    out.bwd[lv].push_back({1, 0, 0});
-    phases(out.bwd[lv], C);
+    if (lv == 0 && nsub > 0) out.fwd[lv].push_back({4, 0, nsub});
+    if (!out.level_nodes[lv].empty()) {
+      out.fwd[lv].push_back({0, 0, 0});
+      phases(out.fwd[lv], A);
+      phases(out.fwd[lv], B);
+      out.bwd[lv].push_back({1, 0, 0});
+      phases(out.bwd[lv], C);
+    }
+    if (lv == 0 && nsub > 0) out.bwd[lv].push_back({5, 0, nsub});
   }
   // large tasks use the high-MLP kernel
   for (auto* dir : {&out.fwd, &out.bwd})
     for (auto& lv : *dir) {
       std::vector<SolvePhase> sp;
       for (const SolvePhase& p : lv) {
-        if (p.kind < 2) {
+        if (p.kind < 2 || p.kind > 3) {
           sp.push_back(p);
           continue;
         }
@@ -324,6 +379,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     gwoff.push_back((int)gw.size());
     return (int)need.size() - 1;
   };
+  int cur_node = -1, cur_stage = 0;
   auto add = [&](int type, int a, int b, int g) {
     dev::DfTask t{};
     t.type = type;
@@ -331,6 +387,8 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     t.b = b;
     t.group = g;
     t.group2 = -1;
+    t.node = cur_node;
+    t.stage = cur_stage;
     tasks.push_back(t);
     need[g] += 1u;
   };
@@ -361,11 +419,26 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   auto chunks = [&](int j) { return (P.nodes[j].ncols + P.nodes[j].R() + VCH - 1) / VCH; };
   std::vector<NodeStages> st(ns);
   for (int j = 0; j < ns; ++j) st[j] = node_stages(P, L, j);
+  const Subtrees& SB = L.sub;
+  const int nsub = (int)SB.root.size();
+  std::vector<std::vector<int>> lnodes(P.levels.size());
+  for (size_t lv = 0; lv < P.levels.size(); ++lv)
+    for (int j : P.levels[lv])
+      if (SB.of[j] < 0) lnodes[lv].push_back(j);
+  for (int sb = 0; sb < nsub; ++sb) {
+    cur_node = SB.root[sb];
+    cur_stage = 3;
+    const int g = new_group({});
+    add(4, sb, 0, g);
+    lastA[SB.root[sb]] = lastB[SB.root[sb]] = g;
+  }
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
-    const std::vector<int>& nodes = P.levels[lv];
+    const std::vector<int>& nodes = lnodes[lv];
     for (int j : nodes) {
       std::vector<int> waits;
       for (int c : P.nodes[j].children) waits.push_back(lastA[c]);
+      cur_node = j;
+      cur_stage = 0;
       const int g = new_group(waits);
       for (int c = 0; c < chunks(j); ++c) add(0, j, c, g);
       lastA[j] = g;
@@ -378,20 +451,28 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     for (size_t k = 0; k < nA; ++k)
       for (int j : nodes)
         if (k < st[j].A.size()) {
+          cur_node = j;
+          cur_stage = 10 + (int)k;
           lastA[j] = stage(st[j].A[k], lastA[j]);
           if (k == 0) a0[j] = lastA[j];
         }
     for (int j : nodes) lastB[j] = a0[j] >= 0 ? a0[j] : lastA[j];
     for (size_t k = 0; k < nB; ++k)
       for (int j : nodes)
-        if (k < st[j].B.size()) lastB[j] = stage(st[j].B[k], lastB[j]);
+        if (k < st[j].B.size()) {
+          cur_node = j;
+          cur_stage = 20 + (int)k;
+          lastB[j] = stage(st[j].B[k], lastB[j]);
+        }
   }
   for (int lv = (int)P.levels.size() - 1; lv >= 0; --lv) {
-    const std::vector<int>& nodes = P.levels[lv];
+    const std::vector<int>& nodes = lnodes[lv];
     for (int j : nodes) {
       const int par = P.nodes[j].parent;
       std::vector<int> waits = {lastB[j]};
       if (par >= 0) waits.push_back(lastC[par]);
+      cur_node = j;
+      cur_stage = 1;
       const int g = new_group(waits);
       for (int c = 0; c < chunks(j); ++c) add(1, j, c, g);
       lastC[j] = g;
@@ -400,7 +481,21 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     for (int j : nodes) nC = std::max(nC, st[j].C.size());
     for (size_t k = 0; k < nC; ++k)
       for (int j : nodes)
-        if (k < st[j].C.size()) lastC[j] = stage(st[j].C[k], lastC[j]);
+        if (k < st[j].C.size()) {
+          cur_node = j;
+          cur_stage = 30 + (int)k;
+          lastC[j] = stage(st[j].C[k], lastC[j]);
+        }
+  }
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int r = SB.root[sb], par = P.nodes[r].parent;
+    std::vector<int> waits = {lastB[r]};
+    if (par >= 0) waits.push_back(lastC[par]);
+    cur_node = r;
+    cur_stage = 4;
+    const int g = new_group(waits);
+    add(5, sb, 0, g);
+    for (int q = SB.ptr[sb]; q < SB.ptr[sb + 1]; ++q) lastC[SB.nodes[q]] = g;
   }
   // scatter after every node's last backward group
   const int ready = new_group({});
@@ -412,6 +507,8 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
       need[ready] += 1u;
     }
   const int gsc = new_group({ready});
+  cur_node = -1;
+  cur_stage = 2;
   for (int c = 0; c * SCH < P.n; ++c) add(3, c, 0, gsc);
   // every group waited on must complete before the waiting task's ticket
   std::vector<int> last_task(need.size(), -1);

@@ -50,20 +50,36 @@ struct VecNode {
 // running or done, so the launch drains whatever the occupancy.
 struct DfTask {
   int type;    // 0 forward assembly chunk, 1 backward gather chunk, 2 matrix-vector
-               // task, 3 scatter chunk
-  int a, b;    // type 0/1: node, chunk; type 2: GvTask index; type 3: chunk
+               // task, 3 scatter chunk, 4 forward subtree, 5 backward subtree
+  int a, b;    // type 0/1: node, chunk; type 2: GvTask index; type 3: chunk; 4/5: subtree
   int group;   // group signalled on completion
   int group2;  // second group signalled (-1: none)
-  int pad[3];
+  int node;    // supernode the task belongs to (-1: scatter)
+  int stage;   // 0 assembly, 1 gather, 2 scatter, 10+k chain A, 20+k chain B, 30+k chain C
+  int pad;
 };
 
 }  // namespace dev
 
 constexpr int kSolveChunk = 1024;    // front-vector entries per assembly/gather task
 constexpr int kScatterChunk = 4096;  // vector entries per scatter task
+// Subtree tasks: one workgroup runs every supernode of a small subtree in
+// turn; a node's ncols and off-diagonal rows must fit its LDS vectors.
+constexpr int kSubtreeMaxCols = 512;
+constexpr int kSubtreeMaxRows = 1024;
+
+// Subtrees of uncompressed supernodes handled by one task each.
+struct Subtrees {
+  std::vector<int> of;          // per supernode: subtree id, -1 if none
+  std::vector<int> ptr, nodes;  // nodes of subtree s (children before parents)
+  std::vector<int> root;        // root supernode of subtree s
+};
+// Maximal subtrees whose panels hold at most max_elems elements in total.
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S);
 
 struct SolvePhase {
-  int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks
+  int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks,
+             // 4 forward subtree tasks, 5 backward subtree tasks
   int first, count;
 };
 
@@ -75,12 +91,15 @@ struct SolveLayout {
   std::vector<int> children, relmap, rows;
   std::vector<int64_t> fv_off, tv_node;
   std::vector<std::vector<int64_t>> tv_h;
+  std::vector<int64_t> panel;  // factor offset of each uncompressed panel (-1 otherwise)
   int64_t fv_stride = 1, tv_stride = 1;
+  Subtrees sub;
 };
 
 struct LevelTasks {
   std::vector<dev::GvTask> gv;
   std::vector<std::vector<SolvePhase>> fwd, bwd;  // per level
+  std::vector<std::vector<int>> level_nodes;      // per level, nodes outside subtrees
   int64_t slots = 0;                              // partial-sum slots (32 each)
 };
 
@@ -92,7 +111,8 @@ struct DataflowTasks {
   int64_t slots = 0;
 };
 
-void build_solve_layout(const Plan& P, SolveLayout& L);
+// Vector layout; subtree_elems = 0 disables subtree tasks.
+void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems);
 void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
 void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out);
 

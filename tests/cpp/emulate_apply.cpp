@@ -13,6 +13,7 @@
 // dependency or out-of-range index shows up.
 //
 //   emulate_apply <problem> <N> <max_dense_block> <max_diagonal_block> <nrhs> <threads>
+//                 [subtree_elems]
 //   problem: p3d (7-point log-normal Poisson), p2d, el (elasticity)
 // Prints "ok ..." and exits 0 on success.
 #include <atomic>
@@ -174,12 +175,72 @@ void scatter(State& S, int chunk) {
       S.X.at(chk(S.perm, i, "perm") + (int64_t)col * S.n) = S.w.at(i + (int64_t)col * S.n);
 }
 
+// subtree tasks: every node of subtree sb, children first (forward) or
+// parents first (backward); panel = [Lc^{-T}; Zp]
+void subtree(State& S, int sb, bool fwd) {
+  const Subtrees& T = S.L->sub;
+  const int q0 = chk(T.ptr, sb, "st ptr"), q1 = chk(T.ptr, sb + 1, "st ptr");
+  for (int step = 0; step < q1 - q0; ++step) {
+    const int j = chk(T.nodes, fwd ? q0 + step : q1 - 1 - step, "st nodes");
+    const dev::VecNode& N = chk(S.L->vn, j, "vn");
+    const int64_t P = chk(S.L->panel, j, "panel");
+    if (P < 0) fail("subtree node without a panel");
+    const int c = N.ncols, R = N.R, ld = c + R;
+    if (c > kSubtreeMaxCols || R > kSubtreeMaxRows) fail("subtree node too large");
+    for (int col = 0; col < S.k; ++col) {
+      const int64_t fb = N.fv_off + (int64_t)col * S.L->fv_stride;
+      const int64_t vb = (int64_t)col * S.n;
+      if (fwd) {
+        std::vector<double> fc(c), y(c);
+        for (int i = 0; i < c; ++i) fc[i] = S.B.at(chk(S.perm, N.first + i, "perm") + vb);
+        for (int r = 0; r < R; ++r) S.fv.at(fb + c + r) = 0.0;
+        for (int q = 0; q < N.nchild; ++q) {
+          const int cj = chk(S.L->children, N.child_off + q, "children");
+          if (T.of[cj] != sb) fail("subtree child outside the subtree");
+          const dev::VecNode& C = chk(S.L->vn, cj, "vn");
+          for (int a = 0; a < C.R; ++a) {
+            const int p = chk(S.L->relmap, C.relmap_off + a, "relmap");
+            const double v = S.fv.at(C.fv_off + (int64_t)col * S.L->fv_stride + C.ncols + a);
+            if (p < c) fc[p] += v;
+            else S.fv.at(fb + p) += v;
+          }
+        }
+        for (int i = 0; i < c; ++i) {
+          double acc = 0;
+          for (int jj = 0; jj <= i; ++jj) acc += S.F.at(P + jj + (int64_t)i * ld) * fc[jj];
+          y[i] = acc;
+        }
+        for (int r = 0; r < R; ++r) {
+          double acc = 0;
+          for (int jj = 0; jj < c; ++jj) acc += S.F.at(P + c + r + (int64_t)jj * ld) * fc[jj];
+          S.fv.at(fb + c + r) -= acc;
+        }
+        for (int i = 0; i < c; ++i) {
+          double acc = 0;
+          for (int jj = i; jj < c; ++jj) acc += S.F.at(P + i + (int64_t)jj * ld) * y[jj];
+          S.z.at(N.first + i + vb) = acc;
+        }
+      } else {
+        std::vector<double> xr(R);
+        for (int r = 0; r < R; ++r) xr[r] = S.w.at(chk(S.L->rows, N.rows_off + r, "rows") + vb);
+        for (int i = 0; i < c; ++i) {
+          double acc = 0;
+          for (int r = 0; r < R; ++r) acc += S.F.at(P + c + r + (int64_t)i * ld) * xr[r];
+          S.w.at(N.first + i + vb) = S.z.at(N.first + i + vb) - acc;
+        }
+      }
+    }
+  }
+}
+
 void run_df_task(State& S, const DataflowTasks& D, const dev::DfTask& t) {
   switch (t.type) {
     case 0: assemble(S, t.a, t.b, false); break;
     case 1: gather(S, t.a, t.b); break;
     case 2: gemv(S, chk(D.gv, t.a, "df gv")); break;
     case 3: scatter(S, t.a); break;
+    case 4: subtree(S, t.a, true); break;
+    case 5: subtree(S, t.a, false); break;
     default: fail("bad task type");
   }
 }
@@ -213,7 +274,8 @@ int main(int argc, char** argv) {
   Plan P;
   build_plan(A, xyz.empty() ? nullptr : xyz.data(), opt, nullptr, P);
   SolveLayout L;
-  build_solve_layout(P, L);
+  const int64_t sub_elems = argc > 7 ? std::atoll(argv[7]) : std::max<int64_t>(256, P.factor_elems / 64);
+  build_solve_layout(P, L, sub_elems);
   LevelTasks LT;
   build_level_tasks(P, L, LT);
   DataflowTasks D;
@@ -246,8 +308,10 @@ int main(int argc, char** argv) {
       S.w.at(i + (int64_t)col * S.n) = S.B.at(S.perm[i] + (int64_t)col * S.n);
   auto run_level = [&](const std::vector<SolvePhase>& ph, int lv) {
     for (const SolvePhase& p : ph) {
-      if (p.kind == 0 || p.kind == 1) {
-        for (int j : P.levels[lv]) {
+      if (p.kind == 4 || p.kind == 5) {
+        for (int sb = 0; sb < p.count; ++sb) subtree(S, sb, p.kind == 4);
+      } else if (p.kind == 0 || p.kind == 1) {
+        for (int j : LT.level_nodes[lv]) {
           const int nch = (P.nodes[j].ncols + P.nodes[j].R() + kSolveChunk - 1) / kSolveChunk;
           for (int c = 0; c < nch; ++c) {
             if (p.kind == 0) assemble(S, j, c, true);
@@ -337,7 +401,8 @@ int main(int argc, char** argv) {
     if (stuck) fail("threaded dataflow run stalled");
   }
   if (S.X.v != x_level) fail("threaded dataflow result differs from the level path");
-  std::printf("ok n=%d nodes=%d levels=%d level_tasks=%zu df_tasks=%zu groups=%d\n", S.n,
-              (int)P.nodes.size(), nlev, LT.gv.size(), D.tasks.size(), ng);
+  std::printf("ok n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d\n",
+              S.n, (int)P.nodes.size(), nlev, L.sub.root.size(), LT.gv.size(), D.tasks.size(),
+              ng);
   return 0;
 }

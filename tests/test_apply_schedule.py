@@ -1,7 +1,7 @@
 """The apply's task lists on the CPU (tests/cpp/emulate_apply.cpp): the
 dataflow list run in ticket order and on threads gives bitwise the result of
 the level-scheduled list, every wait is satisfied when its task starts, and
-every buffer access is in range."""
+every buffer access is in range; with and without subtree tasks."""
 import os
 import subprocess
 
@@ -28,6 +28,8 @@ def built():
     "el 5 30 40 1 8",        # elasticity (3 dof per node)
     "p2d 48 1000000 400 1 8",  # no compression
     "p3d 24 63 128 3 8",     # split column tasks (partial sums)
+    "p3d 20 63 50 1 8 0",    # no subtree tasks
+    "p3d 20 63 50 1 8 100000000",  # subtree tasks as large as allowed
 ])
 def test_dataflow_schedule_matches_level_schedule(args):
     r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)

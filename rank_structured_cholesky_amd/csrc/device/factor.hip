@@ -823,6 +823,17 @@ void Engine<T>::emit_compressed(int j) {
     char* K = wp(nsb + (int64_t)R * s.rank + (int64_t)s.rank * s.rank);
     lowrank_update(wp(upd_base() + s.off_update), R, R, J.V, J.U, c, s.rank, K, Tm);
   }
+  if (s.off_dinv >= 0) {
+    // dense L_D^{-T} = I L_D^{-T} through the tree (leaves still hold L)
+    ctx_ = "node " + std::to_string(j) + " dense diagonal inverse";
+    Step st{OP_EYE, 0, (int64_t)etiles_.size()};
+    const int jid = (int)ejobs_.size();
+    ejobs_.push_back({F() + s.off_dinv, c, c});
+    for (int c0 = 0; c0 < c; c0 += dev::TILE) etiles_.push_back({jid, c0});
+    st.count = (int)(etiles_.size() - st.first);
+    push(st);
+    hsolve_right(s, hodlr_root(s), fp(s.off_dinv), c, c, Tm);
+  }
   ctx_ = "node " + std::to_string(j) + " invert leaves";
   std::vector<CholMat> leaves;
   for (auto& h : s.hnodes)
@@ -1095,8 +1106,10 @@ void Engine<T>::setup_static() {
     part_stride_ = std::max<int64_t>(std::max(LT.slots, DT.slots) * 32, 1);
   }
   {
-    const char* df = std::getenv("RSC_APPLY_DATAFLOW");
-    use_dataflow_ = df && df[0] == '1';
+    // one persistent dataflow launch per apply; RSC_APPLY_LEVELS=1 selects the
+    // level-scheduled launches instead
+    const char* lv = std::getenv("RSC_APPLY_LEVELS");
+    use_dataflow_ = !(lv && lv[0] == '1');
     int dev = 0, cus = 1, khz = 0;
     RSC_HIP(hipGetDevice(&dev));
     RSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

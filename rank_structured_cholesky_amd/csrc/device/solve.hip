@@ -23,6 +23,7 @@ namespace rsc {
 namespace dev {
 
 constexpr int MAXR = 1024;  // largest rank handled in LDS by the solves
+constexpr int kXlds = 2048;  // x entries staged in LDS per ROW batch (dataflow tasks)
 
 __global__ void k_gather(const double* __restrict__ b, int ldb, double* __restrict__ w,
                          const int* __restrict__ perm, int n, int k,
@@ -398,7 +399,7 @@ template void launch_gemv_tasks<float>(const float*, const GvTask*, int, VecBufs
 // row blocks per batch.
 template <typename T, bool COH>
 __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBufs& b, int k,
-                          double (*red)[64]) {
+                          double (*red)[64], double* xl) {
   const GvTask t = uniform_task(t0);
   const T* __restrict__ M = F + t.mat;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -415,26 +416,31 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
     }
     const double* x = vsel(b, t.xsel, col) + t.xoff;
     if (t.kind == 0) {
+      // x staged in LDS (coalesced, once per task); 16 matrix loads per
+      // lane in flight per batch
       const bool act = lane < t.rows;
       const T* rp = M + (act ? lane : 0);
       const int64_t ld4 = 4 * (int64_t)t.ld;
       double acc0 = 0, acc1 = 0;
-      int j = wave;
-      for (; j + 28 < t.cols; j += 32) {
-        const T* p = rp + (int64_t)j * t.ld;
-        double m[8], xv[8];
+      for (int jb = 0; jb < t.cols; jb += kXlds) {
+        const int nb = min(kXlds, t.cols - jb);
+        for (int i = tid; i < nb; i += 256) xl[i] = vld<COH>(x + jb + i);
+        __syncthreads();
+        int j = wave;
+        for (; j + 60 < nb; j += 64) {
+          const T* p = rp + (int64_t)(jb + j) * t.ld;
+          double m[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          m[u] = (double)p[u * ld4];
-          xv[u] = vld<COH>(x + j + 4 * u);
-        }
+          for (int u = 0; u < 16; ++u) m[u] = (double)p[u * ld4];
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-          acc0 += m[u] * xv[u];
-          acc1 += m[u + 1] * xv[u + 1];
+          for (int u = 0; u < 16; u += 2) {
+            acc0 += m[u] * xl[j + 4 * u];
+            acc1 += m[u + 1] * xl[j + 4 * u + 4];
+          }
         }
+        for (; j < nb; j += 4) acc0 += (double)rp[(int64_t)(jb + j) * t.ld] * xl[j];
+        __syncthreads();
       }
-      for (; j < t.cols; j += 4) acc0 += (double)rp[(int64_t)j * t.ld] * vld<COH>(x + j);
       red[wave][lane] = act ? acc0 + acc1 : 0.0;
       __syncthreads();
       if (wave == 0 && act) {
@@ -633,10 +639,13 @@ __device__ bool df_wait(const DfArgs& a, int g) {
 // cannot merge a tid == 0 region of one iteration with one of the next (a
 // merged region lets the other lanes run the loop without thread 0).
 template <typename T>
-__global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArgs a,
+__global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, DfArgs a,
                                                   VecBufs b, int k) {
   __shared__ double red[4][64];
-  __shared__ SubtreeLds sl;
+  __shared__ union {
+    SubtreeLds sl;
+    double xl[kXlds];
+  } lds;
   __shared__ int s_ticket;
   __shared__ int s_ok;
   __shared__ unsigned long long s_tgrab, s_tready;
@@ -672,11 +681,11 @@ __global__ __launch_bounds__(256) void k_apply_df(const T* __restrict__ F, DfArg
     } else if (type == 1) {
       gather_chunk<true>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
-      gemv_lite<T, true>(F, a.gv[ta], b, k, red);
+      gemv_lite<T, true>(F, a.gv[ta], b, k, red, lds.xl);
     } else if (type == 4) {
-      subtree_forward<T, true>(F, a.sub, ta, b, k, sl);
+      subtree_forward<T, true>(F, a.sub, ta, b, k, lds.sl);
     } else if (type == 5) {
-      subtree_backward<T, true>(F, a.sub, ta, b, k, sl);
+      subtree_backward<T, true>(F, a.sub, ta, b, k, lds.sl);
     } else {
       const int i0 = ta * DF_SCATTER_CHUNK;
       const int i1 = min(i0 + DF_SCATTER_CHUNK, (int)b.n);

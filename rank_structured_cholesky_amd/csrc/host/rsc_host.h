@@ -132,6 +132,7 @@ struct Supernode {
   int rank = 0;
   int64_t off_V = -1, off_U = -1;
   int64_t off_Xoff = -1;        // L_D^{-T} U (ncols x rank), the apply's form of U
+  int64_t off_dinv = -1;        // dense L_D^{-T} (ncols x ncols) of a root separator
   int64_t rng_off = -1;
   std::vector<HNode> hnodes;    // HODLR tree (compressed only); [0] = root
   std::vector<HStep> hsteps;

@@ -159,6 +159,12 @@ NodeStages node_stages(const Plan& P, const SolveLayout& L, int j) {
       ns.C.push_back({{true, s.off_panel + c, c + R, R, c, 1, FV, fo + c, W, s.first, false}});
     return ns;
   }
+  if (s.off_dinv >= 0) {
+    // dense L_D^{-T} (root separator, no off-diagonal rows)
+    ns.A.push_back({{true, s.off_dinv, c, c, c, 0, FV, fo, W, s.first, true}});
+    ns.B.push_back({{false, s.off_dinv, c, c, c, 0, W, s.first, ZB, s.first, true}});
+    return ns;
+  }
   const int64_t tvn = L.tv_node[j];
   const std::vector<int64_t>& tvh = L.tv_h[j];
   Stage a0;

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <functional>
 #include <numeric>
@@ -171,6 +172,9 @@ void replan_ranks(Plan& plan, double diag_const) {
   const rsc_options& o = plan.opt;
   int64_t foff = 0, woff = 0, uoff = 0, rng = 0;
   double flops = 0, apply = 0;
+  const int64_t es = o.factor_precision == RSC_FP32 ? 4 : 8;
+  int64_t dense_root_bytes = (int64_t)512 << 20;
+  if (const char* e = std::getenv("RSC_DENSE_ROOT_BYTES")) dense_root_bytes = std::atoll(e);
   auto potrf_f = [](double c) { return c * c * c / 3.0 + c * c / 2.0 + c / 6.0; };
   for (auto& s : plan.nodes) {
     const int64_t c = s.ncols, R = s.R();
@@ -187,13 +191,14 @@ void replan_ranks(Plan& plan, double diag_const) {
     s.off_panel = -1;
     s.off_work_panel = woff;
     woff += (c + R) * c;
+    double tree_apply = 0;
     // diagonal tree: ranks, leaf/V/U storage, RNG offsets in draw order
     for (auto& h : s.hnodes) {
       if (h.leaf()) {
         int64_t b = h.r1 - h.r0;
         h.off_leaf = foff;
         foff += b * b;
-        apply += 2.0 * (double)b * (b + 1) / 2;
+        tree_apply += 2.0 * (double)b * (b + 1) / 2;
       } else {
         const HNode& L = s.hnodes[h.left];
         const HNode& Rt = s.hnodes[h.right];
@@ -209,8 +214,19 @@ void replan_ranks(Plan& plan, double diag_const) {
         h.off_X = foff;
         foff += (int64_t)cols * h.rank;
         // forward reads U and W, backward V and X
-        apply += 2.0 * (double)h.rank * (rows + cols);
+        tree_apply += 2.0 * (double)h.rank * (rows + cols);
       }
+    }
+    // A root separator (no off-diagonal rows) small enough keeps its
+    // diagonal inverse dense: the apply then needs two matrix-vector stages
+    // there instead of two per tree height.
+    s.off_dinv = -1;
+    if (R == 0 && s.hnodes.size() > 1 && c * c * es <= dense_root_bytes) {
+      s.off_dinv = foff;
+      foff += c * c;
+      apply += (double)c * (c + 1);
+    } else {
+      apply += tree_apply;
     }
     // draw order: traversal; explicit blocks draw in pre-order
     std::function<void(int)> pre = [&](int id) {

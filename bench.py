@@ -166,6 +166,18 @@ def cpu_leg(A, xyz, B, cfg, eps, full):
                     tf, ts, info["iterations"], reps, w_sample, w_full))
 
 
+def measured_traffic(workload, precision):
+    """HBM bytes per apply from the committed counter summary of this
+    workload (profiles/apply_traffic_<workload>.md), FP64 factor only."""
+    path = os.path.join(ROOT, "profiles", "apply_traffic_%s.md" % workload)
+    if precision != 0 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_apply:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -190,6 +202,7 @@ def main():
     p50 = statistics.median(lat) * 1e3
     p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))] * 1e3
     apply_bytes = g64["info"]["apply_bytes"] * g64["k"]
+    traffic, traffic_src = measured_traffic(a.config, 0) if g64["k"] == 1 else (None, None)
     achieved = apply_bytes / g64["apply_s"] / 1e9
     line = {
         "metric": "PCG solves/s to 1e-8 relres (FP64 factor; mixed FP32 factor beside)",
@@ -217,7 +230,8 @@ def main():
         "factor_bytes": g64["info"]["factor_bytes"],
         "roofline": {"bound": "hbm", "kernel": "preconditioner apply (fwd+bwd sweeps)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "apply_ms": round(g64["apply_s"] * 1e3, 4),
                      "apply_bytes": apply_bytes},
     }

@@ -397,10 +397,129 @@ template void launch_gemv_tasks<float>(const float*, const GvTask*, int, VecBufs
 // dependencies).  ROW: lane = row, wave g takes columns g, g+4, ..., eight
 // per batch; COL: wave w owns columns 8w..8w+7, lane l rows l + 64m, two
 // row blocks per batch.
+// Matrix-vector task body for any number of right-hand sides: groups of
+// KRHS columns share every matrix load, and the arithmetic of each column
+// does not depend on the group size, so batched and single solves agree
+// bitwise.  ROW: x staged in LDS in chunks of NB entries, wave g takes
+// columns g, g+4, ... sixteen at a time (even/odd into two sums); COL: wave
+// w owns columns 8w..8w+7 (two halves of four), lanes take rows l and
+// l + 64 of every 128-row block.  KRHS: right-hand sides per group.
+template <typename T, bool COH, int KRHS>
+__device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBufs& b, int k,
+                           double (*red)[64], double* xl) {
+  const T* __restrict__ M = F + t.mat;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NB = kXlds / KRHS;
+  for (int c0 = 0; c0 < k; c0 += KRHS) {
+    const int kb = min(KRHS, k - c0);
+    if (t.kind == 0) {
+      const bool act = lane < t.rows;
+      const T* rp = M + (act ? lane : 0);
+      const int64_t ld4 = 4 * (int64_t)t.ld;
+      double acc0[KRHS], acc1[KRHS];
+#pragma unroll
+      for (int cc = 0; cc < KRHS; ++cc) acc0[cc] = acc1[cc] = 0;
+      for (int jb = 0; jb < t.cols; jb += NB) {
+        const int nb = min(NB, t.cols - jb);
+        for (int idx = tid; idx < kb * nb; idx += 256) {
+          const int cc = idx / nb, jj = idx - cc * nb;
+          xl[cc * NB + jj] = vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + jb + jj);
+        }
+        __syncthreads();
+        int j = wave;
+        for (; j + 60 < nb; j += 64) {
+          const T* p = rp + (int64_t)(jb + j) * t.ld;
+          double m[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) m[u] = (double)p[u * ld4];
+#pragma unroll
+          for (int cc = 0; cc < KRHS; ++cc)
+#pragma unroll
+            for (int u = 0; u < 16; u += 2) {
+              acc0[cc] += m[u] * xl[cc * NB + j + 4 * u];
+              acc1[cc] += m[u + 1] * xl[cc * NB + j + 4 * u + 4];
+            }
+        }
+        for (; j < nb; j += 4) {
+          const double m = (double)rp[(int64_t)(jb + j) * t.ld];
+#pragma unroll
+          for (int cc = 0; cc < KRHS; ++cc) acc0[cc] += m * xl[cc * NB + j];
+        }
+        __syncthreads();
+      }
+      for (int cc = 0; cc < kb; ++cc) {
+        red[wave][lane] = act ? acc0[cc] + acc1[cc] : 0.0;
+        __syncthreads();
+        if (wave == 0 && act) {
+          double* y = vsel(b, t.ysel, c0 + cc) + t.yoff;
+          const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+          vst<COH>(y + lane, t.op ? vld<COH>(y + lane) - v : v);
+        }
+        __syncthreads();
+      }
+    } else {
+      for (int h = 0; h < 2; ++h) {
+        const int q0 = wave * 8 + 4 * h;
+        const int nq = min(4, t.cols - q0);
+        double acc[4][KRHS];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int cc = 0; cc < KRHS; ++cc) acc[u][cc] = 0;
+        if (nq > 0) {
+          const T* cbase = M + (int64_t)q0 * t.ld;
+          for (int i = lane; i < t.rows; i += 128) {
+            const int i2 = i + 64;
+            const bool ok2 = i2 < t.rows;
+            double x1[KRHS], x2[KRHS], m1[4], m2[4];
+#pragma unroll
+            for (int cc = 0; cc < KRHS; ++cc) {
+              const double* xc = vsel(b, t.xsel, c0 + (cc < kb ? cc : 0)) + t.xoff;
+              x1[cc] = cc < kb ? vld<COH>(xc + i) : 0.0;
+              x2[cc] = (cc < kb && ok2) ? vld<COH>(xc + i2) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
+              m1[u] = (double)cp[i];
+              m2[u] = ok2 ? (double)cp[i2] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int cc = 0; cc < KRHS; ++cc) acc[u][cc] += m1[u] * x1[cc] + m2[u] * x2[cc];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int cc = 0; cc < KRHS; ++cc) {
+            const double v = wave_sum(acc[u][cc]);
+            if (lane == 0 && u < nq && cc < kb) {
+              if (t.part >= 0)
+                vst<COH>(b.part + (int64_t)(c0 + cc) * b.ps + (int64_t)t.part * 32 + q0 + u, v);
+              else {
+                double* y = vsel(b, t.ysel, c0 + cc) + t.yoff + q0 + u;
+                vst<COH>(y, t.op ? vld<COH>(y) - v : v);
+              }
+            }
+          }
+      }
+    }
+  }
+}
+
 template <typename T, bool COH>
 __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBufs& b, int k,
                           double (*red)[64], double* xl) {
   const GvTask t = uniform_task(t0);
+  if (t.kind != 2) {
+    // chunks of NB = kXlds / KRHS (multiples of 64) keep every column's sum
+    // order the same for one and four right-hand sides per group
+    if (k == 1) gemv_multi<T, COH, 1>(F, t, b, k, red, xl);
+    else gemv_multi<T, COH, 4>(F, t, b, k, red, xl);
+    return;
+  }
   const T* __restrict__ M = F + t.mat;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int col = 0; col < k; ++col) {
@@ -493,14 +612,16 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
 // Only the subtree root's fv_R, z and x leave the workgroup; with COH they
 // are written with device-scope stores.
 struct SubtreeLds {
-  double c[kSubtreeMaxCols];
-  double y[kSubtreeMaxCols];
-  double x[kSubtreeMaxRows];
+  double c[kSubtreeRhs][kSubtreeMaxCols];
+  double y[kSubtreeRhs][kSubtreeMaxCols];
+  double x[kSubtreeRhs][kSubtreeMaxRows];
 };
 
-template <typename T, bool COH>
-__device__ void subtree_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
-                                const VecBufs& b, int k, SubtreeLds& L) {
+// Right-hand sides in groups of KS (1 or kSubtreeRhs) per pass over the
+// panels; each right-hand side's sums run in the same order for any KS.
+template <typename T, bool COH, int KS>
+__device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                                  const VecBufs& b, int k, SubtreeLds& L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   const int rootn = uni(a.nodes[q1 - 1]);
@@ -510,57 +631,87 @@ __device__ void subtree_forward(const T* __restrict__ F, const SubtreeArgs& a, i
     const int64_t P = uni64(a.panel[j]);
     const int c = S.ncols, R = S.R, ld = c + R;
     const T* __restrict__ Lt = F + P;
-    for (int col = 0; col < k; ++col) {
-      double* fvR = b.fv + S.fv_off + (int64_t)col * b.fvs + c;
-      const double* bc = a.B + (int64_t)col * a.ldb;
-      for (int i = tid; i < c; i += 256) L.c[i] = bc[a.perm[S.first + i]];
-      for (int i = tid; i < R; i += 256) fvR[i] = 0.0;
+    for (int c0 = 0; c0 < k; c0 += KS) {
+      const int kb = min(KS, k - c0);
+      for (int cc = 0; cc < kb; ++cc) {
+        double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
+        const double* bc = a.B + (int64_t)(c0 + cc) * a.ldb;
+        for (int i = tid; i < c; i += 256) L.c[cc][i] = bc[a.perm[S.first + i]];
+        for (int i = tid; i < R; i += 256) fvR[i] = 0.0;
+      }
       __syncthreads();
       for (int q = 0; q < S.nchild; ++q) {
         const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
         const int* rm = a.relmap + C.relmap_off;
-        const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
-        for (int t = tid; t < C.R; t += 256) {
-          const int p = rm[t];
-          if (p < c) L.c[p] += fc[t];
-          else fvR[p - c] += fc[t];
+        for (int cc = 0; cc < kb; ++cc) {
+          const double* fc = b.fv + C.fv_off + (int64_t)(c0 + cc) * b.fvs + C.ncols;
+          double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
+          for (int t = tid; t < C.R; t += 256) {
+            const int p = rm[t];
+            if (p < c) L.c[cc][p] += fc[t];
+            else fvR[p - c] += fc[t];
+          }
         }
         __syncthreads();
       }
       // y_i = sum_{j <= i} Lt[j, i] fv_c[j]   (one wave per column)
       for (int i = wave; i < c; i += 4) {
-        double acc = 0;
+        double acc[KS];
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
         const T* cp = Lt + (int64_t)i * ld;
-        for (int jj = lane; jj <= i; jj += 64) acc += (double)cp[jj] * L.c[jj];
-        acc = wave_sum(acc);
-        if (lane == 0) L.y[i] = acc;
+        for (int jj = lane; jj <= i; jj += 64) {
+          const double m = (double)cp[jj];
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.c[cc][jj];
+        }
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) {
+          const double v = wave_sum(acc[cc]);
+          if (lane == 0) L.y[cc][i] = v;
+        }
       }
       // fv_R -= Zp fv_c   (one thread per row)
       for (int r = tid; r < R; r += 256) {
         const T* rp = Lt + c + r;
-        double acc = 0;
-        for (int jj = 0; jj < c; ++jj) acc += (double)rp[(int64_t)jj * ld] * L.c[jj];
-        fvR[r] -= acc;
+        double acc[KS];
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
+        for (int jj = 0; jj < c; ++jj) {
+          const double m = (double)rp[(int64_t)jj * ld];
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.c[cc][jj];
+        }
+        for (int cc = 0; cc < kb; ++cc) b.fv[S.fv_off + (int64_t)(c0 + cc) * b.fvs + c + r] -= acc[cc];
       }
       __syncthreads();
       // z_i = sum_{j >= i} Lt[i, j] y_j   (one thread per row)
-      double* zc = b.z + (int64_t)col * b.n + S.first;
       for (int i = tid; i < c; i += 256) {
         const T* rp = Lt + i;
-        double acc = 0;
-        for (int jj = i; jj < c; ++jj) acc += (double)rp[(int64_t)jj * ld] * L.y[jj];
-        vst<COH>(zc + i, acc);
+        double acc[KS];
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
+        for (int jj = i; jj < c; ++jj) {
+          const double m = (double)rp[(int64_t)jj * ld];
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.y[cc][jj];
+        }
+        for (int cc = 0; cc < kb; ++cc)
+          vst<COH>(b.z + (int64_t)(c0 + cc) * b.n + S.first + i, acc[cc]);
       }
       if (COH && j == rootn)
-        for (int r = tid; r < R; r += 256) vst<true>(fvR + r, fvR[r]);
+        for (int cc = 0; cc < kb; ++cc) {
+          double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
+          for (int r = tid; r < R; r += 256) vst<true>(fvR + r, fvR[r]);
+        }
       __syncthreads();
     }
   }
 }
 
-template <typename T, bool COH>
-__device__ void subtree_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
-                                 const VecBufs& b, int k, SubtreeLds& L) {
+template <typename T, bool COH, int KS>
+__device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                                   const VecBufs& b, int k, SubtreeLds& L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   for (int qi = q1 - 1; qi >= q0; --qi) {
@@ -570,22 +721,50 @@ __device__ void subtree_backward(const T* __restrict__ F, const SubtreeArgs& a, 
     const int c = S.ncols, R = S.R, ld = c + R;
     const T* __restrict__ Zp = F + P + c;
     const int* rw = a.rows + S.rows_off;
-    for (int col = 0; col < k; ++col) {
-      double* wc = b.w + (int64_t)col * b.n;
-      const double* zc = b.z + (int64_t)col * b.n;
-      for (int t = tid; t < R; t += 256) L.x[t] = vld<COH>(wc + rw[t]);
+    for (int c0 = 0; c0 < k; c0 += KS) {
+      const int kb = min(KS, k - c0);
+      for (int cc = 0; cc < kb; ++cc) {
+        const double* wc = b.w + (int64_t)(c0 + cc) * b.n;
+        for (int t = tid; t < R; t += 256) L.x[cc][t] = vld<COH>(wc + rw[t]);
+      }
       __syncthreads();
       // x_i = z_i - sum_r Zp[r, i] x_R[r]   (one wave per column)
       for (int i = wave; i < c; i += 4) {
-        double acc = 0;
+        double acc[KS];
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
         const T* cp = Zp + (int64_t)i * ld;
-        for (int r = lane; r < R; r += 64) acc += (double)cp[r] * L.x[r];
-        acc = wave_sum(acc);
-        if (lane == 0) vst<COH>(wc + S.first + i, vld<COH>(zc + S.first + i) - acc);
+        for (int r = lane; r < R; r += 64) {
+          const double m = (double)cp[r];
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.x[cc][r];
+        }
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) {
+          const double v = wave_sum(acc[cc]);
+          if (lane == 0 && cc < kb) {
+            const int64_t o = (int64_t)(c0 + cc) * b.n + S.first + i;
+            vst<COH>(b.w + o, vld<COH>(b.z + o) - v);
+          }
+        }
       }
       __syncthreads();
     }
   }
+}
+
+template <typename T, bool COH>
+__device__ void subtree_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                                const VecBufs& b, int k, SubtreeLds& L) {
+  if (k == 1) subtree_forward_g<T, COH, 1>(F, a, sb, b, k, L);
+  else subtree_forward_g<T, COH, kSubtreeRhs>(F, a, sb, b, k, L);
+}
+
+template <typename T, bool COH>
+__device__ void subtree_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                                 const VecBufs& b, int k, SubtreeLds& L) {
+  if (k == 1) subtree_backward_g<T, COH, 1>(F, a, sb, b, k, L);
+  else subtree_backward_g<T, COH, kSubtreeRhs>(F, a, sb, b, k, L);
 }
 
 template <typename T, bool FWD>

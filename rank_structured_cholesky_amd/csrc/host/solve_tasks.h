@@ -65,8 +65,9 @@ constexpr int kSolveChunk = 1024;    // front-vector entries per assembly/gather
 constexpr int kScatterChunk = 4096;  // vector entries per scatter task
 // Subtree tasks: one workgroup runs every supernode of a small subtree in
 // turn; a node's ncols and off-diagonal rows must fit its LDS vectors.
-constexpr int kSubtreeMaxCols = 512;
-constexpr int kSubtreeMaxRows = 1024;
+constexpr int kSubtreeMaxCols = 256;
+constexpr int kSubtreeMaxRows = 512;
+constexpr int kSubtreeRhs = 4;  // right-hand sides per pass over a subtree's panels
 
 // Subtrees of uncompressed supernodes handled by one task each.
 struct Subtrees {

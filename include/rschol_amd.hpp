@@ -106,6 +106,7 @@ class FactorPreconditioner : public PRECONDITIONER {
     if (_factor && !_symbolicFactorNeeded && _numericFactorNeeded &&
         matrix._p == _pattern_p && matrix._i == _pattern_i) {
       check(rsc_factor_refactor(_factor, matrix._x.data()), "rsc_factor_refactor");
+      _values = matrix._x;
       _numericFactorNeeded = false;
       return;
     }

@@ -21,7 +21,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <functional>
+#include <stdexcept>
+#include <thread>
 
 #include "factor.h"
 #include "solve.h"
@@ -87,9 +90,14 @@ class Engine final : public FactorEngine {
  public:
   Engine(const Plan& plan, const Csc& A, hipStream_t st)
       : plan_(plan), A_(A), st_(st) {
+    start_rng();
     setup_static();
   }
-  ~Engine() override { (void)hipStreamSynchronize(st_); }
+  ~Engine() override {
+    (void)hipStreamSynchronize(st_);
+    if (rng_thread_.joinable()) rng_thread_.join();
+    if (rng_host_) (void)hipHostFree(rng_host_);
+  }
 
   void numeric(const double* Ax) override;
   void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
@@ -108,6 +116,41 @@ class Engine final : public FactorEngine {
   const Plan& plan_;
   const Csc& A_;
   hipStream_t st_;
+  // Gaussian sketch entries: drawn on host threads into pinned memory while
+  // the host plans and the device runs the levels below the first compressed
+  // supernode, then copied once before the first step that reads them.
+  std::thread rng_thread_;
+  T* rng_host_ = nullptr;
+  std::exception_ptr rng_err_;
+  int rng_step_ = -1;  // first program step reading the sketch entries
+  bool rng_on_device_ = false;
+  void start_rng() {
+    if (plan_.rng_count <= 0 || rng_host_ || rng_thread_.joinable()) return;
+    rng_thread_ = std::thread([this]() {
+      try {
+        void* p = nullptr;
+        RSC_HIP(hipHostMalloc(&p, (size_t)plan_.rng_count * sizeof(T)));
+        gaussian_stream(plan_.opt.seed, plan_.rng_count, (T*)p);
+        rng_host_ = (T*)p;
+      } catch (...) {
+        rng_err_ = std::current_exception();
+      }
+    });
+  }
+  void upload_rng() {
+    start_rng();
+    if (rng_thread_.joinable()) rng_thread_.join();
+    if (rng_err_) {
+      std::exception_ptr e = rng_err_;
+      rng_err_ = nullptr;
+      std::rethrow_exception(e);
+    }
+    if (!rng_host_) throw std::runtime_error("sketch entries were not drawn");
+    RSC_HIP(hipMemcpyAsync(Wb() + rng_base(), rng_host_,
+                           (size_t)plan_.rng_count * sizeof(T), hipMemcpyHostToDevice,
+                           st_));
+    rng_on_device_ = true;
+  }
   DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
@@ -742,6 +785,7 @@ void Engine<T>::emit_compressed(int j) {
   int rmax = s.rank;
   for (auto& h : s.hnodes) rmax = std::max(rmax, h.rank);
   char* Tm = wp(nsb + (int64_t)R * s.rank + 2 * (int64_t)s.rank * s.rank);
+  if (rng_step_ < 0) rng_step_ = (int)prog_.size();
   const T* G = Wb() + rng_base();
   auto hsk = [&](int q) -> SketchJob {
     const HNode& h = s.hnodes[q];
@@ -1166,15 +1210,10 @@ void Engine<T>::build_program() {
   tasks_.clear();
   const int64_t welems = zp_base() + zp_elems_;
   work_.alloc(std::max<int64_t>(welems, 1) * sizeof(T));
-  // Gaussian sketches: the host MT19937 stream, converted on the device
-  if (P.rng_count > 0) {
-    std::vector<double> g(P.rng_count);
-    gaussian_stream(P.opt.seed, P.rng_count, g.data());
-    DBuf tmp;
-    tmp.upload(g, st_);
-    dev::launch_convert<T>(tmp.as<double>(), Wb() + rng_base(), P.rng_count, st_);
-    RSC_HIP(hipStreamSynchronize(st_));
-  }
+  // the sketch entries are copied in by numeric() before step rng_step_
+  rng_step_ = -1;
+  rng_on_device_ = false;
+  start_rng();
   // front descriptors
   std::vector<dev::FrontDev> fr(ns);
   {
@@ -1295,7 +1334,9 @@ void Engine<T>::numeric(const double* Ax) {
   int status0[4] = {0, INT_MAX, 0, INT_MAX};
   RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
                          hipMemcpyHostToDevice, st_));
-  for (const Step& s : prog_) {
+  for (size_t si = 0; si < prog_.size(); ++si) {
+    const Step& s = prog_[si];
+    if ((int)si == rng_step_ && !rng_on_device_) upload_rng();
     switch (s.op) {
       case OP_ASM:
         dev::launch_assemble<T>(F(), dfronts_.as<dev::FrontDev>(),
@@ -1365,6 +1406,10 @@ void Engine<T>::numeric(const double* Ax) {
   int status[4];
   RSC_HIP(hipMemcpyAsync(status, dstatus_.p, sizeof(status), hipMemcpyDeviceToHost, st_));
   RSC_HIP(hipStreamSynchronize(st_));
+  if (rng_on_device_ && rng_host_) {
+    RSC_HIP(hipHostFree(rng_host_));
+    rng_host_ = nullptr;
+  }
   pivot_replacements_ = status[2];
   if (status[3] != INT_MAX)
     throw DeviceError("non-finite factor values first after program step " +

@@ -116,8 +116,6 @@ void launch_assemble(T* arena, const FrontDev* fronts, const AsmTask* tasks,
                      int ntasks, const int* children, const int* relmap,
                      const int* lowp, const int64_t* a_dst, const int64_t* a_src,
                      const double* Ax, hipStream_t s);
-template <typename T>
-void launch_convert(const double* src, T* dst, int64_t count, hipStream_t s);
 
 }  // namespace dev
 }  // namespace rsc

@@ -430,22 +430,6 @@ void launch_convjobs(const ConvJob* jobs, int njobs, hipStream_t s) {
   k_convjobs<<<dim3(64, njobs), 256, 0, s>>>(jobs);
 }
 
-template <typename T>
-__global__ void k_convert(const double* __restrict__ src, T* __restrict__ dst,
-                          int64_t count) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = (T)src[i];
-}
-
-template <typename T>
-void launch_convert(const double* src, T* dst, int64_t count, hipStream_t s) {
-  if (count <= 0) return;
-  int64_t blocks = (count + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  k_convert<T><<<(int)blocks, 256, 0, s>>>(src, dst, count);
-}
-
 #define RSC_INST(T)                                                            \
   template void launch_gemm<T>(bool, bool, const GemmJob*, const GemmTile*,    \
                                int, hipStream_t);                             \
@@ -460,8 +444,7 @@ void launch_convert(const double* src, T* dst, int64_t count, hipStream_t s) {
   template void launch_assemble<T>(T*, const FrontDev*, const AsmTask*, int,   \
                                    const int*, const int*, const int*,         \
                                    const int64_t*, const int64_t*,             \
-                                   const double*, hipStream_t);               \
-  template void launch_convert<T>(const double*, T*, int64_t, hipStream_t);
+                                   const double*, hipStream_t);
 
 RSC_INST(double)
 RSC_INST(float)

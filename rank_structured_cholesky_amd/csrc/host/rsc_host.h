@@ -170,7 +170,11 @@ void replan_ranks(Plan& plan, double diagonal_rank_constant);
 
 // Gaussian sketch stream: MT19937 seeded with `seed`, Box-Muller as in
 // MERSENNETWISTER::randNorm (MERSENNETWISTER.h:170-177), count draws.
+// Runs on up to 16 host threads; the values do not depend on the count.
+// The float form rounds each double draw to nearest.
 void gaussian_stream(uint32_t seed, int64_t count, double* out);
+void gaussian_stream(uint32_t seed, int64_t count, float* out);
+void gaussian_stream_threads(uint32_t seed, int64_t count, double* out, int nthreads);
 
 // Permuted lower triangle of A: for each permuted column, (row, value) with
 // row >= col (Ordering::reorderMatrix with lowerOnly, Ordering.cpp:162).

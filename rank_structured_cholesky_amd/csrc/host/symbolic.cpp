@@ -13,6 +13,7 @@
 // keep their dense (ncols+R) x ncols panel, compressed ones keep only their
 // diagonal-block leaves and low-rank (V, U) pairs.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -21,6 +22,7 @@
 #include <numeric>
 #include <random>
 #include <stdexcept>
+#include <thread>
 
 #include "rsc_host.h"
 
@@ -38,15 +40,123 @@ int block_rank(int nrows, int ncols, double constant, int oversampling,
   return b;
 }
 
-void gaussian_stream(uint32_t seed, int64_t count, double* out) {
-  std::mt19937 g(seed);
-  for (int64_t k = 0; k < count; ++k) {
-    double u1 = ((double)g() + 0.5) * (1.0 / 4294967296.0);
-    double r = std::sqrt(-2.0 * std::log(1.0 - u1));
-    double phi = 2.0 * 3.14159265358979323846264338328 *
-                 ((double)g() * (1.0 / 4294967296.0));
-    out[k] = r * std::cos(phi);
+namespace {
+
+// MT19937 (Matsumoto & Nishimura 1998) in block form: init_genrand seeding
+// and the 624-word refill of MERSENNETWISTER.h:74-78,285-297, written
+// branch-free so the refill loops vectorize.
+struct Mt19937 {
+  static constexpr int N = 624, M = 397;
+  uint32_t mt[N];
+  int idx = N;
+  explicit Mt19937(uint32_t seed) {
+    mt[0] = seed;
+    for (int i = 1; i < N; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
   }
+  static uint32_t twist(uint32_t a, uint32_t b) {
+    uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    return (y >> 1) ^ ((0u - (b & 1u)) & 0x9908b0dfu);
+  }
+  void refill() {
+    for (int i = 0; i < N - M; ++i) mt[i] = mt[i + M] ^ twist(mt[i], mt[i + 1]);
+    for (int i = N - M; i < N - 1; ++i) mt[i] = mt[i + M - N] ^ twist(mt[i], mt[i + 1]);
+    mt[N - 1] = mt[M - 1] ^ twist(mt[N - 1], mt[0]);
+    idx = 0;
+  }
+  static uint32_t temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    return y ^ (y >> 18);
+  }
+  // the next n outputs
+  void fill(uint32_t* out, int64_t n) {
+    while (n > 0) {
+      if (idx == N) refill();
+      int64_t m = std::min<int64_t>(n, N - idx);
+      for (int64_t k = 0; k < m; ++k) out[k] = temper(mt[idx + k]);
+      idx += (int)m;
+      out += m;
+      n -= m;
+    }
+  }
+};
+
+// randNorm on one pair of draws: r from randDblExc (the first draw), phi
+// from randExc (the second), value r cos(phi) (MERSENNETWISTER.h:153-177).
+inline double box_muller(uint32_t a, uint32_t b) {
+  double u1 = ((double)a + 0.5) * (1.0 / 4294967296.0);
+  double r = std::sqrt(-2.0 * std::log(1.0 - u1));
+  double phi = 2.0 * 3.14159265358979323846264338328 * ((double)b * (1.0 / 4294967296.0));
+  return r * std::cos(phi);
+}
+
+// The raw MT19937 stream is sequential; its transform is not.  One thread
+// generates the raw pairs chunk by chunk and publishes its progress, the
+// others transform published chunks; every entry is computed by the same
+// scalar code as a single-threaded loop, so the result does not depend on
+// the thread count.
+template <typename T>
+void gaussian_fill(uint32_t seed, int64_t count, T* out, int nthreads) {
+  if (count <= 0) return;
+  constexpr int64_t kChunk = 1 << 18;
+  const int64_t nchunks = (count + kChunk - 1) / kChunk;
+  nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, nchunks));
+  if (nthreads == 1) {
+    Mt19937 g(seed);
+    uint32_t pair[2];
+    for (int64_t k = 0; k < count; ++k) {
+      g.fill(pair, 2);
+      out[k] = (T)box_muller(pair[0], pair[1]);
+    }
+    return;
+  }
+  std::vector<uint32_t> raw(2 * (size_t)count);
+  std::atomic<int64_t> produced{0};  // chunks whose raw draws are written
+  std::atomic<int64_t> next{0};      // next chunk to transform
+  auto transform = [&](int64_t c) {
+    const int64_t b = c * kChunk, e = std::min(count, b + kChunk);
+    for (int64_t k = b; k < e; ++k) out[k] = (T)box_muller(raw[2 * k], raw[2 * k + 1]);
+  };
+  auto worker = [&]() {
+    for (;;) {
+      int64_t c = next.fetch_add(1);
+      if (c >= nchunks) return;
+      while (produced.load(std::memory_order_acquire) <= c) std::this_thread::yield();
+      transform(c);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker);
+  {
+    Mt19937 g(seed);
+    for (int64_t c = 0; c < nchunks; ++c) {
+      const int64_t b = c * kChunk, e = std::min(count, b + kChunk);
+      g.fill(raw.data() + 2 * b, 2 * (e - b));
+      produced.store(c + 1, std::memory_order_release);
+    }
+  }
+  worker();
+  for (auto& t : pool) t.join();
+}
+
+}  // namespace
+
+int stream_threads() {
+  unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hw, 16u));
+}
+
+void gaussian_stream(uint32_t seed, int64_t count, double* out) {
+  gaussian_fill<double>(seed, count, out, stream_threads());
+}
+
+void gaussian_stream(uint32_t seed, int64_t count, float* out) {
+  gaussian_fill<float>(seed, count, out, stream_threads());
+}
+
+void gaussian_stream_threads(uint32_t seed, int64_t count, double* out, int nthreads) {
+  gaussian_fill<double>(seed, count, out, nthreads);
 }
 
 LowerCsc permuted_lower(const Csc& A, const std::vector<int>& pinv) {

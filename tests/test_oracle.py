@@ -63,6 +63,19 @@ def test_box_muller_matches_formula_and_device_stream():
     assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
 
 
+def test_threaded_stream_matches_oracle_across_chunks():
+    # the library draws raw pairs on one thread and transforms chunks of
+    # 2^18 entries on others; several chunks and a ragged tail must give the
+    # oracle's single-threaded stream bit for bit
+    n = 3 * (1 << 18) + 12345
+    dev = np.empty(n)
+    _lib.check(_lib.lib().rsc_gaussian_stream(152, n, _lib.dptr(dev)))
+    assert np.array_equal(dev, gaussian(152, n))
+    small = np.empty(5)
+    _lib.check(_lib.lib().rsc_gaussian_stream(152, 5, _lib.dptr(small)))
+    assert np.array_equal(small, dev[:5])
+
+
 def test_partition_golden():
     gold = json.load(open(os.path.join(GOLD, "partition_poisson2d_8.json")))
     A, xyz = R.poisson2d(8)

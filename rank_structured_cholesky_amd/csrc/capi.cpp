@@ -643,13 +643,17 @@ static rsc_status selftest_gemm_t(int ta, int tb, int m, int n, int k, double al
   rsc::dev::GemmJob J{da.p, db.p, dc.p, m, n, k, lda, ldb, ldc, alpha, beta};
   std::vector<rsc::dev::GemmJob> jobs{J};
   std::vector<rsc::dev::GemmTile> tiles;
-  for (int a = 0; a < (m + rsc::dev::TILE - 1) / rsc::dev::TILE; ++a)
-    for (int b = 0; b < (n + rsc::dev::TILE - 1) / rsc::dev::TILE; ++b)
-      tiles.push_back({0, a, b});
+  int ncnt = 0;
+  const int parts = rsc::dev::plan_gemm_tiles(jobs.data(), 1, false, tiles, &ncnt);
+  rsc::DBuf dpart, dcnt;
+  dpart.alloc((size_t)std::max(parts, 1) * rsc::dev::TILE * rsc::dev::TILE * sizeof(T));
+  dcnt.alloc((size_t)std::max(ncnt, 1) * sizeof(unsigned));
+  RSC_HIP(hipMemset(dcnt.p, 0, dcnt.bytes));
   dj.upload(jobs, s);
   dt.upload(tiles, s);
   rsc::dev::launch_gemm<T>(ta != 0, tb != 0, dj.as<rsc::dev::GemmJob>(),
-                           dt.as<rsc::dev::GemmTile>(), (int)tiles.size(), s);
+                           dt.as<rsc::dev::GemmTile>(), (int)tiles.size(), dpart.p,
+                           dcnt.as<unsigned>(), s);
   RSC_HIP(hipGetLastError());
   RSC_HIP(hipMemcpy(hc.data(), dc.p, csz * sizeof(T), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < csz; ++i) Cm[i] = (double)hc[i];

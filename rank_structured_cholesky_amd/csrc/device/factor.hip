@@ -194,6 +194,9 @@ class Engine final : public FactorEngine {
   std::vector<Step> prog_;
   std::vector<dev::GemmJob> gjobs_;
   std::vector<dev::GemmTile> gtiles_;
+  int ngcnt_ = 0;            // split-k tile counters of the program
+  int64_t gpart_bytes_ = 0;  // largest split-k partial set of one launch
+  DBuf dgpart_, dgcnt_;
   std::vector<dev::PotrfJob> pjobs_;
   std::vector<dev::TrsmJob> tjobs_;
   std::vector<dev::RowTile> ttiles_;
@@ -275,17 +278,14 @@ template <typename T>
 void Engine<T>::emit_gemm(int op, const std::vector<dev::GemmJob>& jobs,
                           bool lower) {
   Step st{op, 0, (int64_t)gtiles_.size()};
-  for (const auto& J : jobs) {
-    if (J.m <= 0 || J.n <= 0) continue;
-    int jid = (int)gjobs_.size();
-    gjobs_.push_back(J);
-    int tm = (J.m + dev::TILE - 1) / dev::TILE, tn = (J.n + dev::TILE - 1) / dev::TILE;
-    for (int a = 0; a < tm; ++a)
-      for (int b = 0; b < tn; ++b) {
-        if (lower && a < b) continue;
-        gtiles_.push_back({jid, a, b});
-      }
-  }
+  const int base = (int)gjobs_.size();
+  gjobs_.insert(gjobs_.end(), jobs.begin(), jobs.end());
+  const size_t t0 = gtiles_.size();
+  const int parts = dev::plan_gemm_tiles(gjobs_.data() + base, (int)jobs.size(), lower,
+                                         gtiles_, &ngcnt_);
+  for (size_t t = t0; t < gtiles_.size(); ++t) gtiles_[t].job += base;
+  gpart_bytes_ = std::max<int64_t>(gpart_bytes_,
+                                   (int64_t)parts * dev::TILE * dev::TILE * esize());
   st.count = (int)(gtiles_.size() - st.first);
   if (st.count > 0) push(st);
 }
@@ -1197,6 +1197,8 @@ void Engine<T>::build_program() {
   step_ctx_.clear();
   gjobs_.clear();
   gtiles_.clear();
+  ngcnt_ = 0;
+  gpart_bytes_ = 0;
   pjobs_.clear();
   tjobs_.clear();
   ttiles_.clear();
@@ -1311,6 +1313,9 @@ void Engine<T>::build_program() {
   if (debug_check_) add_checks();
   dgjobs_.upload(gjobs_, st_);
   dgtiles_.upload(gtiles_, st_);
+  dgpart_.alloc((size_t)std::max<int64_t>(gpart_bytes_, 8));
+  dgcnt_.alloc((size_t)std::max(ngcnt_, 1) * sizeof(unsigned));
+  RSC_HIP(hipMemsetAsync(dgcnt_.p, 0, dgcnt_.bytes, st_));
   dpjobs_.upload(pjobs_, st_);
   dtjobs_.upload(tjobs_, st_);
   dttiles_.upload(ttiles_, st_);
@@ -1353,10 +1358,12 @@ void Engine<T>::numeric(const double* Ax) {
         bool tb = s.op == OP_GEMM_NT || s.op == OP_GEMM_TT;
         if (s.f64)
           dev::launch_gemm<double>(ta, tb, dgjobs_.as<dev::GemmJob>(),
-                                   dgtiles_.as<dev::GemmTile>() + s.first, s.count, st_);
+                                   dgtiles_.as<dev::GemmTile>() + s.first, s.count,
+                                   dgpart_.p, dgcnt_.as<unsigned>(), st_);
         else
           dev::launch_gemm<T>(ta, tb, dgjobs_.as<dev::GemmJob>(),
-                              dgtiles_.as<dev::GemmTile>() + s.first, s.count, st_);
+                              dgtiles_.as<dev::GemmTile>() + s.first, s.count,
+                              dgpart_.p, dgcnt_.as<unsigned>(), st_);
         break;
       }
       case OP_POTRF:

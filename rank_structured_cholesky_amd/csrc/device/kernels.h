@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cstdint>
 
 namespace rsc {
@@ -20,9 +22,22 @@ struct GemmJob {
   int m, n, k, lda, ldb, ldc;
   double alpha, beta;
 };
+// One workgroup's share of a GEMM: output tile (tm, tn) over k in [k0, k1).
+// ns > 1 splits the k range of one output tile over ns workgroups: each
+// writes its partial tile to partials[part + si], and the last to finish
+// (counted in counters[cnt]) sums the ns partials in index order and writes
+// C, so the result does not depend on which workgroup finishes last.
 struct GemmTile {
   int job, tm, tn;
+  int k0, k1;
+  int ns, si, part, cnt;
 };
+// Tiles of jobs[first_job..] for one launch, k split where the launch has
+// too few output tiles to fill the device.  Counter indices continue from
+// *ncounters; partial slots (TILE*TILE elements each) start at 0 per
+// launch, and their count is returned.
+int plan_gemm_tiles(const GemmJob* jobs, int njobs, bool lower,
+                    std::vector<GemmTile>& tiles, int* ncounters);
 
 // In-place Cholesky of an n x n (n <= TILE) lower block.  tolerant:
 //   0  a non-positive pivot is reported (status[0] = 1, status[1] = node)
@@ -94,7 +109,7 @@ struct AsmTask {
 
 template <typename T>
 void launch_gemm(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
-                 int ntiles, hipStream_t s);
+                 int ntiles, void* partials, unsigned* counters, hipStream_t s);
 template <typename T>
 void launch_potrf(const PotrfJob* jobs, int njobs, int* status, hipStream_t s);
 template <typename T>

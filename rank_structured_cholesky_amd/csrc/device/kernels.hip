@@ -4,6 +4,7 @@
 // -packed-fp32-ops disabled; no packed-FP32 instructions are written here.
 #include "kernels.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace rsc {
@@ -42,10 +43,13 @@ struct Acc<float> {
 // ---------------------------------------------------------------------------
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
-                                              const GemmTile* __restrict__ tiles) {
+                                              const GemmTile* __restrict__ tiles,
+                                              T* __restrict__ partials,
+                                              unsigned* __restrict__ counters) {
   constexpr int KS = 16;
   __shared__ T As[KS][TILE + 1];
   __shared__ T Bs[KS][TILE + 1];
+  __shared__ int last;
   const GemmTile tl = tiles[blockIdx.x];
   const GemmJob J = jobs[tl.job];
   const T* __restrict__ A = (const T*)J.A;
@@ -59,14 +63,15 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 4; ++v) acc[a][b][v] = 0;
-  for (int k0 = 0; k0 < J.k; k0 += KS) {
+  const int kend = tl.k1;
+  for (int k0 = tl.k0; k0 < kend; k0 += KS) {
     for (int e = tid; e < KS * TILE; e += 256) {
       int i, kk;
       if (!TA) { i = e % TILE; kk = e / TILE; }
       else { kk = e % KS; i = e / KS; }
       int gi = m0 + i, gk = k0 + kk;
       T v = 0;
-      if (gi < J.m && gk < J.k)
+      if (gi < J.m && gk < kend)
         v = TA ? A[gk + (int64_t)gi * J.lda] : A[gi + (int64_t)gk * J.lda];
       As[kk][i] = v;
     }
@@ -76,7 +81,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
       else { j = e % TILE; kk = e / TILE; }
       int gj = n0 + j, gk = k0 + kk;
       T v = 0;
-      if (gj < J.n && gk < J.k)
+      if (gj < J.n && gk < kend)
         v = TB ? B[gj + (int64_t)gk * J.ldb] : B[gk + (int64_t)gj * J.ldb];
       Bs[kk][j] = v;
     }
@@ -94,6 +99,32 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
     }
     __syncthreads();
   }
+  if (tl.ns > 1) {
+    // split k: publish this share, and let the last share of the tile sum
+    // all of them in index order
+    T* P = partials + (int64_t)(tl.part + tl.si) * (TILE * TILE);
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int v = 0; v < 4; ++v)
+          P[(wm + a * 16 + Acc<T>::row(lane, v)) + (wn + b * 16 + (lane & 15)) * TILE] =
+              acc[a][b][v];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&counters[tl.cnt], 1u) == (unsigned)(tl.ns - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const T* P0 = partials + (int64_t)tl.part * (TILE * TILE);
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int v = 0; v < 4; ++v) {
+          const int o = (wm + a * 16 + Acc<T>::row(lane, v)) + (wn + b * 16 + (lane & 15)) * TILE;
+          T sum = 0;
+          for (int q = 0; q < tl.ns; ++q) sum += P0[(int64_t)q * (TILE * TILE) + o];
+          acc[a][b][v] = sum;
+        }
+    if (tid == 0) counters[tl.cnt] = 0;  // ready for the next factorization
+  }
   const T alpha = (T)J.alpha, beta = (T)J.beta;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -109,14 +140,60 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
       }
 }
 
+int plan_gemm_tiles(const GemmJob* jobs, int njobs, bool lower,
+                    std::vector<GemmTile>& tiles, int* ncounters) {
+  const size_t first = tiles.size();
+  for (int jb = 0; jb < njobs; ++jb) {
+    const GemmJob& J = jobs[jb];
+    if (J.m <= 0 || J.n <= 0) continue;
+    int tm = (J.m + TILE - 1) / TILE, tn = (J.n + TILE - 1) / TILE;
+    for (int a = 0; a < tm; ++a)
+      for (int b = 0; b < tn; ++b) {
+        if (lower && a < b) continue;
+        tiles.push_back({jb, a, b, 0, J.k, 1, 0, 0, 0});
+      }
+  }
+  // about two workgroups per CU of 256, and at least 256 of k per share
+  const int64_t ntile = (int64_t)(tiles.size() - first);
+  if (ntile == 0 || ntile >= 256) return 0;
+  const int want = (int)std::max<int64_t>(1, 512 / ntile);
+  std::vector<GemmTile> out;
+  int parts = 0;
+  for (size_t t = first; t < tiles.size(); ++t) {
+    GemmTile g = tiles[t];
+    const int k = jobs[g.job].k;
+    int ns = std::min(want, std::min(64, (k + 255) / 256));
+    if (ns <= 1) {
+      out.push_back(g);
+      continue;
+    }
+    int chunk = (k + ns - 1) / ns;
+    chunk = (chunk + 15) / 16 * 16;
+    ns = (k + chunk - 1) / chunk;
+    if (ns <= 1) {
+      out.push_back(g);
+      continue;
+    }
+    const int cnt = (*ncounters)++;
+    for (int q = 0; q < ns; ++q)
+      out.push_back({g.job, g.tm, g.tn, q * chunk, std::min(k, (q + 1) * chunk), ns, q,
+                     parts, cnt});
+    parts += ns;
+  }
+  tiles.resize(first);
+  tiles.insert(tiles.end(), out.begin(), out.end());
+  return parts;
+}
+
 template <typename T>
 void launch_gemm(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
-                 int ntiles, hipStream_t s) {
+                 int ntiles, void* partials, unsigned* counters, hipStream_t s) {
   if (ntiles <= 0) return;
-  if (!ta && !tb) k_gemm<T, false, false><<<ntiles, 256, 0, s>>>(jobs, tiles);
-  else if (ta && !tb) k_gemm<T, true, false><<<ntiles, 256, 0, s>>>(jobs, tiles);
-  else if (!ta && tb) k_gemm<T, false, true><<<ntiles, 256, 0, s>>>(jobs, tiles);
-  else k_gemm<T, true, true><<<ntiles, 256, 0, s>>>(jobs, tiles);
+  T* P = (T*)partials;
+  if (!ta && !tb) k_gemm<T, false, false><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+  else if (ta && !tb) k_gemm<T, true, false><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+  else if (!ta && tb) k_gemm<T, false, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+  else k_gemm<T, true, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,7 +509,7 @@ void launch_convjobs(const ConvJob* jobs, int njobs, hipStream_t s) {
 
 #define RSC_INST(T)                                                            \
   template void launch_gemm<T>(bool, bool, const GemmJob*, const GemmTile*,    \
-                               int, hipStream_t);                             \
+                               int, void*, unsigned*, hipStream_t);           \
   template void launch_potrf<T>(const PotrfJob*, int, int*, hipStream_t);      \
   template void launch_trsm<T>(const TrsmJob*, const RowTile*, int,            \
                                hipStream_t);                                   \

@@ -14,7 +14,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("prec,tol", [(R.RSC_FP64, 1e-13), (R.RSC_FP32, 1e-5)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (70, 33, 19), (128, 64, 200), (5, 130, 64)])
+# the last three split k over several workgroups (few output tiles, long k)
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (70, 33, 19), (128, 64, 200), (5, 130, 64),
+                                   (64, 64, 257), (70, 33, 3001), (200, 150, 20000)])
 def test_gemm_selftest(prec, tol, ta, tb, m, n, k):
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
     A = rng.standard_normal((k, m) if ta else (m, k))

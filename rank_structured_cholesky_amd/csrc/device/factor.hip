@@ -55,6 +55,22 @@ struct Step {
   int f64 = 0;  // 1: run the op in FP64 on FP64 buffers (QR of an FP32 factor)
 };
 
+// A step recorded while the compressed supernodes of one level are captured
+// (one list per supernode); lists are then merged into shared launches.
+struct Pending {
+  int kind;  // OpKind (the GEMM variants carry their transposes)
+  bool lower = false;
+  int f64 = 0;
+  std::string ctx;
+  std::vector<dev::GemmJob> g;
+  std::vector<dev::PotrfJob> p;
+  std::vector<dev::TrsmJob> t;
+  std::vector<dev::CopyJob> c;
+  std::vector<dev::EyeJob> e;
+  std::vector<dev::ConvJob> v;
+  std::vector<dev::ShiftJob> s;
+};
+
 struct CholMat {
   char* A;
   int n, m, ld, node;
@@ -261,6 +277,22 @@ class Engine final : public FactorEngine {
   void emit_potrf(const std::vector<dev::PotrfJob>& jobs);
   void emit_trsm(const std::vector<dev::TrsmJob>& jobs);
   void emit_copy(const std::vector<dev::CopyJob>& jobs);
+  void emit_eye(const std::vector<dev::EyeJob>& jobs);
+  void emit_conv(const std::vector<dev::ConvJob>& jobs);
+  void emit_shift(const std::vector<dev::ShiftJob>& jobs);
+  // Cross-node batching: while capture_ is set the emitters record into it
+  std::vector<Pending>* capture_ = nullptr;
+  Pending& record(int kind) {
+    capture_->emplace_back();
+    Pending& p = capture_->back();
+    p.kind = kind;
+    p.f64 = cur_f64_;
+    p.ctx = ctx_;
+    return p;
+  }
+  void emit_level_compressed(const std::vector<int>& nodes);
+  int64_t inv_cursor_ = 0, qr64_cursor_ = 0;  // scratch of the supernode being emitted
+  std::vector<int64_t> inv_need_, qr64_need_;  // per supernode (T elements)
   void invert(const std::vector<CholMat>& tris);
   void panel_chol(const std::vector<CholMat>& mats);
   void trsm_right(const std::vector<TrsmMat>& mats);
@@ -277,6 +309,14 @@ class Engine final : public FactorEngine {
 template <typename T>
 void Engine<T>::emit_gemm(int op, const std::vector<dev::GemmJob>& jobs,
                           bool lower) {
+  if (capture_) {
+    if (!jobs.empty()) {
+      Pending& p = record(op);
+      p.lower = lower;
+      p.g = jobs;
+    }
+    return;
+  }
   Step st{op, 0, (int64_t)gtiles_.size()};
   const int base = (int)gjobs_.size();
   gjobs_.insert(gjobs_.end(), jobs.begin(), jobs.end());
@@ -293,6 +333,10 @@ void Engine<T>::emit_gemm(int op, const std::vector<dev::GemmJob>& jobs,
 template <typename T>
 void Engine<T>::emit_potrf(const std::vector<dev::PotrfJob>& jobs) {
   if (jobs.empty()) return;
+  if (capture_) {
+    record(OP_POTRF).p = jobs;
+    return;
+  }
   Step st{OP_POTRF, (int)jobs.size(), (int64_t)pjobs_.size()};
   pjobs_.insert(pjobs_.end(), jobs.begin(), jobs.end());
   push(st);
@@ -300,6 +344,10 @@ void Engine<T>::emit_potrf(const std::vector<dev::PotrfJob>& jobs) {
 
 template <typename T>
 void Engine<T>::emit_trsm(const std::vector<dev::TrsmJob>& jobs) {
+  if (capture_) {
+    if (!jobs.empty()) record(OP_TRSM).t = jobs;
+    return;
+  }
   Step st{OP_TRSM, 0, (int64_t)ttiles_.size()};
   for (const auto& J : jobs) {
     if (J.m <= 0) continue;
@@ -313,6 +361,10 @@ void Engine<T>::emit_trsm(const std::vector<dev::TrsmJob>& jobs) {
 
 template <typename T>
 void Engine<T>::emit_copy(const std::vector<dev::CopyJob>& jobs) {
+  if (capture_) {
+    if (!jobs.empty()) record(OP_COPY).c = jobs;
+    return;
+  }
   Step st{OP_COPY, 0, (int64_t)ctiles_.size()};
   for (const auto& J : jobs) {
     if (J.m <= 0 || J.n <= 0) continue;
@@ -328,12 +380,53 @@ void Engine<T>::emit_copy(const std::vector<dev::CopyJob>& jobs) {
 // (upper): X = I, X <- X L^{-T} (blocked triangular solve), copy back.  The
 // solves then apply diagonal blocks as matrix-vector products.
 template <typename T>
+void Engine<T>::emit_eye(const std::vector<dev::EyeJob>& jobs) {
+  if (jobs.empty()) return;
+  if (capture_) {
+    record(OP_EYE).e = jobs;
+    return;
+  }
+  Step st{OP_EYE, 0, (int64_t)etiles_.size()};
+  for (auto& J : jobs) {
+    int jid = (int)ejobs_.size();
+    ejobs_.push_back(J);
+    for (int c = 0; c < J.n; c += dev::TILE) etiles_.push_back({jid, c});
+  }
+  st.count = (int)(etiles_.size() - st.first);
+  if (st.count > 0) push(st);
+}
+
+template <typename T>
+void Engine<T>::emit_conv(const std::vector<dev::ConvJob>& jobs) {
+  if (jobs.empty()) return;
+  if (capture_) {
+    record(OP_CONV).v = jobs;
+    return;
+  }
+  Step st{OP_CONV, (int)jobs.size(), (int64_t)vjobs_.size()};
+  vjobs_.insert(vjobs_.end(), jobs.begin(), jobs.end());
+  push(st);
+}
+
+template <typename T>
+void Engine<T>::emit_shift(const std::vector<dev::ShiftJob>& jobs) {
+  if (jobs.empty()) return;
+  if (capture_) {
+    record(OP_SHIFT).s = jobs;
+    return;
+  }
+  Step st{OP_SHIFT, (int)jobs.size(), (int64_t)sjobs_.size()};
+  sjobs_.insert(sjobs_.end(), jobs.begin(), jobs.end());
+  push(st);
+}
+
+template <typename T>
 void Engine<T>::invert(const std::vector<CholMat>& tris) {
   if (tris.empty()) return;
   std::vector<dev::EyeJob> ej;
   std::vector<TrsmMat> tm;
   std::vector<dev::CopyJob> cp;
-  int64_t off = inv_base();
+  int64_t off = inv_cursor_;
   for (auto& t : tris) {
     T* X = Wb() + off;
     off += (int64_t)t.n * t.n;
@@ -341,15 +434,8 @@ void Engine<T>::invert(const std::vector<CholMat>& tris) {
     tm.push_back({(char*)X, t.n, t.n, t.A, t.n, t.ld});
     cp.push_back({X, t.A, t.n, t.n, t.n, t.ld});
   }
-  if (off - inv_base() > inv_elems_) throw std::runtime_error("inverse scratch too small");
-  Step st{OP_EYE, 0, (int64_t)etiles_.size()};
-  for (auto& J : ej) {
-    int jid = (int)ejobs_.size();
-    ejobs_.push_back(J);
-    for (int c = 0; c < J.n; c += dev::TILE) etiles_.push_back({jid, c});
-  }
-  st.count = (int)(etiles_.size() - st.first);
-  if (st.count > 0) push(st);
+  if (off > inv_base() + inv_elems_) throw std::runtime_error("inverse scratch too small");
+  emit_eye(ej);
   trsm_right(tm);
   emit_copy(cp);
 }
@@ -451,7 +537,7 @@ void Engine<T>::qr(const std::vector<QrMat>& jobs0) {
   if (to64) {
     // FP32 factor: orthonormalize in FP64 (CholeskyQR's guarantees need
     // 11 (m r + r (r+1)) u < 1, which FP32 does not give at these sizes)
-    double* base = (double*)(Wb() + qr64_base());
+    double* base = (double*)(Wb() + qr64_cursor_);
     int64_t off = 0;
     std::vector<dev::ConvJob> cv;
     for (auto& J : jobs) {
@@ -463,10 +549,9 @@ void Engine<T>::qr(const std::vector<QrMat>& jobs0) {
       J.Y = (char*)Y64;
       J.W = (char*)W64;
     }
-    if (2 * off > qr64_elems_) throw std::runtime_error("qr64 scratch too small");
-    Step st{OP_CONV, (int)cv.size(), (int64_t)vjobs_.size()};
-    vjobs_.insert(vjobs_.end(), cv.begin(), cv.end());
-    push(st);
+    if (qr64_cursor_ - qr64_base() + 2 * off > qr64_elems_)
+      throw std::runtime_error("qr64 scratch too small");
+    emit_conv(cv);
     cur_f64_ = 1;
   }
   for (int pass = 0; pass < 3; ++pass) {
@@ -492,11 +577,7 @@ void Engine<T>::qr(const std::vector<QrMat>& jobs0) {
       tr.push_back({J.Y, J.m, J.m, J.W, J.r, J.r});
     }
     emit_gemm(OP_GEMM_TN, gram, false);
-    if (pass == 0) {
-      Step st{OP_SHIFT, (int)sh.size(), (int64_t)sjobs_.size()};
-      sjobs_.insert(sjobs_.end(), sh.begin(), sh.end());
-      push(st);
-    }
+    if (pass == 0) emit_shift(sh);
     panel_chol(ch);
     trsm_right(tr);
   }
@@ -505,9 +586,7 @@ void Engine<T>::qr(const std::vector<QrMat>& jobs0) {
     std::vector<dev::ConvJob> cv;
     for (size_t q = 0; q < jobs.size(); ++q)
       cv.push_back({jobs[q].Y, jobs0[q].Y, (int64_t)jobs[q].m * jobs[q].r, 0, 0});
-    Step st{OP_CONV, (int)cv.size(), (int64_t)vjobs_.size()};
-    vjobs_.insert(vjobs_.end(), cv.begin(), cv.end());
-    push(st);
+    emit_conv(cv);
   }
 }
 
@@ -694,13 +773,22 @@ void Engine<T>::layout_scratch() {
       if (!plan_.nodes[j].compress) unc += (int64_t)plan_.nodes[j].ncols * plan_.nodes[j].ncols;
     inv_elems_ = std::max(inv_elems_, unc);
   }
+  // compressed supernodes of one level are emitted together, each with its
+  // own leaf-inverse and FP64-QR scratch
+  inv_need_.assign(plan_.nodes.size(), 0);
+  qr64_need_.assign(plan_.nodes.size(), 0);
   for (size_t j = 0; j < plan_.nodes.size(); ++j) {
     const Supernode& s = plan_.nodes[j];
     if (!s.compress) continue;
     int64_t lv = 0;
     for (auto& h : s.hnodes)
       if (h.leaf()) lv += (int64_t)(h.r1 - h.r0) * (h.r1 - h.r0);
-    inv_elems_ = std::max(inv_elems_, lv);
+    inv_need_[j] = lv;
+  }
+  for (auto& lv : plan_.levels) {
+    int64_t sum = 0;
+    for (int j : lv) sum += inv_need_[j];
+    inv_elems_ = std::max(inv_elems_, sum);
   }
   for (size_t j = 0; j < plan_.nodes.size(); ++j) {
     const Supernode& s = plan_.nodes[j];
@@ -751,9 +839,10 @@ void Engine<T>::layout_scratch() {
   // max(rows, cols) x r + r x r doubles, counted in T elements
   qr64_elems_ = 0;
   if (sizeof(T) == 4) {
-    int64_t mx = 0;
-    for (const Supernode& s : plan_.nodes) {
+    for (size_t j = 0; j < plan_.nodes.size(); ++j) {
+      const Supernode& s = plan_.nodes[j];
       if (!s.compress) continue;
+      int64_t mx = 0;
       for (const HStep& st : s.hsteps) {
         int64_t sum = 0;
         std::function<void(int)> acc = [&](int q) {
@@ -771,9 +860,99 @@ void Engine<T>::layout_scratch() {
         mx = std::max(mx, sum);
       }
       mx = std::max(mx, (int64_t)std::max(s.R(), s.ncols) * s.rank + (int64_t)s.rank * s.rank);
+      qr64_need_[j] = 2 * mx;
     }
-    qr64_elems_ = 2 * mx;
+    for (auto& lv : plan_.levels) {
+      int64_t sum = 0;
+      for (int j : lv) sum += qr64_need_[j];
+      qr64_elems_ = std::max(qr64_elems_, sum);
+    }
   }
+}
+
+// The compressed supernodes of one level are independent (each writes its
+// own panel, factor storage, update matrix and scratch), so their step lists
+// are recorded separately and merged: each merged launch takes the next step
+// of every supernode whose next step has the same kind, choosing first the
+// kind at the head of the longest remaining list.  Each supernode's own
+// order is kept.  RSC_FACTOR_BATCH=0 emits them one after another instead.
+template <typename T>
+void Engine<T>::emit_level_compressed(const std::vector<int>& nodes) {
+  const char* e = std::getenv("RSC_FACTOR_BATCH");
+  const bool batch = !(e && e[0] == '0') && nodes.size() > 1;
+  int64_t inv_off = inv_base(), q64_off = qr64_base();
+  if (!batch) {
+    for (int j : nodes) {
+      inv_cursor_ = inv_base();
+      qr64_cursor_ = qr64_base();
+      emit_compressed(j);
+    }
+    return;
+  }
+  std::vector<std::vector<Pending>> q(nodes.size());
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    inv_cursor_ = inv_off;
+    qr64_cursor_ = q64_off;
+    inv_off += inv_need_[nodes[i]];
+    q64_off += qr64_need_[nodes[i]];
+    capture_ = &q[i];
+    try {
+      emit_compressed(nodes[i]);
+    } catch (...) {
+      capture_ = nullptr;
+      throw;
+    }
+    capture_ = nullptr;
+  }
+  if (cur_f64_ != 0) throw std::logic_error("unbalanced FP64 section");
+  std::vector<size_t> head(nodes.size(), 0);
+  auto same = [](const Pending& a, const Pending& b) {
+    return a.kind == b.kind && a.lower == b.lower && a.f64 == b.f64;
+  };
+  for (;;) {
+    size_t lead = nodes.size(), most = 0;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+      const size_t left = q[i].size() - head[i];
+      if (left > most) {
+        most = left;
+        lead = i;
+      }
+    }
+    if (lead == nodes.size()) break;
+    Pending m = q[lead][head[lead]];
+    m.g.clear(); m.p.clear(); m.t.clear(); m.c.clear(); m.e.clear(); m.v.clear(); m.s.clear();
+    int merged = 0;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+      if (head[i] >= q[i].size()) continue;
+      const Pending& h = q[i][head[i]];
+      if (!same(h, m)) continue;
+      m.g.insert(m.g.end(), h.g.begin(), h.g.end());
+      m.p.insert(m.p.end(), h.p.begin(), h.p.end());
+      m.t.insert(m.t.end(), h.t.begin(), h.t.end());
+      m.c.insert(m.c.end(), h.c.begin(), h.c.end());
+      m.e.insert(m.e.end(), h.e.begin(), h.e.end());
+      m.v.insert(m.v.end(), h.v.begin(), h.v.end());
+      m.s.insert(m.s.end(), h.s.begin(), h.s.end());
+      ++head[i];
+      ++merged;
+    }
+    cur_f64_ = m.f64;
+    ctx_ = m.ctx + (merged > 1 ? " (+" + std::to_string(merged - 1) + " supernodes)" : "");
+    switch (m.kind) {
+      case OP_GEMM_NN:
+      case OP_GEMM_TN:
+      case OP_GEMM_NT:
+      case OP_GEMM_TT: emit_gemm(m.kind, m.g, m.lower); break;
+      case OP_POTRF: emit_potrf(m.p); break;
+      case OP_TRSM: emit_trsm(m.t); break;
+      case OP_COPY: emit_copy(m.c); break;
+      case OP_EYE: emit_eye(m.e); break;
+      case OP_CONV: emit_conv(m.v); break;
+      case OP_SHIFT: emit_shift(m.s); break;
+      default: throw std::logic_error("unexpected recorded step");
+    }
+  }
+  cur_f64_ = 0;
 }
 
 template <typename T>
@@ -870,12 +1049,7 @@ void Engine<T>::emit_compressed(int j) {
   if (s.off_dinv >= 0) {
     // dense L_D^{-T} = I L_D^{-T} through the tree (leaves still hold L)
     ctx_ = "node " + std::to_string(j) + " dense diagonal inverse";
-    Step st{OP_EYE, 0, (int64_t)etiles_.size()};
-    const int jid = (int)ejobs_.size();
-    ejobs_.push_back({F() + s.off_dinv, c, c});
-    for (int c0 = 0; c0 < c; c0 += dev::TILE) etiles_.push_back({jid, c0});
-    st.count = (int)(etiles_.size() - st.first);
-    push(st);
+    emit_eye({{F() + s.off_dinv, c, c}});
     hsolve_right(s, hodlr_root(s), fp(s.off_dinv), c, c, Tm);
   }
   ctx_ = "node " + std::to_string(j) + " invert leaves";
@@ -1272,6 +1446,7 @@ void Engine<T>::build_program() {
     }
     panel_chol(mats);
     emit_gemm(OP_GEMM_NT, syrk, true);
+    inv_cursor_ = inv_base();
     invert(mats);
     // Partitioned-inverse form of the uncompressed panels: Zp = L21 Lc^{-1}
     // replaces L21 once the Schur update has used it.  The forward solve then
@@ -1306,8 +1481,10 @@ void Engine<T>::build_program() {
       emit_gemm(OP_GEMM_NT, zg, false);
       emit_copy(zc);
     }
+    std::vector<int> comp;
     for (int j : nodes)
-      if (P.nodes[j].compress) emit_compressed(j);
+      if (P.nodes[j].compress) comp.push_back(j);
+    emit_level_compressed(comp);
 
   }
   if (debug_check_) add_checks();

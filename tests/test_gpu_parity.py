@@ -134,6 +134,28 @@ def test_block_pcg_equals_single():
         assert np.array_equal(x, X[:, j])
 
 
+@pytest.mark.parametrize("prec,tol", [(R.RSC_FP64, 1e-11), (R.RSC_FP32, 1e-4)])
+def test_level_batched_factor_matches_node_by_node(monkeypatch, prec, tol):
+    """The compressed supernodes of a level are emitted as shared launches;
+    emitting them one supernode at a time must give the same factor up to
+    the summation order of split-k GEMM shares."""
+    A, xyz = R.poisson3d(20, 1.0, 1234)
+    opts = dict(max_dense_block=63, max_diagonal_block=50, explicit_block_threshold=120,
+                factor_precision=prec)
+    sym = symbolic(A, xyz, max_dense_block=63)
+    comp_levels = sym["level"][sym["compressed"].astype(bool)]
+    assert np.max(np.bincount(comp_levels)) > 1  # some level holds several
+    b = R.uniform_rhs(A._nrow, 5678)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("RSC_FACTOR_BATCH", mode)
+        M = _factor(A, xyz, **opts)
+        x = np.empty_like(b)
+        M.solve(x, b)
+        out[mode] = x
+    assert relerr(out["1"], out["0"]) < tol
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_fp32_factor_apply_close_to_oracle(name):
     """FP32 factor (FP64 vectors): the apply stays within FP32 factor accuracy

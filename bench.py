@@ -43,7 +43,38 @@ def parse():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor leg")
+    ap.add_argument("--traffic-run", action="store_true",
+                    help="for HBM counter passes: factor (FP64), 3 applies, then 3 plain "
+                         "256 MiB copies as the calibration kernel; no timing line")
     return ap.parse_args()
+
+
+def traffic_run(cfg):
+    """The program the FETCH_SIZE / WRITE_SIZE counter passes profile: the
+    apply dispatches (k_apply_df) and a copy kernel of known size."""
+    import torch
+
+    import rank_structured_cholesky_amd as R  # noqa: F401
+    from rank_structured_cholesky_amd import _lib
+    L = _lib.lib()
+    A, xyz, B = build_problem(cfg)
+    M, _ = factor(A, xyz, cfg, 0)
+    n, k = B.shape
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")
+    dY = torch.empty_like(dB)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        _lib.check(L.rsc_apply(M.handle, k, C.c_void_p(dB.data_ptr()), n,
+                               C.c_void_p(dY.data_ptr()), n,
+                               C.c_void_p(stream.cuda_stream)))
+    torch.cuda.synchronize()
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"traffic_run": True, "apply_bytes": M.info()["apply_bytes"] * k,
+                      "nrhs": k}), flush=True)
 
 
 def build_problem(cfg):
@@ -189,6 +220,9 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+    if a.traffic_run:
+        traffic_run(cfg)
+        return
     A, xyz, B = build_problem(cfg)
     g64 = gpu_leg(A, xyz, B, cfg, 0, a.steps, a.warmup, a.eps)
     mixed = None

@@ -173,6 +173,44 @@ rsc_status rsc_block_pcg_solve_host(rsc_factor* f, int k, const double* B,
                                     int ldb, double* X, int ldx, double eps,
                                     int max_iters, rsc_pcg_result* res);
 
+/* ---- request server: continuous batching over resident factors ----------
+ * No counterpart in the reference (its drivers solve one system at a time,
+ * src/pcg_solver.cpp:70-120 and python/pcg_pysolver.cpp:83-101); this is a
+ * caller of the preconditioner and PCG path for serving loops.
+ *
+ * The server owns a pool of column slots per factor (slot memory: b, x and
+ * five PCG vectors of n doubles, plus the apply's per-column buffers).  A
+ * request carries nrhs right-hand sides for one factor; its columns are
+ * admitted into free slots in arrival order as slots free up, every busy
+ * slot advances one PCG iteration per step, and a converged column is
+ * retired (x copied out, true residual computed) while the others go on.
+ * Factors on different streams run concurrently.  Each column's arithmetic
+ * is that of rsc_block_pcg_solve with x0 = 0, so a result does not depend
+ * on which requests share the device with it.
+ *
+ * Factors passed to a server must not be used by any other call until
+ * rsc_server_destroy returns. */
+typedef struct rsc_server rsc_server;
+typedef struct {
+  double eps;          /* relative residual target of every column */
+  int max_iters;       /* per column */
+  int slots;           /* column slots per factor (1..64) */
+} rsc_server_options;
+rsc_status rsc_server_options_default(rsc_server_options* o);  /* 1e-8, 10000, 32 */
+rsc_status rsc_server_create(rsc_factor* const* factors, int nfactors,
+                             const rsc_server_options* opt, rsc_server** out);
+/* B (n x nrhs, ld ldb) and X (n x nrhs, ld ldx) are DEVICE pointers that
+ * stay valid until rsc_server_wait returns for the ticket. */
+rsc_status rsc_server_submit(rsc_server* s, int factor, int nrhs, const double* B,
+                             int ldb, double* X, int ldx, int64_t* ticket);
+/* Blocks until every column of the request is retired; res (nrhs entries,
+ * may be NULL) receives each column's result, seconds = the request's
+ * latency from submission to completion. */
+rsc_status rsc_server_wait(rsc_server* s, int64_t ticket, rsc_pcg_result* res);
+/* Steps run and the largest number of busy slots of any step, so far. */
+rsc_status rsc_server_stats(const rsc_server* s, int64_t* steps, int* max_busy);
+void rsc_server_destroy(rsc_server* s);
+
 /* y = A x on the device (the PCG operator; DEVICE pointers). */
 rsc_status rsc_spmv(rsc_factor* f, const double* x, double* y, void* stream);
 

@@ -307,6 +307,49 @@ class PCG_SOLVER:
         return self.last
 
 
+class RequestServer:
+    """Continuous-batching PCG server over factors resident in HBM (the C
+    ABI rsc_server_*).  Right-hand sides and solutions are DEVICE buffers
+    given by address (e.g. a torch tensor's data_ptr()); each must stay
+    alive until wait() returns for its ticket.  The factors must not be used
+    elsewhere while the server exists."""
+
+    def __init__(self, factors, eps=1e-8, max_iters=10000, slots=32):
+        self._factors = list(factors)  # keep the handles alive
+        hs = (C.c_void_p * len(self._factors))(*[M.handle for M in self._factors])
+        o = _lib.ServerOptions(float(eps), int(max_iters), int(slots))
+        h = C.c_void_p()
+        check(lib().rsc_server_create(hs, len(self._factors), C.byref(o), C.byref(h)))
+        self._h = h
+
+    def submit(self, factor, nrhs, B_ptr, ldb, X_ptr, ldx):
+        t = C.c_int64()
+        check(lib().rsc_server_submit(self._h, int(factor), int(nrhs), C.c_void_p(B_ptr),
+                                      int(ldb), C.c_void_p(X_ptr), int(ldx), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, nrhs):
+        res = (_lib.PcgResult * nrhs)()
+        st = lib().rsc_server_wait(self._h, int(ticket), res)
+        if st not in (_lib.RSC_OK, _lib.RSC_NOT_CONVERGED):
+            check(st)
+        return [dict(iterations=r.iterations, recurrence_relres=r.recurrence_relres,
+                     true_relres=r.true_relres, seconds=r.seconds) for r in res]
+
+    def stats(self):
+        steps, busy = C.c_int64(), C.c_int()
+        check(lib().rsc_server_stats(self._h, C.byref(steps), C.byref(busy)))
+        return dict(steps=steps.value, max_busy=busy.value)
+
+    def close(self):
+        if self._h:
+            lib().rsc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def solve(A, b, positions=None, eps=1e-6, iterations=10000, **options):
     """rschol.solve(A, b) counterpart (rschol.pyx:38-49): factor with
     maxDenseBlock=200 / maxDiagonalBlock=400 and run PCG from zero."""

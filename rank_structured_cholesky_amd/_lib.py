@@ -32,6 +32,10 @@ class PcgResult(C.Structure):
                 ("true_relres", C.c_double), ("seconds", C.c_double)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("eps", C.c_double), ("max_iters", C.c_int), ("slots", C.c_int)]
+
+
 class FactorInfo(C.Structure):
     _fields_ = [
         ("n", C.c_int), ("nsuper", C.c_int), ("ncompressed", C.c_int),
@@ -93,6 +97,12 @@ SIGNATURES = {
     "rsc_uniform_rhs": (_I, [_I64, C.c_uint64, _DP]),
     "rsc_selftest_gemm": (_I, [_I, _I, _I, _I, _I, _I, _D, _DP, _I, _DP, _I, _D, _DP, _I]),
     "pcg_pysolve": (None, [_I, _IP, _IP, _DP, _DP, _DP]),
+    "rsc_server_options_default": (_I, [C.POINTER(ServerOptions)]),
+    "rsc_server_create": (_I, [C.POINTER(_P), _I, C.POINTER(ServerOptions), C.POINTER(_P)]),
+    "rsc_server_submit": (_I, [_P, _I, _I, _P, _I, _P, _I, _I64P]),
+    "rsc_server_wait": (_I, [_P, _I64, C.POINTER(PcgResult)]),
+    "rsc_server_stats": (_I, [_P, _I64P, _IP]),
+    "rsc_server_destroy": (None, [_P]),
 }
 
 _lib = None

@@ -15,22 +15,8 @@
 #include <new>
 #include <stdexcept>
 
-#include "../../include/rsc.h"
-#include "device/factor.h"
-#include "device/pcg.h"
-#include "host/rsc_host.h"
+#include "capi_internal.h"
 
-struct rsc_factor {
-  rsc::Csc A;
-  std::vector<double> xyz;
-  bool has_xyz = false;
-  rsc::Plan plan;
-  hipStream_t stream = nullptr;
-  std::unique_ptr<rsc::FactorEngine> eng;
-  rsc::DevCsr dA;
-  rsc::PcgWork pcgw;
-  rsc_factor_info info{};
-};
 
 struct rsc_symbolic {
   rsc::Csc A;

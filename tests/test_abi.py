@@ -107,3 +107,18 @@ def test_factor_without_gpu_fails_loudly():
     M = R.FactorPreconditioner()
     with pytest.raises(R.RscError):
         M.initialize(A, xyz, 200, 400)
+
+
+def test_server_options_and_bad_input():
+    L = _lib.lib()
+    o = _lib.ServerOptions()
+    _lib.check(L.rsc_server_options_default(C.byref(o)))
+    assert (o.eps, o.max_iters, o.slots) == (1e-8, 10000, 32)
+    h = C.c_void_p()
+    assert L.rsc_server_create(None, 0, C.byref(o), C.byref(h)) == _lib.RSC_BAD_INPUT
+    none = (C.c_void_p * 1)(None)
+    assert L.rsc_server_create(none, 1, C.byref(o), C.byref(h)) == _lib.RSC_BAD_INPUT
+    t = C.c_int64()
+    assert L.rsc_server_submit(None, 0, 1, None, 1, None, 1, C.byref(t)) == _lib.RSC_BAD_INPUT
+    assert L.rsc_server_wait(None, 1, None) == _lib.RSC_BAD_INPUT
+    L.rsc_server_destroy(None)

@@ -1,0 +1,96 @@
+"""Request server (continuous batching): every request's solution, iteration
+count and residuals equal a solo block PCG solve of the same right-hand
+sides bit for bit, whatever else shares the device with it."""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+import rank_structured_cholesky_amd as R
+from rank_structured_cholesky_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-8
+
+
+def _factors():
+    out = []
+    for N, sigma in ((16, 1.0), (20, 1.5)):
+        A, xyz = R.poisson3d(N, sigma, 1234)
+        M = R.FactorPreconditioner()
+        M.initialize(A, xyz, 63, 400)
+        out.append((A, M))
+    return out
+
+
+def _solo(torch, M, B):
+    """rsc_block_pcg_solve from x0 = 0 on device copies of B (n x k)."""
+    n, k = B.shape
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")
+    dX = torch.zeros_like(dB)
+    res = (_lib.PcgResult * k)()
+    st = _lib.lib().rsc_block_pcg_solve(M.handle, k, C.c_void_p(dB.data_ptr()), n,
+                                        C.c_void_p(dX.data_ptr()), n, EPS, 500, res, None)
+    assert st == _lib.RSC_OK
+    torch.cuda.synchronize()
+    return dX.cpu().numpy().T.copy(), [(r.iterations, r.recurrence_relres, r.true_relres)
+                                       for r in res]
+
+
+def test_served_results_equal_solo_solves():
+    torch = pytest.importorskip("torch")
+    fac = _factors()
+    # (factor, number of right-hand sides): more columns than slots in total,
+    # one request larger than the slot pool
+    reqs = [(0, 3), (1, 5), (0, 12), (1, 1), (0, 2), (1, 7)]
+    rng_seed = 100
+    data = []
+    for f, m in reqs:
+        n = fac[f][0]._nrow
+        B = R.uniform_rhs(n * m, rng_seed).reshape(m, n).T.copy()
+        rng_seed += 1
+        data.append(B)
+    solo = [_solo(torch, fac[f][1], B) for (f, _), B in zip(reqs, data)]
+
+    def serve(order, stagger):
+        srv = R.RequestServer([M for _, M in fac], eps=EPS, max_iters=500, slots=8)
+        dev, tickets = {}, {}
+        for i in order:
+            f, m = reqs[i]
+            dB = torch.from_numpy(np.ascontiguousarray(data[i].T)).to("cuda")
+            dX = torch.full_like(dB, float("nan"))
+            torch.cuda.synchronize()
+            n = dB.shape[1]
+            tickets[i] = srv.submit(f, m, dB.data_ptr(), n, dX.data_ptr(), n)
+            dev[i] = (dB, dX)
+            if stagger:
+                time.sleep(0.002)
+        out = {}
+        for i in order:
+            res = srv.wait(tickets[i], reqs[i][1])
+            torch.cuda.synchronize()
+            out[i] = (dev[i][1].cpu().numpy().T.copy(), res)
+        st = srv.stats()
+        srv.close()
+        return out, st
+
+    mixed, st = serve(list(range(len(reqs))), stagger=True)
+    assert st["max_busy"] > 1
+    alone = {}
+    for i in (2, 5):
+        o, _ = serve([i], stagger=False)
+        alone.update(o)
+    reversed_mix, _ = serve(list(reversed(range(len(reqs)))), stagger=False)
+    for i in range(len(reqs)):
+        xs, rs = solo[i]
+        for got in (mixed[i], reversed_mix[i]) + ((alone[i],) if i in alone else ()):
+            x, res = got
+            assert np.array_equal(x, xs), i
+            for r, (it, rec, tru) in zip(res, rs):
+                assert r["iterations"] == it
+                assert r["recurrence_relres"] == rec
+                assert r["true_relres"] == tru
+                assert rec <= EPS
+                assert r["seconds"] > 0

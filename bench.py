@@ -29,6 +29,10 @@ CONFIGS = {
     "poisson3d_64": dict(N=64, sigma=1.0, max_dense_block=255, max_rank=64, nrhs=1),
     "poisson3d_128": dict(N=128, sigma=2.0, max_dense_block=255, max_rank=0, nrhs=16),
     "poisson2d_256": dict(N2=256, max_dense_block=127, max_rank=32, nrhs=1),
+    # request serving over eight resident factors (continuous batching)
+    "serving": dict(grids=(32, 48, 64, 80, 96, 96, 112, 128), sigma=1.0,
+                    max_dense_block=255, max_rank=64, requests=24, max_rhs=256,
+                    rate=20.0, slots=32),
 }
 
 HBM_PEAK_GBS = 8000.0
@@ -43,10 +47,118 @@ def parse():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor leg")
+    ap.add_argument("--requests", type=int, default=0,
+                    help="serving: number of requests (default from the config)")
+    ap.add_argument("--max-rhs", type=int, default=0,
+                    help="serving: largest right-hand-side count of a request")
     ap.add_argument("--traffic-run", action="store_true",
                     help="for HBM counter passes: factor (FP64), 3 applies, then 3 plain "
                          "256 MiB copies as the calibration kernel; no timing line")
     return ap.parse_args()
+
+
+def serving_leg(cfg, eps):
+    """Eight log-normal Poisson systems factored and kept resident; a seeded
+    Poisson stream of requests (matrix uniform, 1..max_rhs uniform[-1,1]
+    right-hand sides each) served (a) by the continuous-batching server and
+    (b) one request at a time in arrival order with block PCG (columns in
+    chunks of the slot count).  Right-hand sides are drawn on the device
+    before the clock starts.  Returns throughput (right-hand sides solved per
+    second) and request latency percentiles of both."""
+    import threading
+
+    import torch
+
+    import rank_structured_cholesky_amd as R
+    from rank_structured_cholesky_amd import _lib
+    L = _lib.lib()
+    mats, facs, tf = [], [], []
+    for i, N in enumerate(cfg["grids"]):
+        A, xyz = R.poisson3d(N, cfg["sigma"], 1234 + i)
+        M, t = factor(A, xyz, cfg, 0)
+        inf = M.info()
+        print("factored %d^3: n %d, %.2f s, factor %.2f GB, workspace %.2f GB" % (
+            N, A._nrow, t, inf["factor_bytes"] / 1e9, inf["workspace_bytes"] / 1e9),
+            file=sys.stderr, flush=True)
+        mats.append(A)
+        facs.append(M)
+        tf.append(t)
+    rng = np.random.default_rng(91011)
+    nreq = cfg["requests"]
+    gaps = rng.exponential(1.0 / cfg["rate"], nreq)
+    arrive = np.cumsum(gaps) - gaps[0]
+    which = rng.integers(0, len(facs), nreq)
+    nrhs = rng.integers(1, cfg["max_rhs"] + 1, nreq)
+    gen = torch.Generator(device="cuda").manual_seed(91011)
+    reqs = []
+    for q in range(nreq):
+        n = mats[which[q]]._nrow
+        B = torch.rand((int(nrhs[q]), n), generator=gen, device="cuda",
+                       dtype=torch.float64) * 2 - 1
+        reqs.append((int(which[q]), int(nrhs[q]), B, torch.zeros_like(B)))
+    torch.cuda.synchronize()
+
+    def pct(v, p):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(round(p * (len(v) - 1))))]
+
+    # (a) continuous batching
+    srv = R.RequestServer(facs, eps=eps, max_iters=10000, slots=cfg["slots"])
+    tickets = [None] * nreq
+    t0 = time.perf_counter()
+
+    def client():
+        for q in range(nreq):
+            d = t0 + arrive[q] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            f, m, B, X = reqs[q]
+            tickets[q] = srv.submit(f, m, B.data_ptr(), B.shape[1], X.data_ptr(), X.shape[1])
+
+    th = threading.Thread(target=client)
+    th.start()
+    lat_a, iters_a, worst = [], [], 0.0
+    for q in range(nreq):
+        while tickets[q] is None:
+            time.sleep(0.0005)
+        res = srv.wait(tickets[q], reqs[q][1])
+        lat_a.append(res[0]["seconds"])
+        iters_a += [r["iterations"] for r in res]
+        worst = max(worst, max(r["true_relres"] for r in res))
+    th.join()
+    wall_a = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    # (b) one request at a time (FIFO, no overlap between requests)
+    lat_b = []
+    stream = torch.cuda.Stream()
+    t0 = time.perf_counter()
+    for q in range(nreq):
+        d = t0 + arrive[q] - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        f, m, B, X = reqs[q]
+        X.zero_()
+        torch.cuda.synchronize()
+        n = B.shape[1]
+        for c0 in range(0, m, cfg["slots"]):
+            k = min(cfg["slots"], m - c0)
+            res = (_lib.PcgResult * k)()
+            L.rsc_block_pcg_solve(facs[f].handle, k, C.c_void_p(B[c0:].data_ptr()), n,
+                                  C.c_void_p(X[c0:].data_ptr()), n, eps, 10000, res,
+                                  C.c_void_p(stream.cuda_stream))
+        lat_b.append(time.perf_counter() - (t0 + arrive[q]))
+    wall_b = time.perf_counter() - t0
+    total = int(nrhs.sum())
+    return dict(
+        rhs_total=total, requests=nreq, wall_s=wall_a, solves_per_s=total / wall_a,
+        p50_ms=1e3 * pct(lat_a, 0.5), p99_ms=1e3 * pct(lat_a, 0.99),
+        steps=st["steps"], max_busy_slots=st["max_busy"], worst_true_relres=worst,
+        iterations_mean=float(np.mean(iters_a)),
+        one_at_a_time=dict(wall_s=wall_b, solves_per_s=total / wall_b,
+                           p50_ms=1e3 * pct(lat_b, 0.5), p99_ms=1e3 * pct(lat_b, 0.99)),
+        factor_s=[round(t, 3) for t in tf],
+        n=[m._nrow for m in mats])
 
 
 def traffic_run(cfg):
@@ -222,6 +334,23 @@ def main():
         sys.exit(1)
     if a.traffic_run:
         traffic_run(cfg)
+        return
+    if a.config == "serving":
+        cfg = dict(cfg)
+        if a.requests:
+            cfg["requests"] = a.requests
+        if a.max_rhs:
+            cfg["max_rhs"] = a.max_rhs
+        r = serving_leg(cfg, a.eps)
+        print(json.dumps({
+            "metric": "served PCG solves/s to 1e-8 (right-hand sides per second), "
+                      "p50/p99 request latency",
+            "value": round(r["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "serving", "grids": list(cfg["grids"]),
+                       "requests": cfg["requests"], "max_rhs": cfg["max_rhs"],
+                       "rate_per_s": cfg["rate"], "slots_per_factor": cfg["slots"],
+                       "eps": a.eps}, **r}))
         return
     A, xyz, B = build_problem(cfg)
     g64 = gpu_leg(A, xyz, B, cfg, 0, a.steps, a.warmup, a.eps)

@@ -45,8 +45,8 @@ rsc_status guard(F&& f) {
     return fail(RSC_INDEFINITE, e.what());
   } catch (const rsc::DeviceError& e) {
     return fail(RSC_DEVICE_ERROR, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(RSC_OOM, "allocation failed");
+  } catch (const std::bad_alloc& e) {
+    return fail(RSC_OOM, std::string("allocation failed: ") + e.what());
   } catch (const std::invalid_argument& e) {
     return fail(RSC_BAD_INPUT, e.what());
   } catch (const std::exception& e) {

@@ -26,6 +26,14 @@ struct IndefiniteError : std::runtime_error {
 };
 
 // Owning device buffer.
+// A device allocation that failed (an out-of-memory condition: RSC_OOM).
+struct DeviceOom : std::bad_alloc {
+  std::string msg;
+  explicit DeviceOom(size_t bytes)
+      : msg("device allocation of " + std::to_string(bytes) + " bytes failed") {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -44,7 +52,8 @@ struct DBuf {
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
       p = nullptr;
-      throw std::bad_alloc();
+      (void)hipGetLastError();
+      throw DeviceOom(b);
     }
     bytes = b;
   }

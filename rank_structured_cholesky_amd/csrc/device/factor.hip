@@ -1385,6 +1385,12 @@ void Engine<T>::build_program() {
   vjobs_.clear();
   tasks_.clear();
   const int64_t welems = zp_base() + zp_elems_;
+  if (P.opt.verbose)
+    std::printf("workspace elements: work %lld update %lld scratch %lld rng %lld inverse %lld "
+                "qr64 %lld zp %lld\n",
+                (long long)P.work_elems, (long long)P.update_elems, (long long)scratch_elems_,
+                (long long)P.rng_count, (long long)inv_elems_, (long long)qr64_elems_,
+                (long long)zp_elems_);
   work_.alloc(std::max<int64_t>(welems, 1) * sizeof(T));
   // the sketch entries are copied in by numeric() before step rng_step_
   rng_step_ = -1;

@@ -167,6 +167,9 @@ int hodlr_root(const Supernode& s);
 // Recompute ranks / storage / RNG layout for a new diagonal rank constant
 // without redoing the ordering (SparseSolver.cpp:192-204).
 void replan_ranks(Plan& plan, double diagonal_rank_constant);
+// Offsets of the update matrices and compressed work panels, reused across
+// levels (sets update_elems and work_elems).
+void place_transient(Plan& plan);
 
 // Gaussian sketch stream: MT19937 seeded with `seed`, Box-Muller as in
 // MERSENNETWISTER::randNorm (MERSENNETWISTER.h:170-177), count draws.

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <functional>
+#include <map>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -278,9 +279,75 @@ int hodlr_root(const Supernode& s) {
   return 0;
 }
 
+namespace {
+
+// First-fit offsets in one region, with coalescing frees.
+struct FirstFit {
+  std::map<int64_t, int64_t> holes;  // offset -> length
+  int64_t top = 0;
+  int64_t take(int64_t len) {
+    for (auto it = holes.begin(); it != holes.end(); ++it)
+      if (it->second >= len) {
+        const int64_t off = it->first, rest = it->second - len;
+        holes.erase(it);
+        if (rest > 0) holes[off + len] = rest;
+        return off;
+      }
+    const int64_t off = top;
+    top += len;
+    return off;
+  }
+  void give(int64_t off, int64_t len) {
+    auto it = holes.emplace(off, len).first;
+    auto nx = std::next(it);
+    if (nx != holes.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      holes.erase(nx);
+    }
+    if (it != holes.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        holes.erase(it);
+      }
+    }
+  }
+};
+
+}  // namespace
+
+// Update matrices (R x R) and compressed supernodes' work panels live only
+// while the level program needs them.  Level L's assembly launch zeroes and
+// writes the fronts and update matrices of its supernodes while it reads
+// their children's update matrices, so level L's regions are taken while
+// the children's are still held; the children's updates and level L's work
+// panels are given back after level L, for the levels above to reuse (the
+// program runs in level order on one stream).
+void place_transient(Plan& plan) {
+  FirstFit U, W;
+  for (auto& lv : plan.levels) {
+    for (int j : lv) {
+      Supernode& s = plan.nodes[j];
+      const int64_t c = s.ncols, R = s.R();
+      s.off_update = R > 0 ? U.take(R * R) : -1;
+      s.off_work_panel = s.compress ? W.take((c + R) * c) : -1;
+    }
+    for (int j : lv) {
+      const Supernode& s = plan.nodes[j];
+      for (int ch : s.children) {
+        const Supernode& cs = plan.nodes[ch];
+        if (cs.R() > 0) U.give(cs.off_update, (int64_t)cs.R() * cs.R());
+      }
+      if (s.compress) W.give(s.off_work_panel, (int64_t)(s.ncols + s.R()) * s.ncols);
+    }
+  }
+  plan.update_elems = U.top;
+  plan.work_elems = W.top;
+}
+
 void replan_ranks(Plan& plan, double diag_const) {
   const rsc_options& o = plan.opt;
-  int64_t foff = 0, woff = 0, uoff = 0, rng = 0;
+  int64_t foff = 0, rng = 0;
   double flops = 0, apply = 0;
   const int64_t es = o.factor_precision == RSC_FP32 ? 4 : 8;
   int64_t dense_root_bytes = (int64_t)512 << 20;
@@ -288,19 +355,14 @@ void replan_ranks(Plan& plan, double diag_const) {
   auto potrf_f = [](double c) { return c * c * c / 3.0 + c * c / 2.0 + c / 6.0; };
   for (auto& s : plan.nodes) {
     const int64_t c = s.ncols, R = s.R();
-    s.off_update = R > 0 ? uoff : -1;
-    uoff += R * R;
     if (!s.compress) {
       s.off_panel = foff;
       foff += (c + R) * c;
-      s.off_work_panel = -1;
       flops += potrf_f((double)c) + (double)R * c * c + (double)c * R * (R + 1);
       apply += 2.0 * ((double)c * (c + 1) / 2 + (double)R * c);
       continue;
     }
     s.off_panel = -1;
-    s.off_work_panel = woff;
-    woff += (c + R) * c;
     double tree_apply = 0;
     // diagonal tree: ranks, leaf/V/U storage, RNG offsets in draw order
     for (auto& h : s.hnodes) {
@@ -393,8 +455,7 @@ void replan_ranks(Plan& plan, double diag_const) {
     }
   }
   plan.factor_elems = foff;
-  plan.work_elems = woff;
-  plan.update_elems = uoff;
+  place_transient(plan);
   plan.rng_count = rng;
   plan.factor_flops = flops;
   plan.apply_bytes_per_elem = apply;

@@ -117,7 +117,7 @@ class Engine final : public FactorEngine {
 
   void numeric(const double* Ax) override;
   void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
-             const int* live) override;
+             const int* live, int multi_min_k) override;
   void check_apply(hipStream_t s) override;
   int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
   int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
@@ -1279,8 +1279,9 @@ void Engine<T>::setup_static() {
   // solve: vector layout, level-scheduled and dataflow task lists
   SolveLayout SL;
   {
-    // subtree tasks: about 4096 subtrees' worth of panel storage each
-    int64_t lim = std::max<int64_t>(16384, P.factor_elems / 4096);
+    // subtree tasks: up to 16384 panel elements each (measured best for one
+    // and for sixteen right-hand sides at 64^3 and 128^3)
+    int64_t lim = 16384;
     if (const char* e = std::getenv("RSC_SUBTREE_ELEMS")) lim = std::atoll(e);
     build_solve_layout(P, SL, lim);
   }
@@ -1613,7 +1614,7 @@ void Engine<T>::numeric(const double* Ax) {
 
 template <typename T>
 void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
-                      hipStream_t s, const int* live) {
+                      hipStream_t s, const int* live, int multi_min_k) {
   const int n = plan_.n;
   if (k > fv_k_) {
     dfv_.alloc((size_t)fv_stride_ * k * sizeof(double));
@@ -1625,7 +1626,7 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   }
   dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(),
                   dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_,
-                  live, dz_.as<double>()};
+                  live, dz_.as<double>(), multi_min_k};
   if (use_dataflow_) {
     dev::DfArgs a{};
     a.tasks = ddf_.as<dev::DfTask>();

@@ -523,7 +523,7 @@ void ServeBatch::step(FactorEngine& M, const DevCsr& A, int kact, double eps, in
   k_scalars<<<kact, RB, 0, st>>>(2, part1.as<double>(), nullptr, nb, c, eps, max_iters);
   k_update_xr<<<dim3(ublocks, kact), 256, 0, st>>>(n, kact, c, X, n, R, D, Q);
   // INIT and RUN columns: s = M^{-1} r, r.s, r.r
-  M.apply(kact, R, n, S, n, st, anyb.as<int>());
+  M.apply(kact, R, n, S, n, st, anyb.as<int>(), 1);
   k_dot_masked<<<dim3(nb, kact), RB, 0, st>>>(n, R, n, S, n, part1.as<double>(), BUSY);
   k_dot_masked<<<dim3(nb, kact), RB, 0, st>>>(n, R, n, R, n, part2.as<double>(), BUSY);
   k_serve_scalars<<<kact, RB, 0, st>>>(part1.as<double>(), part2.as<double>(), nb, c,

@@ -26,6 +26,9 @@ struct VecBufs {
   int64_t n, fvs, tvs, ps;
   const int* live;  // when non-null and *live == 0 every kernel of the apply returns
   double* z;        // z = L^{-T} y per supernode, stride n per column
+  // dataflow matrix-vector tasks use the matrix-core bodies (sixteen
+  // right-hand sides per matrix pass) when k >= multi_min_k
+  int multi_min_k = 2;
 };
 
 // forward: fv_c = w[cols], fv_R = 0, plus children's fv_R via relmap

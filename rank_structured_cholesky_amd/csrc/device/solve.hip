@@ -64,6 +64,7 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
 // chunk adds, child by child, the child entries whose parent position falls
 // in the chunk: deterministic, no atomics.
 constexpr int VCH = 1024;
+constexpr int kAsmRhs = 4;  // right-hand sides per LDS pass of a front-vector assembly
 
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -138,32 +139,47 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
                                const int* __restrict__ children,
                                const int* __restrict__ relmap, const VecBufs& b, int k,
                                const double* __restrict__ src, int64_t lds,
-                               const int* __restrict__ perm) {
+                               const int* __restrict__ perm, double* __restrict__ sm) {
+  // The chunk of kAsmRhs right-hand sides at a time is summed in LDS (sm:
+  // kAsmRhs * VCH doubles): its own entries, then child by child (a child's
+  // rows map to distinct positions), then written once.  Each entry's sum
+  // order is the same for any k.
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
   const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
   if (p0 >= c + R) return;
-  for (int col = 0; col < k; ++col) {
-    double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
-    const double* sc = src + (int64_t)col * lds;
-    for (int i = p0 + tid; i < p1; i += 256)
-      vst<COH>(f + i, i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0);
+  const int len = p1 - p0;
+  for (int c0 = 0; c0 < k; c0 += kAsmRhs) {
+    const int g = min(kAsmRhs, k - c0);
+    for (int idx = tid; idx < g * len; idx += 256) {
+      const int cc = idx / len, i = p0 + (idx - cc * len);
+      const double* sc = src + (int64_t)(c0 + cc) * lds;
+      sm[cc * VCH + (i - p0)] = i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0;
+    }
     __syncthreads();
     for (int q = 0; q < S.nchild; ++q) {
       const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
       const int* rm = relmap + C.relmap_off;
-      const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
-      // one predicated pass over the child's rows (no dependent searches)
-      for (int a = tid; a < C.R; a += 256) {
+      const int cr = C.R;
+      for (int idx = tid; idx < g * cr; idx += 256) {
+        const int cc = idx / cr, a = idx - cc * cr;
         const int p = rm[a];
-        if (p >= p0 && p < p1) vst<COH>(f + p, vld<COH>(f + p) + vld<COH>(fc + a));
+        if (p >= p0 && p < p1)
+          sm[cc * VCH + (p - p0)] +=
+              vld<COH>(b.fv + C.fv_off + (int64_t)(c0 + cc) * b.fvs + C.ncols + a);
       }
       __syncthreads();
     }
+    for (int idx = tid; idx < g * len; idx += 256) {
+      const int cc = idx / len, i = idx - cc * len;
+      vst<COH>(b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + p0 + i, sm[cc * VCH + i]);
+    }
+    __syncthreads();
   }
 }
 
 // Backward gather of one chunk: fv_R[a] = w[rows[a]] (the solution at the
-// node's off-diagonal rows) and w[cols] = z[cols] (start of x_c).
+// node's off-diagonal rows) and w[cols] = z[cols] (start of x_c), all
+// right-hand sides in one pass.
 template <bool COH>
 __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
                              const VecBufs& b, int k) {
@@ -171,14 +187,12 @@ __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict_
   const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
   if (p0 >= c + R) return;
   const int* rw = rows + S.rows_off;
-  for (int col = 0; col < k; ++col) {
-    double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
+  const int len = p1 - p0;
+  for (int idx = tid; idx < k * len; idx += 256) {
+    const int col = idx / len, i = p0 + (idx - col * len);
     double* wc = b.w + (int64_t)col * b.n;
-    const double* zc = b.z + (int64_t)col * b.n;
-    for (int i = p0 + tid; i < p1; i += 256) {
-      if (i < c) vst<COH>(wc + S.first + i, vld<COH>(zc + S.first + i));
-      else vst<COH>(f + i, vld<COH>(wc + rw[i - c]));
-    }
+    if (i < c) vst<COH>(wc + S.first + i, vld<COH>(b.z + (int64_t)col * b.n + S.first + i));
+    else vst<COH>(b.fv + S.fv_off + (int64_t)col * b.fvs + i, vld<COH>(wc + rw[i - c]));
   }
 }
 
@@ -187,9 +201,10 @@ __global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict_
                                                       const int* __restrict__ children,
                                                       const int* __restrict__ relmap,
                                                       VecBufs b, int k) {
+  __shared__ double sm[kAsmRhs * VCH];
   if (b.live && *b.live == 0) return;
   const VecNode S = nodes[lnodes[blockIdx.x]];
-  assemble_chunk<false>(S, blockIdx.y, nodes, children, relmap, b, k, b.w, b.n, nullptr);
+  assemble_chunk<false>(S, blockIdx.y, nodes, children, relmap, b, k, b.w, b.n, nullptr, sm);
 }
 
 __global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ nodes,
@@ -509,11 +524,133 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
   }
 }
 
+// Matrix-vector task bodies for several right-hand sides on the FP64 matrix
+// cores: sixteen right-hand sides per pass over the matrix block, as one
+// MFMA 16x16x4 product per four matrix columns (ROW) or rows (COL).  Lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; accumulator element v of lane l
+// is D[l/16 + 4v][l%16].  A right-hand side's sums run in the same order for
+// any number of right-hand sides and any position in the group of sixteen.
+//   ROW (rows <= 64): wave w owns rows 16w..16w+15; x is staged in LDS in
+//     chunks of 128 columns (16 x 129 doubles).
+//   COL (cols <= 32): the 64-row matrix tile is staged transposed in LDS
+//     (64 x 33 doubles) with x (16 x 65); wave w takes columns 16(w&1).. and
+//     rows 32(w>>1).. of the tile; the two row halves are added in order.
+typedef double d4v __attribute__((ext_vector_type(4)));
+constexpr int kMfmaRhs = 16;
+constexpr int kRowChunk = 128;
+constexpr int kMfmaLds = 64 * 33 + kMfmaRhs * 65;  // doubles of LDS scratch (COL)
+
+__device__ inline d4v mfma64(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T, bool COH>
+__device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBufs& b, int k,
+                          double* sm) {
+  const T* __restrict__ M = F + t.mat;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, kq = lane >> 4;
+  for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
+    const int kb = min(kMfmaRhs, k - c0);
+    if (t.kind == 0) {
+      constexpr int LD = kRowChunk + 1;
+      const int r = wave * 16 + li;
+      const bool rok = r < t.rows;
+      const bool wact = wave * 16 < t.rows;
+      const T* __restrict__ mrow = M + (rok ? r : 0);
+      d4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+      for (int jb = 0; jb < t.cols; jb += kRowChunk) {
+        const int nb = min(kRowChunk, t.cols - jb);
+        for (int idx = tid; idx < kMfmaRhs * kRowChunk; idx += 256) {
+          const int cc = idx / kRowChunk, jj = idx - cc * kRowChunk;
+          sm[cc * LD + jj] = (cc < kb && jj < nb)
+                                 ? vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + jb + jj)
+                                 : 0.0;
+        }
+        __syncthreads();
+        if (wact) {
+          const double* xr = sm + li * LD + kq;
+#pragma unroll 2
+          for (int j = 0; j < nb; j += 8) {
+            const int ja = j + kq, jc = j + 4 + kq;
+            const double a0 = (rok && ja < nb) ? (double)mrow[(int64_t)(jb + ja) * t.ld] : 0.0;
+            const double a1 = (rok && jc < nb) ? (double)mrow[(int64_t)(jb + jc) * t.ld] : 0.0;
+            acc0 = mfma64(a0, xr[j], acc0);
+            acc1 = mfma64(a1, xr[j + 4], acc1);
+          }
+        }
+        __syncthreads();
+      }
+      if (wact)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = wave * 16 + kq + 4 * v;
+          if (row < t.rows && li < kb) {
+            double* y = vsel(b, t.ysel, c0 + li) + t.yoff + row;
+            const double val = acc0[v] + acc1[v];
+            vst<COH>(y, t.op ? vld<COH>(y) - val : val);
+          }
+        }
+    } else {
+      double* Ms = sm;             // [64][33]: row-major transposed tile
+      double* xs = sm + 64 * 33;   // [16][65]
+      const int g = wave & 1, h = wave >> 1;
+      d4v acc = {0, 0, 0, 0};
+      for (int rb = 0; rb < t.rows; rb += 64) {
+        const int nr = min(64, t.rows - rb);
+        for (int idx = tid; idx < 64 * 32; idx += 256) {
+          const int rr = idx & 63, cc = idx >> 6;
+          Ms[rr * 33 + cc] = (rr < nr && cc < t.cols)
+                                 ? (double)M[(int64_t)(rb + rr) + (int64_t)cc * t.ld]
+                                 : 0.0;
+        }
+        for (int idx = tid; idx < kMfmaRhs * 64; idx += 256) {
+          const int rr = idx & 63, cc = idx >> 6;
+          xs[cc * 65 + rr] = (rr < nr && cc < kb)
+                                 ? vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + rb + rr)
+                                 : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 32; q += 4) {
+          const int rr = 32 * h + q + kq;
+          acc = mfma64(Ms[rr * 33 + 16 * g + li], xs[li * 65 + rr], acc);
+        }
+        __syncthreads();
+      }
+      // rows 32..63 of every tile (h = 1) are added to rows 0..31 (h = 0)
+      if (h == 1)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sm[(g * 4 + v) * 64 + lane] = acc[v];
+      __syncthreads();
+      if (h == 0)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double val = acc[v] + sm[(g * 4 + v) * 64 + lane];
+          const int col = 16 * g + kq + 4 * v;
+          if (col < t.cols && li < kb) {
+            if (t.part >= 0)
+              vst<COH>(b.part + (int64_t)(c0 + li) * b.ps + (int64_t)t.part * 32 + col, val);
+            else {
+              double* y = vsel(b, t.ysel, c0 + li) + t.yoff + col;
+              vst<COH>(y, t.op ? vld<COH>(y) - val : val);
+            }
+          }
+        }
+      __syncthreads();
+    }
+  }
+}
+
 template <typename T, bool COH>
 __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBufs& b, int k,
                           double (*red)[64], double* xl) {
   const GvTask t = uniform_task(t0);
   if (t.kind != 2) {
+    if (k >= b.multi_min_k) {
+      gemv_mfma<T, COH>(F, t, b, k, xl);
+      return;
+    }
     // chunks of NB = kXlds / KRHS (multiples of 64) keep every column's sum
     // order the same for one and four right-hand sides per group
     if (k == 1) gemv_multi<T, COH, 1>(F, t, b, k, red, xl);
@@ -824,6 +961,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
   __shared__ union {
     SubtreeLds sl;
     double xl[kXlds];
+    double mf[kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH];
   } lds;
   __shared__ int s_ticket;
   __shared__ int s_ok;
@@ -856,11 +994,11 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
       assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b, k,
-                           a.B, a.ldb, a.perm);
+                           a.B, a.ldb, a.perm, lds.mf);
     } else if (type == 1) {
       gather_chunk<true>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
-      gemv_lite<T, true>(F, a.gv[ta], b, k, red, lds.xl);
+      gemv_lite<T, true>(F, a.gv[ta], b, k, red, lds.mf);
     } else if (type == 4) {
       subtree_forward<T, true>(F, a.sub, ta, b, k, lds.sl);
     } else if (type == 5) {

@@ -281,36 +281,54 @@ int hodlr_root(const Supernode& s) {
 
 namespace {
 
-// First-fit offsets in one region, with coalescing frees.
+// Best-fit offsets in one region (holes indexed by length and by offset),
+// with coalescing frees.
 struct FirstFit {
-  std::map<int64_t, int64_t> holes;  // offset -> length
+  std::map<int64_t, int64_t> holes;              // offset -> length
+  std::multimap<int64_t, int64_t> by_len;        // length -> offset
   int64_t top = 0;
-  int64_t take(int64_t len) {
-    for (auto it = holes.begin(); it != holes.end(); ++it)
-      if (it->second >= len) {
-        const int64_t off = it->first, rest = it->second - len;
-        holes.erase(it);
-        if (rest > 0) holes[off + len] = rest;
-        return off;
+  void erase_len(int64_t len, int64_t off) {
+    auto r = by_len.equal_range(len);
+    for (auto it = r.first; it != r.second; ++it)
+      if (it->second == off) {
+        by_len.erase(it);
+        return;
       }
-    const int64_t off = top;
-    top += len;
+  }
+  int64_t take(int64_t len) {
+    auto it = by_len.lower_bound(len);
+    if (it == by_len.end()) {
+      const int64_t off = top;
+      top += len;
+      return off;
+    }
+    const int64_t hl = it->first, off = it->second;
+    by_len.erase(it);
+    holes.erase(off);
+    if (hl > len) {
+      holes[off + len] = hl - len;
+      by_len.emplace(hl - len, off + len);
+    }
     return off;
   }
   void give(int64_t off, int64_t len) {
     auto it = holes.emplace(off, len).first;
     auto nx = std::next(it);
     if (nx != holes.end() && it->first + it->second == nx->first) {
+      erase_len(nx->second, nx->first);
       it->second += nx->second;
       holes.erase(nx);
     }
     if (it != holes.begin()) {
       auto pv = std::prev(it);
       if (pv->first + pv->second == it->first) {
+        erase_len(pv->second, pv->first);
         pv->second += it->second;
         holes.erase(it);
+        it = pv;
       }
     }
+    by_len.emplace(it->second, it->first);
   }
 };
 
@@ -467,22 +485,34 @@ void build_plan(const Csc& A, const double* xyz, const rsc_options& opt,
   plan.opt = opt;
   const int n = A.n;
   plan.n = n;
+  auto tp = std::chrono::steady_clock::now();
+  auto phase = [&](const char* what) {
+    if (!opt.verbose) return;
+    auto now = std::chrono::steady_clock::now();
+    std::printf("symbolic %s: %.3f s\n", what,
+                std::chrono::duration<double>(now - tp).count());
+    tp = now;
+  };
   NDResult nd = nd_in ? *nd_in : nested_dissection(A, xyz, opt.nd_small);
+  phase("nested dissection");
   std::vector<int> pinv(n);
   for (int k = 0; k < n; ++k) pinv[nd.perm[k]] = k;
   std::vector<int> parent = elimination_tree(A, pinv);
   std::vector<int> post(n);
   std::iota(post.begin(), post.end(), 0);
   std::vector<int> cc = column_counts(A, pinv, parent, post);
+  phase("etree and column counts");
   std::vector<Separator> seps = separators_from_nd(nd, n);
   plan.specs = build_relaxed_supernodes(parent, cc, seps, nd.cparent,
                                         opt.max_dense_block);
+  phase("relaxed supernodes");
   plan.perm = nd.perm;
   plan.diag_blocks = reorder_nested_separators(
       plan.specs, xyz, opt.max_diagonal_block, plan.perm,
       opt.reorder_separators != 0 && xyz != nullptr);
   plan.pinv.assign(n, 0);
   for (int k = 0; k < n; ++k) plan.pinv[plan.perm[k]] = k;
+  phase("separator trees");
 
   // supernodes and row sets
   const int ns = (int)plan.specs.size();
@@ -522,6 +552,7 @@ void build_plan(const Csc& A, const double* xyz, const rsc_options& opt,
       plan.nodes[sn.parent].children.push_back(s);
     }
   }
+  phase("row sets");
   // levels (height) and relative maps into the parent's front
   int maxlev = 0;
   for (int s = 0; s < ns; ++s) {
@@ -534,31 +565,41 @@ void build_plan(const Csc& A, const double* xyz, const rsc_options& opt,
   }
   plan.levels.assign(maxlev + 1, {});
   for (int s = 0; s < ns; ++s) plan.levels[plan.nodes[s].level].push_back(s);
-  for (int s = 0; s < ns; ++s) {
-    Supernode& sn = plan.nodes[s];
-    if (sn.parent < 0) continue;
-    const Supernode& P = plan.nodes[sn.parent];
-    sn.relmap.resize(sn.rows.size());
-    const int plast = P.first + P.ncols - 1;
-    for (size_t a = 0; a < sn.rows.size(); ++a) {
-      int r = sn.rows[a];
-      if (r <= plast) {
-        sn.relmap[a] = r - P.first;
-      } else {
-        auto it = std::lower_bound(P.rows.begin(), P.rows.end(), r);
-        if (it == P.rows.end() || *it != r)
-          throw std::runtime_error("row structure is not closed under the parent");
-        sn.relmap[a] = P.ncols + (int)(it - P.rows.begin());
+  {
+    // position of every row in the parent's front, one parent at a time
+    std::vector<int> pos(n, -1);
+    for (int p = 0; p < ns; ++p) {
+      const Supernode& P = plan.nodes[p];
+      if (P.children.empty()) continue;
+      for (size_t q = 0; q < P.rows.size(); ++q) pos[P.rows[q]] = P.ncols + (int)q;
+      const int plast = P.first + P.ncols - 1;
+      for (int ch : P.children) {
+        Supernode& sn = plan.nodes[ch];
+        sn.relmap.resize(sn.rows.size());
+        for (size_t a = 0; a < sn.rows.size(); ++a) {
+          const int r = sn.rows[a];
+          if (r <= plast) {
+            sn.relmap[a] = r - P.first;
+          } else {
+            if (pos[r] < 0)
+              throw std::runtime_error("row structure is not closed under the parent");
+            sn.relmap[a] = pos[r];
+          }
+        }
       }
+      for (int r : P.rows) pos[r] = -1;
     }
   }
+  phase("levels and relative maps");
   // diagonal block trees of compressed supernodes
   for (int s = 0; s < ns; ++s) {
     Supernode& sn = plan.nodes[s];
     if (!sn.compress) continue;
     build_hodlr(sn, plan.diag_blocks[s], opt.explicit_block_threshold);
   }
+  phase("diagonal trees");
   replan_ranks(plan, opt.diagonal_rank_constant);
+  phase("ranks and storage");
 }
 
 }  // namespace rsc

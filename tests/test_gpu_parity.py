@@ -120,18 +120,50 @@ def test_fp32_factor_converges():
     assert r32["iterations"] <= r64["iterations"] + 3
 
 
-def test_block_pcg_equals_single():
+def test_block_pcg_is_batch_invariant():
+    """Several right-hand sides run on the matrix-core task bodies, one on
+    the scalar ones.  A column's result does not depend on the other columns
+    or its position (bitwise), and agrees with the single solve to the PCG
+    tolerance."""
     A, xyz = R.poisson3d(16, 1.0, 1234)
     M = _factor(A, xyz, max_dense_block=63)
     n = A._nrow
     B = R.uniform_rhs(4 * n, 5678).reshape(4, n).T.copy()
     X = np.zeros_like(B)
     res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(X, B, M)
+    perm = [2, 0, 3, 1]
+    Xp = np.zeros_like(B)
+    resp = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(Xp, B[:, perm].copy(), M)
+    two = np.zeros((n, 2))
+    res2 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(two, B[:, [3, 1]].copy(), M)
+    for q, j in enumerate(perm):
+        assert np.array_equal(Xp[:, q], X[:, j])
+        assert resp[q]["iterations"] == res[j]["iterations"]
+    assert np.array_equal(two[:, 0], X[:, 3]) and np.array_equal(two[:, 1], X[:, 1])
+    assert res2[0]["true_relres"] == res[3]["true_relres"]
     for j in range(4):
         x = np.zeros(n)
         r1 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, B[:, j].copy(), M)
-        assert res[j]["iterations"] == r1["iterations"]
-        assert np.array_equal(x, X[:, j])
+        assert abs(res[j]["iterations"] - r1["iterations"]) <= 1
+        assert relerr(X[:, j], x) < 1e-6
+        assert res[j]["true_relres"] < 1e-7
+
+
+def test_multi_rhs_apply_matches_oracle():
+    """The matrix-core apply bodies (two and seventeen right-hand sides: one
+    full group of sixteen plus a partial one) against the FP64 oracle."""
+    build, sopts, okw, tol = CASES["p3d20_hodlr"]
+    A, xyz = build()
+    sym = symbolic(A, xyz, **sopts)
+    O = oracle_for(A, sym, **okw)
+    M = _factor(A, xyz, **dict(sopts))
+    n = A._nrow
+    for k in (2, 17):
+        B = R.uniform_rhs(k * n, 99).reshape(k, n).T.copy()
+        X = np.empty_like(B)
+        M.solve(X, B)
+        for j in (0, k - 1):
+            assert relerr(X[:, j], O.apply(B[:, j])) < tol
 
 
 @pytest.mark.parametrize("prec,tol", [(R.RSC_FP64, 1e-11), (R.RSC_FP32, 1e-4)])

@@ -1,6 +1,9 @@
-"""Request server (continuous batching): every request's solution, iteration
-count and residuals equal a solo block PCG solve of the same right-hand
-sides bit for bit, whatever else shares the device with it."""
+"""Request server (continuous batching): a request's solution, iteration
+count and residuals do not depend on what else shares the device with it
+(bitwise), and equal a solo block PCG solve of the same right-hand sides:
+bitwise for several right-hand sides (both run the matrix-core apply
+bodies), to the PCG tolerance for one (the solo apply then runs the scalar
+bodies)."""
 import ctypes as C
 import time
 
@@ -44,7 +47,7 @@ def test_served_results_equal_solo_solves():
     fac = _factors()
     # (factor, number of right-hand sides): more columns than slots in total,
     # one request larger than the slot pool
-    reqs = [(0, 3), (1, 5), (0, 12), (1, 1), (0, 2), (1, 7)]
+    reqs = [(0, 3), (1, 5), (0, 12), (1, 1), (0, 2), (1, 7), (0, 1)]
     rng_seed = 100
     data = []
     for f, m in reqs:
@@ -79,18 +82,27 @@ def test_served_results_equal_solo_solves():
     mixed, st = serve(list(range(len(reqs))), stagger=True)
     assert st["max_busy"] > 1
     alone = {}
-    for i in (2, 5):
+    for i in (2, 3, 6):
         o, _ = serve([i], stagger=False)
         alone.update(o)
     reversed_mix, _ = serve(list(reversed(range(len(reqs)))), stagger=False)
     for i in range(len(reqs)):
-        xs, rs = solo[i]
-        for got in (mixed[i], reversed_mix[i]) + ((alone[i],) if i in alone else ()):
+        x0, res0 = mixed[i]
+        for got in (reversed_mix[i],) + ((alone[i],) if i in alone else ()):
             x, res = got
-            assert np.array_equal(x, xs), i
-            for r, (it, rec, tru) in zip(res, rs):
-                assert r["iterations"] == it
-                assert r["recurrence_relres"] == rec
-                assert r["true_relres"] == tru
-                assert rec <= EPS
-                assert r["seconds"] > 0
+            assert np.array_equal(x, x0), i
+            for r, q in zip(res, res0):
+                assert (r["iterations"], r["recurrence_relres"], r["true_relres"]) == \
+                    (q["iterations"], q["recurrence_relres"], q["true_relres"])
+        xs, rs = solo[i]
+        for j, (r, (it, rec, tru)) in enumerate(zip(res0, rs)):
+            assert r["recurrence_relres"] <= EPS and r["seconds"] > 0
+            if reqs[i][1] > 1:
+                assert (r["iterations"], r["recurrence_relres"], r["true_relres"]) == \
+                    (it, rec, tru)
+            else:
+                assert abs(r["iterations"] - it) <= 1
+        if reqs[i][1] > 1:
+            assert np.array_equal(x0, xs), i
+        else:
+            assert np.linalg.norm(x0 - xs) <= 1e-6 * np.linalg.norm(xs), i

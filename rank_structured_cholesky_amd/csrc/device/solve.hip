@@ -134,6 +134,38 @@ __device__ inline void vst(double* p, double v) {
 // Forward assembly of one 1024-entry chunk of node S's front vector:
 // fv_c = source[perm[cols]] (perm may be null: source indexed directly),
 // fv_R = 0, then the children's update vectors added through their relmaps.
+// Front-vector assembly of one chunk straight in device memory (one
+// right-hand side at a time): own entries, then child by child.
+template <bool COH>
+__device__ void assemble_chunk_direct(const VecNode& S, int chunk,
+                                      const VecNode* __restrict__ nodes,
+                                      const int* __restrict__ children,
+                                      const int* __restrict__ relmap, const VecBufs& b,
+                                      int k, const double* __restrict__ src, int64_t lds,
+                                      const int* __restrict__ perm) {
+  const int c = S.ncols, R = S.R, tid = threadIdx.x;
+  const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
+  if (p0 >= c + R) return;
+  for (int col = 0; col < k; ++col) {
+    double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
+    const double* sc = src + (int64_t)col * lds;
+    for (int i = p0 + tid; i < p1; i += 256)
+      vst<COH>(f + i, i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0);
+    __syncthreads();
+    for (int q = 0; q < S.nchild; ++q) {
+      const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
+      const int* rm = relmap + C.relmap_off;
+      const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
+      // one predicated pass over the child's rows (no dependent searches)
+      for (int a = tid; a < C.R; a += 256) {
+        const int p = rm[a];
+        if (p >= p0 && p < p1) vst<COH>(f + p, vld<COH>(f + p) + vld<COH>(fc + a));
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <bool COH>
 __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
                                const int* __restrict__ children,
@@ -178,15 +210,27 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
 }
 
 // Backward gather of one chunk: fv_R[a] = w[rows[a]] (the solution at the
-// node's off-diagonal rows) and w[cols] = z[cols] (start of x_c), all
-// right-hand sides in one pass.
-template <bool COH>
+// node's off-diagonal rows) and w[cols] = z[cols] (start of x_c); FLAT runs
+// all right-hand sides in one pass.
+template <bool COH, bool FLAT = false>
 __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
                              const VecBufs& b, int k) {
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
   const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
   if (p0 >= c + R) return;
   const int* rw = rows + S.rows_off;
+  if (!FLAT) {
+    for (int col = 0; col < k; ++col) {
+      double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
+      double* wc = b.w + (int64_t)col * b.n;
+      const double* zc = b.z + (int64_t)col * b.n;
+      for (int i = p0 + tid; i < p1; i += 256) {
+        if (i < c) vst<COH>(wc + S.first + i, vld<COH>(zc + S.first + i));
+        else vst<COH>(f + i, vld<COH>(wc + rw[i - c]));
+      }
+    }
+    return;
+  }
   const int len = p1 - p0;
   for (int idx = tid; idx < k * len; idx += 256) {
     const int col = idx / len, i = p0 + (idx - col * len);
@@ -642,12 +686,12 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
   }
 }
 
-template <typename T, bool COH>
+template <typename T, bool COH, bool MULTI>
 __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBufs& b, int k,
                           double (*red)[64], double* xl) {
   const GvTask t = uniform_task(t0);
   if (t.kind != 2) {
-    if (k >= b.multi_min_k) {
+    if constexpr (MULTI) {
       gemv_mfma<T, COH>(F, t, b, k, xl);
       return;
     }
@@ -954,7 +998,7 @@ __device__ bool df_wait(const DfArgs& a, int g) {
 // completion signal and the next ticket are taken together, so the compiler
 // cannot merge a tid == 0 region of one iteration with one of the next (a
 // merged region lets the other lanes run the loop without thread 0).
-template <typename T>
+template <typename T, bool MULTI>
 __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, DfArgs a,
                                                   VecBufs b, int k) {
   __shared__ double red[4][64];
@@ -993,12 +1037,16 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     }
     if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
-      assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b, k,
-                           a.B, a.ldb, a.perm, lds.mf);
+      if (MULTI)
+        assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b,
+                             k, a.B, a.ldb, a.perm, lds.mf);
+      else
+        assemble_chunk_direct<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children,
+                                    a.relmap, b, k, a.B, a.ldb, a.perm);
     } else if (type == 1) {
-      gather_chunk<true>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
+      gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
-      gemv_lite<T, true>(F, a.gv[ta], b, k, red, lds.mf);
+      gemv_lite<T, true, MULTI>(F, a.gv[ta], b, k, red, lds.mf);
     } else if (type == 4) {
       subtree_forward<T, true>(F, a.sub, ta, b, k, lds.sl);
     } else if (type == 5) {
@@ -1047,13 +1095,18 @@ template <typename T>
 void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, int grid,
                            hipStream_t s) {
   if (a.ntasks <= 0) return;
-  k_apply_df<T><<<grid, 256, 0, s>>>(factor, a, b, k);
+  // the matrix-core variant is a kernel of its own, so its registers do not
+  // weigh on the single-right-hand-side kernel
+  if (k >= b.multi_min_k) k_apply_df<T, true><<<grid, 256, 0, s>>>(factor, a, b, k);
+  else k_apply_df<T, false><<<grid, 256, 0, s>>>(factor, a, b, k);
 }
 
 template <typename T>
 int dataflow_blocks_per_cu() {
+  // both variants have the same launch bounds (256 threads, 4 per CU); a
+  // workgroup beyond the resident ones finds the tickets taken and returns
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_apply_df<T>, 256, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_apply_df<T, false>, 256, 0) != hipSuccess ||
       nb < 1)
     nb = 1;
   return nb;

@@ -84,10 +84,11 @@ typedef struct {
   int nd_small;                 /* leaf size of the nested dissection -> 200 */
   /* MI355X build options */
   int factor_precision;         /* RSC_FP64 or RSC_FP32 */
-  int sketch_precision;         /* sketches run in the factor precision:
-                                   RSC_FP64 (default: same as the factor) or
-                                   RSC_FP32 with an FP32 factor; RSC_BF16 is
-                                   reserved and returns RSC_UNSUPPORTED_MODE */
+  int sketch_precision;         /* products of the randomized range finder:
+                                   RSC_FP64 (default: the factor precision),
+                                   RSC_FP32 or RSC_BF16 (BF16 products with
+                                   FP32 accumulation) with an FP32 factor;
+                                   V = L21 U stays in the factor precision */
   int verbose;                  /* 0 = silent */
 } rsc_options;
 
@@ -300,8 +301,9 @@ rsc_status rsc_uniform_rhs(int64_t count, uint64_t seed, double* out);
 
 /* ---- Kernel self-test (no reference counterpart: exercises the batched
  * MFMA GEMM of the factor on one job, host pointers, column-major):
- * C(m x n) = beta*C + alpha*op(A)*op(B), precision RSC_FP64 or RSC_FP32
- * (inputs rounded to FP32 for the latter). */
+ * C(m x n) = beta*C + alpha*op(A)*op(B), precision RSC_FP64, RSC_FP32
+ * (inputs rounded to FP32) or RSC_BF16 (FP32 operands, BF16 products with
+ * FP32 accumulation). */
 rsc_status rsc_selftest_gemm(int precision, int transa, int transb, int m, int n,
                              int k, double alpha, const double* A, int lda,
                              const double* B, int ldb, double beta, double* C,

@@ -74,12 +74,11 @@ rsc_status check_options(const rsc_options& o) {
   if (o.factor_precision != RSC_FP64 && o.factor_precision != RSC_FP32)
     return fail(RSC_UNSUPPORTED_MODE, "factor_precision must be RSC_FP64 or RSC_FP32");
   // sketch products run in the factor's precision
-  if (o.sketch_precision == RSC_BF16)
-    return fail(RSC_UNSUPPORTED_MODE, "BF16 sketches are not built");
-  if (o.sketch_precision != RSC_FP64 && o.sketch_precision != RSC_FP32)
+  if (o.sketch_precision != RSC_FP64 && o.sketch_precision != RSC_FP32 &&
+      o.sketch_precision != RSC_BF16)
     return fail(RSC_UNSUPPORTED_MODE, "bad sketch_precision");
-  if (o.sketch_precision == RSC_FP32 && o.factor_precision != RSC_FP32)
-    return fail(RSC_UNSUPPORTED_MODE, "FP32 sketches need an FP32 factor");
+  if (o.sketch_precision != RSC_FP64 && o.factor_precision != RSC_FP32)
+    return fail(RSC_UNSUPPORTED_MODE, "FP32 and BF16 sketches need an FP32 factor");
   if (o.max_dense_block < 0 || o.max_diagonal_block < 1 || o.power_iterations < 0 ||
       o.min_rank < 1 || o.max_attempts < 1)
     return fail(RSC_BAD_INPUT, "bad option value");
@@ -617,7 +616,7 @@ rsc_status rsc_uniform_rhs(int64_t count, uint64_t seed, double* out) {
 template <typename T>
 static rsc_status selftest_gemm_t(int ta, int tb, int m, int n, int k, double alpha,
                                   const double* A, int lda, const double* B, int ldb,
-                                  double beta, double* Cm, int ldc) {
+                                  double beta, double* Cm, int ldc, bool bf16 = false) {
   const int64_t asz = (int64_t)lda * (ta ? m : k), bsz = (int64_t)ldb * (tb ? k : n),
                 csz = (int64_t)ldc * n;
   std::vector<T> ha(A, A + asz), hb(B, B + bsz), hc(Cm, Cm + csz);
@@ -637,9 +636,14 @@ static rsc_status selftest_gemm_t(int ta, int tb, int m, int n, int k, double al
   RSC_HIP(hipMemset(dcnt.p, 0, dcnt.bytes));
   dj.upload(jobs, s);
   dt.upload(tiles, s);
-  rsc::dev::launch_gemm<T>(ta != 0, tb != 0, dj.as<rsc::dev::GemmJob>(),
-                           dt.as<rsc::dev::GemmTile>(), (int)tiles.size(), dpart.p,
-                           dcnt.as<unsigned>(), s);
+  if (bf16)
+    rsc::dev::launch_gemm_bf16(ta != 0, tb != 0, dj.as<rsc::dev::GemmJob>(),
+                               dt.as<rsc::dev::GemmTile>(), (int)tiles.size(), dpart.p,
+                               dcnt.as<unsigned>(), s);
+  else
+    rsc::dev::launch_gemm<T>(ta != 0, tb != 0, dj.as<rsc::dev::GemmJob>(),
+                             dt.as<rsc::dev::GemmTile>(), (int)tiles.size(), dpart.p,
+                             dcnt.as<unsigned>(), s);
   RSC_HIP(hipGetLastError());
   RSC_HIP(hipMemcpy(hc.data(), dc.p, csz * sizeof(T), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < csz; ++i) Cm[i] = (double)hc[i];
@@ -655,9 +659,9 @@ rsc_status rsc_selftest_gemm(int precision, int transa, int transb, int m, int n
       lda < (transa ? k : m) || ldb < (transb ? n : k))
     return fail(RSC_BAD_INPUT, "bad gemm arguments");
   return guard([&]() -> rsc_status {
-    if (precision == RSC_FP32)
+    if (precision == RSC_FP32 || precision == RSC_BF16)
       return selftest_gemm_t<float>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta,
-                                    C, ldc);
+                                    C, ldc, precision == RSC_BF16);
     return selftest_gemm_t<double>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta,
                                    C, ldc);
   });

@@ -52,7 +52,8 @@ struct Step {
   int op;
   int count;
   int64_t first;
-  int f64 = 0;  // 1: run the op in FP64 on FP64 buffers (QR of an FP32 factor)
+  int f64 = 0;   // 1: run the op in FP64 on FP64 buffers (QR of an FP32 factor)
+  int bf16 = 0;  // 1: GEMM products in BF16 with FP32 accumulation (sketches)
 };
 
 // A step recorded while the compressed supernodes of one level are captured
@@ -60,7 +61,7 @@ struct Step {
 struct Pending {
   int kind;  // OpKind (the GEMM variants carry their transposes)
   bool lower = false;
-  int f64 = 0;
+  int f64 = 0, bf16 = 0;
   std::string ctx;
   std::vector<dev::GemmJob> g;
   std::vector<dev::PotrfJob> p;
@@ -226,6 +227,10 @@ class Engine final : public FactorEngine {
   std::vector<std::string> step_ctx_;
   std::string ctx_;
   int cur_f64_ = 0;
+  int cur_bf16_ = 0;  // set around the range-finder products of BF16 sketches
+  bool bf16_sketch() const {
+    return sizeof(T) == 4 && plan_.opt.sketch_precision == RSC_BF16;
+  }
   int64_t esize() const { return cur_f64_ ? 8 : (int64_t)sizeof(T); }
   std::vector<dev::ConvJob> vjobs_;
   int64_t qr64_elems_ = 0;  // FP64 QR scratch (in T elements)
@@ -244,6 +249,7 @@ class Engine final : public FactorEngine {
   void push(const Step& st0) {
     Step st = st0;
     st.f64 = cur_f64_;
+    st.bf16 = cur_bf16_;
     prog_.push_back(st);
     step_ctx_.push_back(ctx_);
   }
@@ -287,6 +293,7 @@ class Engine final : public FactorEngine {
     Pending& p = capture_->back();
     p.kind = kind;
     p.f64 = cur_f64_;
+    p.bf16 = cur_bf16_;
     p.ctx = ctx_;
     return p;
   }
@@ -616,7 +623,11 @@ void Engine<T>::sketch(const std::vector<SketchJob>& jobs) {
     qu.push_back({J.U, J.cols, J.r, J.W, J.node});
     qz.push_back({J.Z, J.rows, J.r, J.W, J.node});
   }
+  // range-finder products (BF16 with FP32 accumulation for BF16 sketches);
+  // V = X U at the end stays in the factor precision
+  cur_bf16_ = bf16_sketch() ? 1 : 0;
   emit_gemm(OP_GEMM_TT, g0, false);
+  cur_bf16_ = 0;
   qr(qu);
   for (int it = 0; it < plan_.opt.power_iterations; ++it) {
     g1.clear();
@@ -649,9 +660,13 @@ void Engine<T>::sketch(const std::vector<SketchJob>& jobs) {
       b.beta = 0.0;
       g2.push_back(b);
     }
+    cur_bf16_ = bf16_sketch() ? 1 : 0;
     emit_gemm(OP_GEMM_NN, g1, false);
+    cur_bf16_ = 0;
     qr(qz);
+    cur_bf16_ = bf16_sketch() ? 1 : 0;
     emit_gemm(OP_GEMM_TN, g2, false);
+    cur_bf16_ = 0;
     qr(qu);
   }
   for (auto& J : jobs) {
@@ -907,7 +922,7 @@ void Engine<T>::emit_level_compressed(const std::vector<int>& nodes) {
   if (cur_f64_ != 0) throw std::logic_error("unbalanced FP64 section");
   std::vector<size_t> head(nodes.size(), 0);
   auto same = [](const Pending& a, const Pending& b) {
-    return a.kind == b.kind && a.lower == b.lower && a.f64 == b.f64;
+    return a.kind == b.kind && a.lower == b.lower && a.f64 == b.f64 && a.bf16 == b.bf16;
   };
   for (;;) {
     size_t lead = nodes.size(), most = 0;
@@ -937,6 +952,7 @@ void Engine<T>::emit_level_compressed(const std::vector<int>& nodes) {
       ++merged;
     }
     cur_f64_ = m.f64;
+    cur_bf16_ = m.bf16;
     ctx_ = m.ctx + (merged > 1 ? " (+" + std::to_string(merged - 1) + " supernodes)" : "");
     switch (m.kind) {
       case OP_GEMM_NN:
@@ -953,6 +969,7 @@ void Engine<T>::emit_level_compressed(const std::vector<int>& nodes) {
     }
   }
   cur_f64_ = 0;
+  cur_bf16_ = 0;
 }
 
 template <typename T>
@@ -1544,6 +1561,10 @@ void Engine<T>::numeric(const double* Ax) {
           dev::launch_gemm<double>(ta, tb, dgjobs_.as<dev::GemmJob>(),
                                    dgtiles_.as<dev::GemmTile>() + s.first, s.count,
                                    dgpart_.p, dgcnt_.as<unsigned>(), st_);
+        else if (s.bf16 && sizeof(T) == 4)
+          dev::launch_gemm_bf16(ta, tb, dgjobs_.as<dev::GemmJob>(),
+                                dgtiles_.as<dev::GemmTile>() + s.first, s.count, dgpart_.p,
+                                dgcnt_.as<unsigned>(), st_);
         else
           dev::launch_gemm<T>(ta, tb, dgjobs_.as<dev::GemmJob>(),
                               dgtiles_.as<dev::GemmTile>() + s.first, s.count,

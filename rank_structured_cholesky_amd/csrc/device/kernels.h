@@ -110,6 +110,9 @@ struct AsmTask {
 template <typename T>
 void launch_gemm(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
                  int ntiles, void* partials, unsigned* counters, hipStream_t s);
+// FP32 operands and result, BF16 products with FP32 accumulation
+void launch_gemm_bf16(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
+                      int ntiles, void* partials, unsigned* counters, hipStream_t s);
 template <typename T>
 void launch_potrf(const PotrfJob* jobs, int njobs, int* status, hipStream_t s);
 template <typename T>

@@ -41,7 +41,22 @@ struct Acc<float> {
 // slabs of 16.  MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16]
 // and B[l/16][l%16]; accumulator element v of lane l is D[Acc<T>::row(l,v)][l%16].
 // ---------------------------------------------------------------------------
-template <typename T, bool TA, bool TB>
+typedef short bf4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs).  BF16 keeps the
+// FP32 exponent range, so the products need no block scaling: scaling a
+// block by a power of two would change no rounding.
+__device__ inline short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// BF (FP32 storage only): the products run as BF16 MFMA 16x16x16 with FP32
+// accumulation (one instruction per 16-deep slab); lane l supplies
+// A[l%16][4(l/16)..4(l/16)+3] and B[4(l/16)..][l%16], and the accumulator
+// layout is that of the FP32 16x16x4 instruction.
+template <typename T, bool TA, bool TB, bool BF = false>
 __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
                                               const GemmTile* __restrict__ tiles,
                                               T* __restrict__ partials,
@@ -86,16 +101,32 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs,
       Bs[kk][j] = v;
     }
     __syncthreads();
+    if constexpr (BF) {
+      const int k4 = 4 * (lane >> 4), li = lane & 15;
+      bf4 a0, a1, b0, b1;
 #pragma unroll
-    for (int kk = 0; kk < KS; kk += 4) {
-      T a0 = As[kk + (lane >> 4)][wm + (lane & 15)];
-      T a1 = As[kk + (lane >> 4)][wm + 16 + (lane & 15)];
-      T b0 = Bs[kk + (lane >> 4)][wn + (lane & 15)];
-      T b1 = Bs[kk + (lane >> 4)][wn + 16 + (lane & 15)];
-      acc[0][0] = Acc<T>::mfma(a0, b0, acc[0][0]);
-      acc[0][1] = Acc<T>::mfma(a0, b1, acc[0][1]);
-      acc[1][0] = Acc<T>::mfma(a1, b0, acc[1][0]);
-      acc[1][1] = Acc<T>::mfma(a1, b1, acc[1][1]);
+      for (int e = 0; e < 4; ++e) {
+        a0[e] = to_bf16((float)As[k4 + e][wm + li]);
+        a1[e] = to_bf16((float)As[k4 + e][wm + 16 + li]);
+        b0[e] = to_bf16((float)Bs[k4 + e][wn + li]);
+        b1[e] = to_bf16((float)Bs[k4 + e][wn + 16 + li]);
+      }
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc[1][1], 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < KS; kk += 4) {
+        T a0 = As[kk + (lane >> 4)][wm + (lane & 15)];
+        T a1 = As[kk + (lane >> 4)][wm + 16 + (lane & 15)];
+        T b0 = Bs[kk + (lane >> 4)][wn + (lane & 15)];
+        T b1 = Bs[kk + (lane >> 4)][wn + 16 + (lane & 15)];
+        acc[0][0] = Acc<T>::mfma(a0, b0, acc[0][0]);
+        acc[0][1] = Acc<T>::mfma(a0, b1, acc[0][1]);
+        acc[1][0] = Acc<T>::mfma(a1, b0, acc[1][0]);
+        acc[1][1] = Acc<T>::mfma(a1, b1, acc[1][1]);
+      }
     }
     __syncthreads();
   }
@@ -194,6 +225,19 @@ void launch_gemm(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
   else if (ta && !tb) k_gemm<T, true, false><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
   else if (!ta && tb) k_gemm<T, false, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
   else k_gemm<T, true, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+}
+
+void launch_gemm_bf16(bool ta, bool tb, const GemmJob* jobs, const GemmTile* tiles,
+                      int ntiles, void* partials, unsigned* counters, hipStream_t s) {
+  if (ntiles <= 0) return;
+  float* P = (float*)partials;
+  if (!ta && !tb)
+    k_gemm<float, false, false, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+  else if (ta && !tb)
+    k_gemm<float, true, false, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+  else if (!ta && tb)
+    k_gemm<float, false, true, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
+  else k_gemm<float, true, true, true><<<ntiles, 256, 0, s>>>(jobs, tiles, P, counters);
 }
 
 // ---------------------------------------------------------------------------

@@ -64,6 +64,10 @@ def test_bad_input_and_unsupported_modes():
         st = L.rsc_symbolic_create(A._nrow, _lib.iptr(A._p), _lib.iptr(A._i), None,
                                    C.byref(o2), C.byref(h))
         assert st == _lib.RSC_UNSUPPORTED_MODE
+    o3 = _lib.default_options(factor_precision=R.RSC_FP32, sketch_precision=R.RSC_BF16)
+    assert L.rsc_symbolic_create(A._nrow, _lib.iptr(A._p), _lib.iptr(A._i), None,
+                                 C.byref(o3), C.byref(h)) == _lib.RSC_OK
+    L.rsc_symbolic_destroy(h)
     st = L.rsc_factor_create(A._nrow, _lib.iptr(A._p), _lib.iptr(A._i), None, None,
                              C.byref(o), C.byref(h))
     assert st == _lib.RSC_BAD_INPUT

@@ -12,7 +12,8 @@ from helpers import oracle_for, relerr, symbolic
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec,tol", [(R.RSC_FP64, 1e-13), (R.RSC_FP32, 1e-5)])
+@pytest.mark.parametrize("prec,tol", [(R.RSC_FP64, 1e-13), (R.RSC_FP32, 1e-5),
+                                      (R.RSC_BF16, 1e-2)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 # the last three split k over several workgroups (few output tiles, long k)
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (70, 33, 19), (128, 64, 200), (5, 130, 64),
@@ -186,6 +187,31 @@ def test_level_batched_factor_matches_node_by_node(monkeypatch, prec, tol):
         M.solve(x, b)
         out[mode] = x
     assert relerr(out["1"], out["0"]) < tol
+
+
+@pytest.mark.parametrize("name", ["p3d16_c63", "p3d20_hodlr", "el6_c30"])
+def test_bf16_sketch_factor_converges(name):
+    """FP32 factor with BF16 range-finder products (FP32 accumulation): the
+    apply stays within 1e-2 of the FP64 oracle and PCG reaches 1e-8 within
+    a few iterations of the FP64 factor's count."""
+    build, sopts, okw, tol = CASES[name]
+    A, xyz = build()
+    sym = symbolic(A, xyz, **sopts)
+    O = oracle_for(A, sym, **okw)
+    b = R.uniform_rhs(A._nrow, 5678)
+    M64 = _factor(A, xyz, **dict(sopts))
+    x = np.zeros_like(b)
+    r64 = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M64)
+    M = _factor(A, xyz, factor_precision=R.RSC_FP32, sketch_precision=R.RSC_BF16,
+                **dict(sopts))
+    y = np.empty_like(b)
+    M.solve(y, b)
+    assert np.all(np.isfinite(y))
+    assert relerr(y, O.apply(b)) < 1e-2
+    x = np.zeros_like(b)
+    res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M)
+    assert res["true_relres"] < 1e-7
+    assert res["iterations"] <= r64["iterations"] + max(3, r64["iterations"] // 5)
 
 
 @pytest.mark.parametrize("name", list(CASES))

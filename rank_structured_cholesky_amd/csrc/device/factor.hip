@@ -113,6 +113,7 @@ class Engine final : public FactorEngine {
   ~Engine() override {
     (void)hipStreamSynchronize(st_);
     if (rng_thread_.joinable()) rng_thread_.join();
+    if (df_thread_.joinable()) df_thread_.join();
     if (rng_host_) (void)hipHostFree(rng_host_);
   }
 
@@ -195,6 +196,13 @@ class Engine final : public FactorEngine {
     a.ldb = ldb;
     return a;
   }
+  SolveLayout sl_;
+  std::thread df_thread_;
+  std::unique_ptr<DataflowTasks> df_pending_;
+  std::exception_ptr df_err_;
+  bool solve_ready_ = false;
+  int df_resident_ = 1;
+  void finish_solve_setup();
   std::vector<dev::DfTask> df_host_;  // kept for the optional timing trace
   std::string df_trace_path_;
   int df_ntasks_ = 0, df_ngroups_ = 0, df_grid_ = 1;
@@ -805,7 +813,13 @@ void Engine<T>::layout_scratch() {
     for (int j : lv) sum += inv_need_[j];
     inv_elems_ = std::max(inv_elems_, sum);
   }
-  for (size_t j = 0; j < plan_.nodes.size(); ++j) {
+  // per-supernode sketch, helper and tree scratch is used only while its
+  // level is emitted: offsets restart at every level (the supernodes of one
+  // level share launches, so theirs are disjoint)
+  int64_t scratch_max = 0;
+  for (auto& lvl : plan_.levels)
+  for (int j : lvl) {
+    if (j == lvl.front()) off = 0;
     const Supernode& s = plan_.nodes[j];
     if (!s.compress) continue;
     int rmax = s.rank;
@@ -840,8 +854,9 @@ void Engine<T>::layout_scratch() {
       hnode_scratch_[j][q] = off;
       off += 2 * rows * h.rank + 2 * (int64_t)h.rank * h.rank;
     }
+    scratch_max = std::max(scratch_max, off);
   }
-  scratch_elems_ = off;
+  scratch_elems_ = scratch_max;
   // Zp staging: per level, the uncompressed panels' R x c blocks
   zp_elems_ = 0;
   for (auto& lv : plan_.levels) {
@@ -1271,30 +1286,73 @@ void Engine<T>::add_checks() {
   prog_.swap(out);
 }
 
+// Joins the dataflow task builder and uploads its lists (once).
+template <typename T>
+void Engine<T>::finish_solve_setup() {
+  if (solve_ready_) return;
+  if (df_thread_.joinable()) df_thread_.join();
+  if (df_err_) {
+    std::exception_ptr e = df_err_;
+    df_err_ = nullptr;
+    std::rethrow_exception(e);
+  }
+  DataflowTasks& DT = *df_pending_;
+  df_ntasks_ = (int)DT.tasks.size();
+  df_ngroups_ = (int)DT.need.size();
+  ddf_.upload(DT.tasks, st_);
+  const char* tp = std::getenv("RSC_DF_TRACE");
+  if (tp && tp[0]) {
+    df_trace_path_ = tp;
+    df_host_ = DT.tasks;
+    dtrace_.alloc((size_t)4 * DT.tasks.size() * sizeof(unsigned long long));
+  }
+  ddfgv_.upload(DT.gv, st_);
+  dgwoff_.upload(DT.gwoff, st_);
+  dgw_.upload(DT.gw, st_);
+  dneed_.upload(DT.need, st_);
+  dcnt_.alloc((size_t)(df_ngroups_ + 1) * sizeof(unsigned));
+  part_stride_ = std::max<int64_t>(DT.slots * 32, 1);
+  df_grid_ = std::max(1, std::min(df_ntasks_, df_resident_));
+  df_pending_.reset();
+  RSC_HIP(hipStreamSynchronize(st_));
+  solve_ready_ = true;
+}
+
 template <typename T>
 void Engine<T>::setup_static() {
   const Plan& P = plan_;
   const int ns = (int)P.nodes.size();
+  auto tp = std::chrono::steady_clock::now();
+  auto phase = [&](const char* what) {
+    if (!P.opt.verbose) return;
+    auto now = std::chrono::steady_clock::now();
+    std::printf("setup %s: %.3f s\n", what, std::chrono::duration<double>(now - tp).count());
+    tp = now;
+  };
   // permuted lower pattern and the destination of every entry in its panel
   LowerCsc L = permuted_lower(A_, P.pinv);
   std::vector<int64_t> a_dst(L.i.size());
-  for (int j = 0; j < ns; ++j) {
-    const Supernode& s = P.nodes[j];
-    const int c = s.ncols, ld = c + s.R(), last = s.first + c - 1;
-    for (int k = s.first; k <= last; ++k)
-      for (int e = L.p[k]; e < L.p[k + 1]; ++e) {
-        int r = L.i[e];
-        int pos;
-        if (r <= last) pos = r - s.first;
-        else pos = c + (int)(std::lower_bound(s.rows.begin(), s.rows.end(), r) - s.rows.begin());
-        a_dst[e] = (int64_t)pos + (int64_t)(k - s.first) * ld;
-      }
+  {
+    std::vector<int> rpos(P.n, -1);  // front position of a row, one supernode at a time
+    for (int j = 0; j < ns; ++j) {
+      const Supernode& s = P.nodes[j];
+      const int c = s.ncols, ld = c + s.R(), last = s.first + c - 1;
+      for (size_t q = 0; q < s.rows.size(); ++q) rpos[s.rows[q]] = c + (int)q;
+      for (int k = s.first; k <= last; ++k)
+        for (int e = L.p[k]; e < L.p[k + 1]; ++e) {
+          const int r = L.i[e];
+          const int pos = r <= last ? r - s.first : rpos[r];
+          a_dst[e] = (int64_t)pos + (int64_t)(k - s.first) * ld;
+        }
+      for (int r : s.rows) rpos[r] = -1;
+    }
   }
   dlowp_.upload(L.p, st_);
   da_dst_.upload(a_dst, st_);
   da_src_.upload(L.src, st_);
+  phase("permuted lower matrix");
   // solve: vector layout, level-scheduled and dataflow task lists
-  SolveLayout SL;
+  SolveLayout& SL = sl_;
   {
     // subtree tasks: up to 16384 panel elements each (measured best for one
     // and for sixteen right-hand sides at 64^3 and 128^3)
@@ -1302,6 +1360,7 @@ void Engine<T>::setup_static() {
     if (const char* e = std::getenv("RSC_SUBTREE_ELEMS")) lim = std::atoll(e);
     build_solve_layout(P, SL, lim);
   }
+  phase("solve layout");
   nsub_ = (int)SL.sub.root.size();
   dstptr_.upload(SL.sub.ptr, st_);
   dstnodes_.upload(SL.sub.nodes.empty() ? std::vector<int>{0} : SL.sub.nodes, st_);
@@ -1316,6 +1375,23 @@ void Engine<T>::setup_static() {
   tv_stride_ = SL.tv_stride;
   dvnodes_.upload(SL.vn, st_);
   {
+    // one persistent dataflow launch per apply; RSC_APPLY_LEVELS=1 selects the
+    // level-scheduled launches instead
+    const char* lv = std::getenv("RSC_APPLY_LEVELS");
+    use_dataflow_ = !(lv && lv[0] == '1');
+  }
+  if (use_dataflow_) {
+    // the dataflow task list is built on a host thread while the device
+    // factors; finish_solve_setup() joins it before the first apply
+    df_thread_ = std::thread([this]() {
+      try {
+        df_pending_.reset(new DataflowTasks());
+        build_dataflow_tasks(plan_, sl_, *df_pending_);
+      } catch (...) {
+        df_err_ = std::current_exception();
+      }
+    });
+  } else {
     LevelTasks LT;
     build_level_tasks(P, SL, LT);
     fwd_.swap(LT.fwd);
@@ -1323,29 +1399,11 @@ void Engine<T>::setup_static() {
     level_nodes_.swap(LT.level_nodes);
     dgv_.upload(LT.gv, st_);
     ntasks_ = (int64_t)LT.gv.size();
-    DataflowTasks DT;
-    build_dataflow_tasks(P, SL, DT);
-    df_ntasks_ = (int)DT.tasks.size();
-    df_ngroups_ = (int)DT.need.size();
-    ddf_.upload(DT.tasks, st_);
-    const char* tp = std::getenv("RSC_DF_TRACE");
-    if (tp && tp[0]) {
-      df_trace_path_ = tp;
-      df_host_ = DT.tasks;
-      dtrace_.alloc((size_t)4 * DT.tasks.size() * sizeof(unsigned long long));
-    }
-    ddfgv_.upload(DT.gv, st_);
-    dgwoff_.upload(DT.gwoff, st_);
-    dgw_.upload(DT.gw, st_);
-    dneed_.upload(DT.need, st_);
-    dcnt_.alloc((size_t)(df_ngroups_ + 1) * sizeof(unsigned));
-    part_stride_ = std::max<int64_t>(std::max(LT.slots, DT.slots) * 32, 1);
+    part_stride_ = std::max<int64_t>(LT.slots * 32, 1);
+    solve_ready_ = true;
   }
+  phase("task lists");
   {
-    // one persistent dataflow launch per apply; RSC_APPLY_LEVELS=1 selects the
-    // level-scheduled launches instead
-    const char* lv = std::getenv("RSC_APPLY_LEVELS");
-    use_dataflow_ = !(lv && lv[0] == '1');
     int dev = 0, cus = 1, khz = 0;
     RSC_HIP(hipGetDevice(&dev));
     RSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1353,7 +1411,7 @@ void Engine<T>::setup_static() {
         khz <= 0)
       khz = 100000;
     df_timeout_ = (unsigned long long)khz * 1000ull * 4ull;  // 4 s
-    df_grid_ = std::max(1, std::min(df_ntasks_, cus * dev::dataflow_blocks_per_cu<T>()));
+    df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
     std::vector<int> z = {0, 0};
     dapst_.upload(z, st_);
   }
@@ -1376,9 +1434,12 @@ void Engine<T>::setup_static() {
   const char* dbg = std::getenv("RSC_DEBUG_CHECK");
   debug_check_ = dbg && dbg[0] == '1';
   dstatus_.upload(status, st_);
+  phase("uploads");
   factor_.alloc(std::max<int64_t>(P.factor_elems, 1) * sizeof(T));
+  phase("factor allocation");
   layout_scratch();
   RSC_HIP(hipStreamSynchronize(st_));
+  phase("scratch layout");
 }
 
 template <typename T>
@@ -1409,7 +1470,11 @@ void Engine<T>::build_program() {
                 (long long)P.work_elems, (long long)P.update_elems, (long long)scratch_elems_,
                 (long long)P.rng_count, (long long)inv_elems_, (long long)qr64_elems_,
                 (long long)zp_elems_);
+  auto tw = std::chrono::steady_clock::now();
   work_.alloc(std::max<int64_t>(welems, 1) * sizeof(T));
+  if (P.opt.verbose)
+    std::printf("workspace allocation: %.3f s\n",
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count());
   // the sketch entries are copied in by numeric() before step rng_step_
   rng_step_ = -1;
   rng_on_device_ = false;
@@ -1622,6 +1687,7 @@ void Engine<T>::numeric(const double* Ax) {
     RSC_HIP(hipHostFree(rng_host_));
     rng_host_ = nullptr;
   }
+  finish_solve_setup();
   pivot_replacements_ = status[2];
   if (status[3] != INT_MAX)
     throw DeviceError("non-finite factor values first after program step " +
@@ -1636,6 +1702,7 @@ void Engine<T>::numeric(const double* Ax) {
 template <typename T>
 void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
                       hipStream_t s, const int* live, int multi_min_k) {
+  finish_solve_setup();
   const int n = plan_.n;
   if (k > fv_k_) {
     dfv_.alloc((size_t)fv_stride_ * k * sizeof(double));

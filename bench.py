@@ -123,6 +123,8 @@ def serving_leg(cfg, eps):
         while tickets[q] is None:
             time.sleep(0.0005)
         res = srv.wait(tickets[q], reqs[q][1])
+        print("served request %d/%d (%d right-hand sides)" % (q + 1, nreq, reqs[q][1]),
+              file=sys.stderr, flush=True)
         lat_a.append(res[0]["seconds"])
         iters_a += [r["iterations"] for r in res]
         worst = max(worst, max(r["true_relres"] for r in res))
@@ -148,7 +150,9 @@ def serving_leg(cfg, eps):
             L.rsc_block_pcg_solve(facs[f].handle, k, C.c_void_p(B[c0:].data_ptr()), n,
                                   C.c_void_p(X[c0:].data_ptr()), n, eps, 10000, res,
                                   C.c_void_p(stream.cuda_stream))
+        torch.cuda.synchronize()
         lat_b.append(time.perf_counter() - (t0 + arrive[q]))
+        print("one at a time: request %d/%d" % (q + 1, nreq), file=sys.stderr, flush=True)
     wall_b = time.perf_counter() - t0
     total = int(nrhs.sum())
     return dict(

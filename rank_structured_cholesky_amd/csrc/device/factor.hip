@@ -172,7 +172,7 @@ class Engine final : public FactorEngine {
   DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
-  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_, dz_;
+  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_, dz_, dranges_;
   std::vector<std::vector<SolvePhase>> fwd_, bwd_;
   std::vector<std::vector<int>> level_nodes_;  // per level, nodes outside subtrees
   int64_t tv_stride_ = 0, ntasks_ = 0, part_stride_ = 1;
@@ -1367,6 +1367,7 @@ void Engine<T>::setup_static() {
   dpanel_.upload(SL.panel, st_);
   dchildren_.upload(SL.children, st_);
   drelmap_.upload(SL.relmap, st_);
+  dranges_.upload(SL.asm_ranges, st_);
   drows_.upload(SL.rows, st_);
   std::vector<int> permv(P.perm);
   dperm_.upload(permv, st_);
@@ -1730,6 +1731,7 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     a.nodes = dvnodes_.as<dev::VecNode>();
     a.children = dchildren_.as<int>();
     a.relmap = drelmap_.as<int>();
+    a.ranges = dranges_.as<int>();
     a.rows = drows_.as<int>();
     a.perm = dperm_.as<int>();
     a.B = B;
@@ -1750,7 +1752,8 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
         dev::launch_fwd_assemble(dvnodes_.as<dev::VecNode>(),
                                  dlevnodes_.as<int>() + lev_off_[lv],
                                  lev_off_[lv + 1] - lev_off_[lv], dchildren_.as<int>(),
-                                 drelmap_.as<int>(), vb, k, lev_front_[lv], s);
+                                 drelmap_.as<int>(), dranges_.as<int>(), vb, k,
+                                 lev_front_[lv], s);
       else if (p.kind == 1)
         dev::launch_bwd_gather(dvnodes_.as<dev::VecNode>(),
                                dlevnodes_.as<int>() + lev_off_[lv],

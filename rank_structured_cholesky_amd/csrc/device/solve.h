@@ -33,8 +33,8 @@ struct VecBufs {
 
 // forward: fv_c = w[cols], fv_R = 0, plus children's fv_R via relmap
 void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
-                         const int* children, const int* relmap, VecBufs b, int k,
-                         int max_front, hipStream_t s);
+                         const int* children, const int* relmap, const int* ranges,
+                         VecBufs b, int k, int max_front, hipStream_t s);
 // backward: fv_R[a] = w[rows[a]], w[cols] = z[cols]
 void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
                        const int* rows, VecBufs b, int k, int max_front, hipStream_t s);
@@ -73,6 +73,7 @@ struct DfArgs {
   const VecNode* nodes;
   const int* children;
   const int* relmap;
+  const int* ranges;  // child row ranges per assembly chunk (VecNode::range_off)
   const int* rows;
   const int* perm;
   const double* B;

@@ -109,7 +109,7 @@ __device__ inline VecNode uniform_node(const VecNode& n) {
   u.R = uni(n.R);
   u.nchild = uni(n.nchild);
   u.child_off = uni(n.child_off);
-  u.pad = 0;
+  u.range_off = uni(n.range_off);
   u.fv_off = uni64(n.fv_off);
   u.rows_off = uni64(n.rows_off);
   u.relmap_off = uni64(n.relmap_off);
@@ -140,7 +140,8 @@ template <bool COH>
 __device__ void assemble_chunk_direct(const VecNode& S, int chunk,
                                       const VecNode* __restrict__ nodes,
                                       const int* __restrict__ children,
-                                      const int* __restrict__ relmap, const VecBufs& b,
+                                      const int* __restrict__ relmap,
+                                      const int* __restrict__ ranges, const VecBufs& b,
                                       int k, const double* __restrict__ src, int64_t lds,
                                       const int* __restrict__ perm) {
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
@@ -156,10 +157,12 @@ __device__ void assemble_chunk_direct(const VecNode& S, int chunk,
       const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
       const int* rm = relmap + C.relmap_off;
       const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
-      // one predicated pass over the child's rows (no dependent searches)
-      for (int a = tid; a < C.R; a += 256) {
+      // the child rows that land in this chunk (precomputed range)
+      const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * S.nchild + q);
+      const int a0 = uni(rg[0]), a1 = uni(rg[1]);
+      for (int a = a0 + tid; a < a1; a += 256) {
         const int p = rm[a];
-        if (p >= p0 && p < p1) vst<COH>(f + p, vld<COH>(f + p) + vld<COH>(fc + a));
+        vst<COH>(f + p, vld<COH>(f + p) + vld<COH>(fc + a));
       }
       __syncthreads();
     }
@@ -169,7 +172,8 @@ __device__ void assemble_chunk_direct(const VecNode& S, int chunk,
 template <bool COH>
 __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
                                const int* __restrict__ children,
-                               const int* __restrict__ relmap, const VecBufs& b, int k,
+                               const int* __restrict__ relmap,
+                               const int* __restrict__ ranges, const VecBufs& b, int k,
                                const double* __restrict__ src, int64_t lds,
                                const int* __restrict__ perm, double* __restrict__ sm) {
   // The chunk of kAsmRhs right-hand sides at a time is summed in LDS (sm:
@@ -191,13 +195,13 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
     for (int q = 0; q < S.nchild; ++q) {
       const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
       const int* rm = relmap + C.relmap_off;
-      const int cr = C.R;
+      const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * S.nchild + q);
+      const int a0 = uni(rg[0]), cr = uni(rg[1]) - a0;
       for (int idx = tid; idx < g * cr; idx += 256) {
-        const int cc = idx / cr, a = idx - cc * cr;
+        const int cc = idx / cr, a = a0 + (idx - cc * cr);
         const int p = rm[a];
-        if (p >= p0 && p < p1)
-          sm[cc * VCH + (p - p0)] +=
-              vld<COH>(b.fv + C.fv_off + (int64_t)(c0 + cc) * b.fvs + C.ncols + a);
+        sm[cc * VCH + (p - p0)] +=
+            vld<COH>(b.fv + C.fv_off + (int64_t)(c0 + cc) * b.fvs + C.ncols + a);
       }
       __syncthreads();
     }
@@ -244,11 +248,13 @@ __global__ __launch_bounds__(256) void k_fwd_assemble(const VecNode* __restrict_
                                                       const int* __restrict__ lnodes,
                                                       const int* __restrict__ children,
                                                       const int* __restrict__ relmap,
+                                                      const int* __restrict__ ranges,
                                                       VecBufs b, int k) {
   __shared__ double sm[kAsmRhs * VCH];
   if (b.live && *b.live == 0) return;
   const VecNode S = nodes[lnodes[blockIdx.x]];
-  assemble_chunk<false>(S, blockIdx.y, nodes, children, relmap, b, k, b.w, b.n, nullptr, sm);
+  assemble_chunk<false>(S, blockIdx.y, nodes, children, relmap, ranges, b, k, b.w, b.n,
+                        nullptr, sm);
 }
 
 __global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ nodes,
@@ -261,12 +267,12 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const VecNode* __restrict__ 
 }
 
 void launch_fwd_assemble(const VecNode* nodes, const int* level_nodes, int count,
-                         const int* children, const int* relmap, VecBufs b, int k,
-                         int max_front, hipStream_t s) {
+                         const int* children, const int* relmap, const int* ranges,
+                         VecBufs b, int k, int max_front, hipStream_t s) {
   if (count <= 0) return;
   const int ny = (max_front + VCH - 1) / VCH;
-  k_fwd_assemble<<<dim3(count, ny), 256, 0, s>>>(nodes, level_nodes, children, relmap, b,
-                                                  k);
+  k_fwd_assemble<<<dim3(count, ny), 256, 0, s>>>(nodes, level_nodes, children, relmap,
+                                                  ranges, b, k);
 }
 
 void launch_bwd_gather(const VecNode* nodes, const int* level_nodes, int count,
@@ -1038,11 +1044,11 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
       if (MULTI)
-        assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap, b,
-                             k, a.B, a.ldb, a.perm, lds.mf);
+        assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
+                             a.ranges, b, k, a.B, a.ldb, a.perm, lds.mf);
       else
         assemble_chunk_direct<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children,
-                                    a.relmap, b, k, a.B, a.ldb, a.perm);
+                                    a.relmap, a.ranges, b, k, a.B, a.ldb, a.perm);
     } else if (type == 1) {
       gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {

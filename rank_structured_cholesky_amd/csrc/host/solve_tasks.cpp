@@ -81,7 +81,7 @@ void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems) {
     d.R = s.R();
     d.nchild = (int)s.children.size();
     d.child_off = child_off[j];
-    d.pad = 0;
+    d.range_off = 0;
     d.fv_off = fvo;
     L.fv_off[j] = fvo;
     d.rows_off = rows_off[j];
@@ -98,6 +98,22 @@ void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems) {
         }
     }
   }
+  L.asm_ranges.clear();
+  for (int j = 0; j < ns; ++j) {
+    const Supernode& s = P.nodes[j];
+    if (s.children.empty()) continue;
+    L.vn[j].range_off = (int)(L.asm_ranges.size() / 2);
+    const int len = s.ncols + s.R();
+    for (int p0 = 0; p0 < len; p0 += kSolveChunk) {
+      const int p1 = std::min(p0 + kSolveChunk, len);
+      for (int ch : s.children) {
+        const std::vector<int>& rm = P.nodes[ch].relmap;
+        L.asm_ranges.push_back((int)(std::lower_bound(rm.begin(), rm.end(), p0) - rm.begin()));
+        L.asm_ranges.push_back((int)(std::lower_bound(rm.begin(), rm.end(), p1) - rm.begin()));
+      }
+    }
+  }
+  if (L.asm_ranges.empty()) L.asm_ranges.assign(2, 0);
   L.fv_stride = std::max<int64_t>(fvo, 1);
   L.tv_stride = std::max<int64_t>(tvo, 1);
   L.panel.assign(ns, -1);

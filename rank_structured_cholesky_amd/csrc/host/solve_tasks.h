@@ -34,7 +34,9 @@ struct GvTask {
 // Per-supernode vector gathers of a solve level.
 struct VecNode {
   int first, ncols, R, nchild;
-  int child_off, pad;
+  int child_off;
+  int range_off;     // first (begin, end) pair of this node's assembly chunks
+                     // in asm_ranges: chunk q, child i -> pair range_off + q*nchild + i
   int64_t fv_off;    // front vector offset (per RHS column)
   int64_t rows_off;  // row list offset
   int64_t relmap_off;
@@ -88,6 +90,10 @@ struct SolvePhase {
 // entries per right-hand side) and, for compressed nodes, rank-sized vectors
 // for the off-diagonal pair and every HODLR node.
 struct SolveLayout {
+  // per front-vector chunk of kSolveChunk entries and per child: the child
+  // rows [begin, end) whose parent positions fall in the chunk (relative
+  // maps are increasing, so they are contiguous)
+  std::vector<int> asm_ranges;
   std::vector<dev::VecNode> vn;
   std::vector<int> children, relmap, rows;
   std::vector<int64_t> fv_off, tv_node;

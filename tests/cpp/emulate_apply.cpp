@@ -103,10 +103,17 @@ void assemble(State& S, int j, int chunk, bool from_w) {
     for (int q = 0; q < N.nchild; ++q) {
       const int cj = chk(S.L->children, N.child_off + q, "children");
       const dev::VecNode& C = chk(S.L->vn, cj, "vn");
+      // the device visits only the precomputed range of child rows: it must
+      // be exactly the rows whose parent position falls in the chunk
+      const int64_t rq = 2 * ((int64_t)N.range_off + (int64_t)chunk * N.nchild + q);
+      const int a0 = chk(S.L->asm_ranges, rq, "asm_ranges");
+      const int a1 = chk(S.L->asm_ranges, rq + 1, "asm_ranges");
       for (int a = 0; a < C.R; ++a) {
         const int p = chk(S.L->relmap, C.relmap_off + a, "relmap");
         if (p < 0 || p >= c + R) fail("relmap entry outside the parent front");
-        if (p >= p0 && p < p1)
+        const bool in = p >= p0 && p < p1;
+        if (in != (a >= a0 && a < a1)) fail("assembly range does not match the chunk");
+        if (in)
           S.fv.at(fb + p) += S.fv.at(C.fv_off + (int64_t)col * S.L->fv_stride + C.ncols + a);
       }
     }

@@ -10,11 +10,16 @@
 // here it is this file's own bisection producing the same (perm, cParent,
 // cMember) triple.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <exception>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 #include "rsc_host.h"
 
@@ -30,30 +35,45 @@ std::string& last_error() {
 // --------------------------------------------------------------------------
 namespace {
 
+// Per-vertex scratch shared by the dissection of disjoint vertex sets (one
+// thread per set near the top of the tree).  A set is marked with a stamp
+// value no other set uses, so a neighbour in another set never reads as a
+// member; side[], level[] and cmember[] are only touched for members.
+struct NdShared {
+  std::unique_ptr<std::atomic<int>[]> stamp;
+  std::vector<int> side;     // 0 = L, 1 = R side
+  std::vector<int> level;    // BFS levels
+  std::vector<int> cmember;  // component of each vertex (local, then global)
+  std::atomic<int> next_stamp{0};
+  explicit NdShared(int n) : stamp(new std::atomic<int>[n]), side(n, 0), level(n, -1),
+                             cmember(n, -1) {
+    for (int v = 0; v < n; ++v) stamp[v].store(-1, std::memory_order_relaxed);
+  }
+};
+
+// Dissection of one vertex set.  perm and cparent are this set's own, with
+// components numbered from 0 in creation (post)order.
 struct Dissector {
   const Csc& A;
   const double* xyz;
   int nd_small;
+  NdShared& sh;
+  std::vector<int>& side;
+  std::vector<int>& level;
+  std::vector<int>& cmember;
   std::vector<int> perm;
   std::vector<int> cparent;
-  std::vector<int> cmember;
-  std::vector<int> stamp;   // stamp[v] == cur -> v in current set
-  std::vector<int> side;    // 0 = L, 1 = R side, 2 = separator
-  std::vector<int> level;   // BFS levels
   int cur = 0;
 
-  Dissector(const Csc& A_, const double* xyz_, int nd_small_)
-      : A(A_), xyz(xyz_), nd_small(std::max(nd_small_, 1)) {
-    perm.reserve(A.n);
-    cmember.assign(A.n, -1);
-    stamp.assign(A.n, -1);
-    side.assign(A.n, 0);
-    level.assign(A.n, -1);
-  }
+  Dissector(const Csc& A_, const double* xyz_, int nd_small_, NdShared& sh_)
+      : A(A_), xyz(xyz_), nd_small(std::max(nd_small_, 1)), sh(sh_), side(sh_.side),
+        level(sh_.level), cmember(sh_.cmember) {}
+
+  bool member(int u) const { return sh.stamp[u].load(std::memory_order_relaxed) == cur; }
 
   void mark_set(const std::vector<int>& nodes) {
-    ++cur;
-    for (int v : nodes) stamp[v] = cur;
+    cur = sh.next_stamp.fetch_add(1) + 1;
+    for (int v : nodes) sh.stamp[v].store(cur, std::memory_order_relaxed);
   }
 
   // Pseudo-peripheral node of the subgraph induced by `nodes`
@@ -73,7 +93,7 @@ struct Dissector {
       }
       for (int p = A.p[v]; p < A.p[v + 1]; ++p) {
         int u = A.i[p];
-        if (u == v || stamp[u] != cur || level[u] >= 0) continue;
+        if (u == v || !member(u) || level[u] >= 0) continue;
         level[u] = level[v] + 1;
         order.push_back(u);
       }
@@ -169,7 +189,7 @@ struct Dissector {
       bool adj = false;
       for (int p = A.p[v]; p < A.p[v + 1] && !adj; ++p) {
         int u = A.i[p];
-        if (stamp[u] == cur && side[u] == 0) adj = true;
+        if (member(u) && side[u] == 0) adj = true;
       }
       if (adj) S.push_back(v);
       else R.push_back(v);
@@ -234,18 +254,84 @@ struct Dissector {
   }
 };
 
+// A dissected vertex set: its ordering, its components (local numbers) and
+// the roots among them.
+struct NdPart {
+  std::vector<int> perm, cparent, roots;
+};
+
+// The recursion of Dissector::dissect with the two halves of the sets near
+// the top run on two threads; the halves' results are stitched exactly as
+// the sequential recursion would have produced them (left components, right
+// components renumbered after them, then the separator).
+NdPart dissect_parallel(const Csc& A, const double* xyz, int nd_small, NdShared& sh,
+                        std::vector<int>& nodes, int depth) {
+  NdPart out;
+  Dissector d(A, xyz, nd_small, sh);
+  std::vector<int> L, S, R;
+  if (depth <= 0 || nodes.empty() || (int)nodes.size() <= d.nd_small ||
+      !d.split(nodes, L, S, R)) {
+    // sequential from here (dissect repeats the size and split tests)
+    Dissector q(A, xyz, nd_small, sh);
+    out.roots = q.dissect(nodes);
+    out.perm.swap(q.perm);
+    out.cparent.swap(q.cparent);
+    return out;
+  }
+  std::vector<int>().swap(nodes);
+  NdPart left;
+  std::exception_ptr err;
+  std::thread th([&]() {
+    try {
+      left = dissect_parallel(A, xyz, nd_small, sh, L, depth - 1);
+    } catch (...) {
+      err = std::current_exception();
+    }
+  });
+  NdPart right = dissect_parallel(A, xyz, nd_small, sh, R, depth - 1);
+  th.join();
+  if (err) std::rethrow_exception(err);
+  const int nl = (int)left.cparent.size(), nr = (int)right.cparent.size();
+  out.perm.swap(left.perm);
+  out.perm.insert(out.perm.end(), right.perm.begin(), right.perm.end());
+  for (int v : right.perm) sh.cmember[v] += nl;
+  out.cparent.swap(left.cparent);
+  for (int p : right.cparent) out.cparent.push_back(p >= 0 ? p + nl : -1);
+  if (S.empty()) {
+    out.roots = left.roots;
+    for (int r : right.roots) out.roots.push_back(r + nl);
+    return out;
+  }
+  const int c = nl + nr;
+  out.cparent.push_back(-1);
+  std::sort(S.begin(), S.end());
+  for (int v : S) {
+    out.perm.push_back(v);
+    sh.cmember[v] = c;
+  }
+  for (int r : left.roots) out.cparent[r] = c;
+  for (int r : right.roots) out.cparent[r + nl] = c;
+  out.roots.push_back(c);
+  return out;
+}
+
 }  // namespace
 
 NDResult nested_dissection(const Csc& A, const double* xyz, int nd_small) {
-  Dissector d(A, xyz, nd_small);
+  NdShared sh(A.n);
   std::vector<int> all(A.n);
   std::iota(all.begin(), all.end(), 0);
-  d.dissect(all);
-  if ((int)d.perm.size() != A.n) throw std::runtime_error("ND lost vertices");
+  // two threads per level for the top levels, up to 16 at once
+  unsigned hw = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+  if (const char* e = std::getenv("RSC_ND_THREADS")) hw = (unsigned)std::max(1, std::atoi(e));
+  int depth = 0;
+  while ((2u << depth) <= hw) ++depth;
+  NdPart part = dissect_parallel(A, xyz, nd_small, sh, all, depth);
+  if ((int)part.perm.size() != A.n) throw std::runtime_error("ND lost vertices");
   NDResult r;
-  r.perm = std::move(d.perm);
-  r.cparent = std::move(d.cparent);
-  r.cmember = std::move(d.cmember);
+  r.perm = std::move(part.perm);
+  r.cparent = std::move(part.cparent);
+  r.cmember = std::move(sh.cmember);
   return r;
 }
 

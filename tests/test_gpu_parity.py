@@ -150,6 +150,32 @@ def test_block_pcg_is_batch_invariant():
         assert res[j]["true_relres"] < 1e-7
 
 
+@pytest.mark.parametrize("env", [dict(RSC_APPLY_LEVELS="1"), dict(RSC_SUBTREE_ELEMS="1"),
+                                 dict(RSC_SUBTREE_ELEMS="1000000")])
+def test_apply_schedules_agree(monkeypatch, env):
+    """The level-scheduled launches and the dataflow launch with no or with
+    large subtree tasks compute the same apply (one and three right-hand
+    sides) as the default dataflow schedule, to rounding."""
+    build, sopts, okw, tol = CASES["p3d20_hodlr"]
+    A, xyz = build()
+    n = A._nrow
+    B = R.uniform_rhs(3 * n, 77).reshape(3, n).T.copy()
+    ref = {}
+    M = _factor(A, xyz, **dict(sopts))
+    for k in (1, 3):
+        X = np.empty((n, k))
+        M.solve(X, B[:, :k].copy())
+        ref[k] = X
+    del M
+    for key, val in env.items():
+        monkeypatch.setenv(key, val)
+    M = _factor(A, xyz, **dict(sopts))
+    for k in (1, 3):
+        X = np.empty((n, k))
+        M.solve(X, B[:, :k].copy())
+        assert relerr(X, ref[k]) < 1e-12
+
+
 def test_multi_rhs_apply_matches_oracle():
     """The matrix-core apply bodies (two and seventeen right-hand sides: one
     full group of sixteen plus a partial one) against the FP64 oracle."""

@@ -320,3 +320,19 @@ def test_symbolic_is_deterministic():
     b = symbolic(A, xyz, max_dense_block=63)
     for k in ("perm", "first", "compressed", "rows", "rank"):
         assert np.array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("build", [lambda: R.poisson3d(24, 1.0, 1234), lambda: R.elasticity3d(5),
+                                   lambda: R.poisson2d(40)])
+def test_parallel_dissection_equals_sequential(monkeypatch, build):
+    """Near the top of the tree the two halves of a split are dissected on
+    two threads and stitched; the ordering and the component tree must be
+    those of the one-thread recursion."""
+    A, xyz = build()
+    monkeypatch.setenv("RSC_ND_THREADS", "1")
+    seq = symbolic(A, xyz, max_dense_block=31)
+    for t in ("2", "16"):
+        monkeypatch.setenv("RSC_ND_THREADS", t)
+        par = symbolic(A, xyz, max_dense_block=31)
+        for key in ("perm", "first", "compressed", "parent", "rows", "level"):
+            assert np.array_equal(par[key], seq[key]), (t, key)

@@ -516,9 +516,14 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         red[wave][lane] = act ? acc0[cc] + acc1[cc] : 0.0;
         __syncthreads();
         if (wave == 0 && act) {
-          double* y = vsel(b, t.ysel, c0 + cc) + t.yoff;
           const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-          vst<COH>(y + lane, t.op ? vld<COH>(y + lane) - v : v);
+          if (t.part >= 0) {
+            vst<COH>(b.part + (int64_t)(c0 + cc) * b.ps +
+                         (int64_t)(t.part + (lane >> 5)) * 32 + (lane & 31), v);
+          } else {
+            double* y = vsel(b, t.ysel, c0 + cc) + t.yoff;
+            vst<COH>(y + lane, t.op ? vld<COH>(y + lane) - v : v);
+          }
         }
         __syncthreads();
       }
@@ -636,9 +641,14 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
         for (int v = 0; v < 4; ++v) {
           const int row = wave * 16 + kq + 4 * v;
           if (row < t.rows && li < kb) {
-            double* y = vsel(b, t.ysel, c0 + li) + t.yoff + row;
             const double val = acc0[v] + acc1[v];
-            vst<COH>(y, t.op ? vld<COH>(y) - val : val);
+            if (t.part >= 0) {
+              vst<COH>(b.part + (int64_t)(c0 + li) * b.ps +
+                           (int64_t)(t.part + (row >> 5)) * 32 + (row & 31), val);
+            } else {
+              double* y = vsel(b, t.ysel, c0 + li) + t.yoff + row;
+              vst<COH>(y, t.op ? vld<COH>(y) - val : val);
+            }
           }
         }
     } else {
@@ -713,9 +723,10 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
     double* y = vsel(b, t.ysel, col) + t.yoff;
     if (t.kind == 2) {
       const double* pp = b.part + (int64_t)col * b.ps + (int64_t)t.part * 32;
+      const int64_t rs = 32 * (int64_t)(t.ld > 0 ? t.ld : 1);
       if (tid < t.cols) {
         double acc = 0;
-        for (int s2 = 0; s2 < t.rows; ++s2) acc += vld<COH>(pp + (int64_t)s2 * 32 + tid);
+        for (int s2 = 0; s2 < t.rows; ++s2) acc += vld<COH>(pp + (int64_t)s2 * rs + tid);
         vst<COH>(y + tid, t.op ? vld<COH>(y + tid) - acc : acc);
       }
       continue;

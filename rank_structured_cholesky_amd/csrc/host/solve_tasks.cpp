@@ -243,6 +243,7 @@ NodeStages node_stages(const Plan& P, const SolveLayout& L, int j) {
 struct TaskEmitter {
   int64_t slot = 0;
   static constexpr int CHUNK = 512;
+  int row_split = 0;  // > 0: ROW tasks longer than this many columns are split
   void emit(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
     if (m.col) {
       for (int j0 = 0; j0 < m.cols; j0 += 32) {
@@ -298,7 +299,38 @@ struct TaskEmitter {
         t.part = -1;
         t.xoff = m.xo + (m.upper ? i0 : 0);
         t.yoff = m.yo + i0;
-        main.push_back(t);
+        if (row_split <= 0 || t.cols <= row_split) {
+          main.push_back(t);
+          continue;
+        }
+        // long rows: column chunks write partial sums, rows 0..31 of chunk
+        // s2 into slot base + 2 s2 and rows 32..63 into base + 2 s2 + 1;
+        // one reduction per half sums its slots (stride 2)
+        const int nsub = (t.cols + row_split - 1) / row_split;
+        for (int s2 = 0; s2 < nsub; ++s2) {
+          dev::GvTask u = t;
+          const int c0 = s2 * row_split;
+          u.mat = t.mat + (int64_t)c0 * t.ld;
+          u.cols = std::min(row_split, t.cols - c0);
+          u.xoff = t.xoff + c0;
+          u.part = (int)(slot + 2 * s2);
+          main.push_back(u);
+        }
+        for (int h = 0; h < 2; ++h) {
+          const int nr = std::min(32, t.rows - 32 * h);
+          if (nr <= 0) continue;
+          dev::GvTask rd{};
+          rd.kind = 2;
+          rd.part = (int)(slot + h);
+          rd.rows = nsub;
+          rd.cols = nr;
+          rd.ld = 2;
+          rd.op = m.op;
+          rd.ysel = m.ys;
+          rd.yoff = t.yoff + 32 * h;
+          red.push_back(rd);
+        }
+        slot += 2 * nsub;
       }
     }
   }
@@ -395,6 +427,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   std::vector<int> gwoff{0}, gw;
   std::vector<int> lastA(ns, -1), lastB(ns, -1), lastC(ns, -1), a0(ns, -1);
   TaskEmitter em;
+  em.row_split = kRowSplitCols;
   auto new_group = [&](const std::vector<int>& waits) {
     need.push_back(0u);
     gw.insert(gw.end(), waits.begin(), waits.end());

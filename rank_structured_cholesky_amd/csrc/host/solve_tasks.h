@@ -24,7 +24,10 @@ namespace dev {
 struct GvTask {
   int64_t mat;
   int ld, rows, cols;
-  int kind;  // 0 ROW, 1 COL, 2 REDUCE (part = first slot, rows = slot count)
+  int kind;  // 0 ROW, 1 COL, 2 REDUCE (part = first slot, rows = slot count,
+             // ld = slot stride, 0 read as 1)
+             // part >= 0 on ROW / COL: write partial sums to slot part (COL)
+             // or slots part (rows 0..31) and part + 1 (rows 32..63) (ROW)
   int op;    // 0 set, 1 subtract
   int xsel, ysel;
   int part;
@@ -64,6 +67,7 @@ struct DfTask {
 }  // namespace dev
 
 constexpr int kSolveChunk = 1024;    // front-vector entries per assembly/gather task
+constexpr int kRowSplitCols = 1024;  // dataflow ROW tasks: columns per partial task
 constexpr int kScatterChunk = 4096;  // vector entries per scatter task
 // Subtree tasks: one workgroup runs every supernode of a small subtree in
 // turn; a node's ncols and off-diagonal rows must fit its LDS vectors.

@@ -141,10 +141,11 @@ void gemv(State& S, const dev::GvTask& t) {
     Buf& Y = S.vec(t.ysel);
     const int64_t yb = (int64_t)col * S.stride(t.ysel) + t.yoff;
     if (t.kind == 2) {
+      const int64_t rs = t.ld > 0 ? t.ld : 1;
       for (int j = 0; j < t.cols; ++j) {
         double acc = 0;
         for (int s2 = 0; s2 < t.rows; ++s2)
-          acc += S.part.at((int64_t)col * S.ps + ((int64_t)t.part + s2) * 32 + j);
+          acc += S.part.at((int64_t)col * S.ps + ((int64_t)t.part + s2 * rs) * 32 + j);
         if (t.op) Y.at(yb + j) -= acc;
         else Y.at(yb + j) = acc;
       }
@@ -158,7 +159,8 @@ void gemv(State& S, const dev::GvTask& t) {
         double acc = 0;
         for (int j = 0; j < t.cols; ++j)
           acc += S.F.at(t.mat + i + (int64_t)j * t.ld) * Xv.at(xb + j);
-        if (t.op) Y.at(yb + i) -= acc;
+        if (t.part >= 0) S.part.at((int64_t)col * S.ps + ((int64_t)t.part + (i >> 5)) * 32 + (i & 31)) = acc;
+        else if (t.op) Y.at(yb + i) -= acc;
         else Y.at(yb + i) = acc;
       }
     } else {
@@ -372,7 +374,21 @@ int main(int argc, char** argv) {
     for (int q = 0; q < ng; ++q)
       if (cnt[q] != D.need[q]) fail("group " + std::to_string(q) + " count mismatch");
   }
-  if (S.X.v != x_level) fail("sequential dataflow result differs from the level path");
+  // split ROW tasks (long rows, partial sums) exist only in the dataflow
+  // list: then the two lists agree to rounding, otherwise bit for bit
+  bool exact = true;
+  for (const dev::GvTask& t : D.gv)
+    if (t.kind == 0 && t.part >= 0) exact = false;
+  auto agrees = [&](const std::vector<double>& x) {
+    if (exact) return x == x_level;
+    double dn = 0, rn = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+      dn += (x[i] - x_level[i]) * (x[i] - x_level[i]);
+      rn += x_level[i] * x_level[i];
+    }
+    return std::sqrt(dn) <= 1e-12 * std::sqrt(rn);
+  };
+  if (!agrees(S.X.v)) fail("sequential dataflow result differs from the level path");
 
   // (3) dataflow list on threads: tickets + completion counters + waits
   S.reset_work();
@@ -407,9 +423,9 @@ int main(int argc, char** argv) {
     for (auto& t : th) t.join();
     if (stuck) fail("threaded dataflow run stalled");
   }
-  if (S.X.v != x_level) fail("threaded dataflow result differs from the level path");
-  std::printf("ok n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d\n",
-              S.n, (int)P.nodes.size(), nlev, L.sub.root.size(), LT.gv.size(), D.tasks.size(),
-              ng);
+  if (!agrees(S.X.v)) fail("threaded dataflow result differs from the level path");
+  std::printf("ok%s n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d\n",
+              exact ? "" : " (split rows)", S.n, (int)P.nodes.size(), nlev, L.sub.root.size(),
+              LT.gv.size(), D.tasks.size(), ng);
   return 0;
 }

@@ -1,7 +1,8 @@
 """The apply's task lists on the CPU (tests/cpp/emulate_apply.cpp): the
-dataflow list run in ticket order and on threads gives bitwise the result of
-the level-scheduled list, every wait is satisfied when its task starts, and
-every buffer access is in range; with and without subtree tasks."""
+dataflow list run in ticket order and on threads gives the result of the
+level-scheduled list (bitwise, or to rounding when the dataflow list splits
+long rows into partial sums), every wait is satisfied when its task starts,
+and every buffer access is in range; with and without subtree tasks."""
 import os
 import subprocess
 
@@ -30,6 +31,8 @@ def built():
     "p3d 24 63 128 3 8",     # split column tasks (partial sums)
     "p3d 20 63 50 1 8 0",    # no subtree tasks
     "p3d 20 63 50 1 8 100000000",  # subtree tasks as large as allowed
+    "p3d 36 2000 400 2 8",   # uncompressed separators over 1024 columns: split rows
+    "p2d 1100 1000000 4000 1 8 0",  # 1100-column top separator, no subtree tasks
 ])
 def test_dataflow_schedule_matches_level_schedule(args):
     r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)

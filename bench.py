@@ -53,8 +53,10 @@ def parse():
     ap.add_argument("--max-rhs", type=int, default=0,
                     help="serving: largest right-hand-side count of a request")
     ap.add_argument("--traffic-run", action="store_true",
-                    help="for HBM counter passes: factor (FP64), 3 applies, then 3 plain "
-                         "256 MiB copies as the calibration kernel; no timing line")
+                    help="for counter passes: factor, 3 applies, then 3 plain 256 MiB "
+                         "copies as the calibration kernel; no timing line")
+    ap.add_argument("--mixed-factor", action="store_true",
+                    help="--traffic-run with the mixed factor (FP32 + BF16 sketches)")
     return ap.parse_args()
 
 
@@ -166,7 +168,7 @@ def serving_leg(cfg, eps):
         n=[m._nrow for m in mats])
 
 
-def traffic_run(cfg):
+def traffic_run(cfg, precision=0):
     """The program the FETCH_SIZE / WRITE_SIZE counter passes profile: the
     apply dispatches (k_apply_df) and a copy kernel of known size."""
     import torch
@@ -175,7 +177,7 @@ def traffic_run(cfg):
     from rank_structured_cholesky_amd import _lib
     L = _lib.lib()
     A, xyz, B = build_problem(cfg)
-    M, _ = factor(A, xyz, cfg, 0)
+    M, _ = factor(A, xyz, cfg, precision)
     n, k = B.shape
     dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")
     dY = torch.empty_like(dB)
@@ -356,7 +358,7 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     if a.traffic_run:
-        traffic_run(cfg)
+        traffic_run(cfg, 1 if a.mixed_factor else 0)
         return
     if a.config == "serving":
         cfg = dict(cfg)

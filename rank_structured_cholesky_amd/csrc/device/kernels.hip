@@ -245,62 +245,87 @@ void launch_gemm_bf16(bool ta, bool tb, const GemmJob* jobs, const GemmTile* til
 // status[0] = 1 and status[1] = min(node); the pivot is replaced by 1 so the
 // remaining arithmetic stays finite (the factorization is then retried).
 // ---------------------------------------------------------------------------
+// Register-resident tile: thread (row i = tid % 64, column class r = tid / 64)
+// holds S[i][r + 4m], m = 0..15, so the global loads and stores are
+// coalesced along rows and the trailing update needs no index arithmetic.
+// Column k is finished by wave k % 4 (its lane k takes the pivot and shares
+// it by a wave shuffle) and published in LDS; one barrier per column, the
+// column buffer alternating between two copies.  The arithmetic per entry is
+// that of the textbook right-looking loop in the same order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf(const PotrfJob* __restrict__ jobs,
                                                int* __restrict__ status) {
-  __shared__ T S[TILE][TILE + 1];
+  __shared__ T colk[2][TILE];
+  __shared__ T diag[TILE];
   __shared__ int bad;
   __shared__ T dfloor;
   const PotrfJob J = jobs[blockIdx.x];
   T* A = (T*)J.A;
   const int n = J.n, tid = threadIdx.x;
-  if (tid == 0) bad = 0;
-  for (int e = tid; e < n * n; e += 256) {
-    int i = e % n, j = e / n;
-    if (i >= j) S[i][j] = A[i + (int64_t)j * J.lda];
+  const int i = tid & 63, r = tid >> 6;
+  T a[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int j = r + 4 * m;
+    a[m] = (i < n && j < n && i >= j) ? A[i + (int64_t)j * J.lda] : (T)0;
   }
+#pragma unroll
+  for (int m = 0; m < 16; ++m)
+    if (i < n && r + 4 * m == i) diag[i] = a[m];
+  if (tid == 0) bad = 0;
   __syncthreads();
   if (tid == 0) {
     T mx = 0;
-    for (int i = 0; i < n; ++i) mx = S[i][i] > mx ? S[i][i] : mx;
+    for (int q = 0; q < n; ++q) mx = diag[q] > mx ? diag[q] : mx;
     const T u = sizeof(T) == 8 ? (T)1.1102230246251565e-16 : (T)5.960464477539063e-08;
     const T rel = J.tolerant == 2 ? (T)1e-5 : u;
     dfloor = mx > (T)0 ? mx * rel : (T)1;
   }
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    if (tid == 0) {
-      T d = S[k][k];
-      if (J.tolerant == 2) {
-        if (!(d > dfloor)) {
-          d = dfloor;
-          bad += 1;
+  // fully unrolled over the tile so that every register index is a constant
+#pragma unroll
+  for (int k = 0; k < TILE; ++k) {
+    if (k >= n) break;
+    const int buf = k & 1;
+    if (r == (k & 3)) {
+      const int km = k >> 2;
+      T v = a[km];
+      if (i == k) {
+        T d = v;
+        if (J.tolerant == 2) {
+          if (!(d > dfloor)) {
+            d = dfloor;
+            bad += 1;
+          }
+        } else if (!(d > (T)0)) {
+          if (J.tolerant) {
+            d = dfloor;
+          } else {
+            bad = 1;
+            d = (T)1;
+          }
         }
-      } else if (!(d > (T)0)) {
-        if (J.tolerant) {
-          d = dfloor;
-        } else {
-          bad = 1;
-          d = (T)1;
-        }
+        v = sqrt(d);
       }
-      S[k][k] = sqrt(d);
+      const T piv = __shfl(v, k, 64);
+      if (i > k) v /= piv;
+      if (i >= k) a[km] = v;
+      colk[buf][i] = (i >= k && i < n) ? v : (T)0;
     }
     __syncthreads();
-    const T piv = S[k][k];
-    for (int i = k + 1 + tid; i < n; i += 256) S[i][k] /= piv;
-    __syncthreads();
-    const int w = n - k - 1;
-    for (int e = tid; e < w * w; e += 256) {
-      int i = k + 1 + e % w, j = k + 1 + e / w;
-      if (i >= j) S[i][j] -= S[i][k] * S[j][k];
+    const T ci = colk[buf][i];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int j = r + 4 * m;
+      if (j > k && j <= i && i < n) a[m] -= ci * colk[buf][j];
     }
-    __syncthreads();
   }
-  for (int e = tid; e < n * n; e += 256) {
-    int i = e % n, j = e / n;
-    if (i >= j) A[i + (int64_t)j * J.lda] = S[i][j];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int j = r + 4 * m;
+    if (i < n && j < n && i >= j) A[i + (int64_t)j * J.lda] = a[m];
   }
+  __syncthreads();
   if (tid == 0 && bad) {
     if (J.tolerant == 2) {
       atomicAdd(&status[2], bad);
@@ -324,8 +349,11 @@ void launch_potrf(const PotrfJob* jobs, int njobs, int* status, hipStream_t s) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm(const TrsmJob* __restrict__ jobs,
                                               const RowTile* __restrict__ tiles) {
+  // thread (row i = tid % 64, column class g = tid / 64) holds B[i][g + 4m]
+  // in registers; column k is finished by wave k % 4 and shared through LDS
+  // (two alternating copies, one barrier per column)
   __shared__ T Ls[TILE][TILE + 1];
-  __shared__ T Bs[TILE][TILE + 1];
+  __shared__ T bk[2][TILE];
   const RowTile tl = tiles[blockIdx.x];
   const TrsmJob J = jobs[tl.job];
   const T* L = (const T*)J.L;
@@ -333,25 +361,39 @@ __global__ __launch_bounds__(256) void k_trsm(const TrsmJob* __restrict__ jobs,
   const int nb = J.nb, tid = threadIdx.x;
   const int rows = min(TILE, J.m - tl.row0);
   for (int e = tid; e < nb * nb; e += 256) {
-    int i = e % nb, j = e / nb;
-    Ls[i][j] = (i >= j) ? L[i + (int64_t)j * J.ldl] : (T)0;
+    int ii = e % nb, jj = e / nb;
+    Ls[ii][jj] = (ii >= jj) ? L[ii + (int64_t)jj * J.ldl] : (T)0;
   }
-  for (int e = tid; e < TILE * nb; e += 256) {
-    int r = e % TILE, j = e / TILE;
-    Bs[r][j] = (r < rows) ? B[(tl.row0 + r) + (int64_t)j * J.ldb] : (T)0;
+  const int i = tid & 63, g = tid >> 6;
+  const bool rok = i < rows;
+  T b[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int j = g + 4 * m;
+    b[m] = (rok && j < nb) ? B[(tl.row0 + i) + (int64_t)j * J.ldb] : (T)0;
   }
   __syncthreads();
-  const int r = tid & 63, g = tid >> 6;
-  for (int k = 0; k < nb; ++k) {
-    if (g == 0) Bs[r][k] /= Ls[k][k];
+#pragma unroll
+  for (int k = 0; k < TILE; ++k) {
+    if (k >= nb) break;
+    const int buf = k & 1;
+    if (g == (k & 3)) {
+      const T v = b[k >> 2] / Ls[k][k];
+      b[k >> 2] = v;
+      bk[buf][i] = v;
+    }
     __syncthreads();
-    const T bk = Bs[r][k];
-    for (int j = k + 1 + g; j < nb; j += 4) Bs[r][j] -= bk * Ls[j][k];
-    __syncthreads();
+    const T bv = bk[buf][i];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int j = g + 4 * m;
+      if (j > k && j < nb) b[m] -= bv * Ls[j][k];
+    }
   }
-  for (int e = tid; e < TILE * nb; e += 256) {
-    int rr = e % TILE, j = e / TILE;
-    if (rr < rows) B[(tl.row0 + rr) + (int64_t)j * J.ldb] = Bs[rr][j];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int j = g + 4 * m;
+    if (rok && j < nb) B[(tl.row0 + i) + (int64_t)j * J.ldb] = b[m];
   }
 }
 

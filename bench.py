@@ -52,6 +52,8 @@ def parse():
                     help="serving: number of requests (default from the config)")
     ap.add_argument("--max-rhs", type=int, default=0,
                     help="serving: largest right-hand-side count of a request")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving: mean request arrivals per second")
     ap.add_argument("--traffic-run", action="store_true",
                     help="for counter passes: factor, 3 applies, then 3 plain 256 MiB "
                          "copies as the calibration kernel; no timing line")
@@ -366,6 +368,8 @@ def main():
             cfg["requests"] = a.requests
         if a.max_rhs:
             cfg["max_rhs"] = a.max_rhs
+        if a.rate:
+            cfg["rate"] = a.rate
         r = serving_leg(cfg, a.eps)
         print(json.dumps({
             "metric": "served PCG solves/s to 1e-8 (right-hand sides per second), "

@@ -1412,6 +1412,11 @@ void Engine<T>::setup_static() {
         khz <= 0)
       khz = 100000;
     df_timeout_ = (unsigned long long)khz * 1000ull * 4ull;  // 4 s
+    // RSC_DF_TIMEOUT_US shortens the bound of every dependency wait (for
+    // tests of the abort path: the launch drains and the apply reports
+    // RSC_DEVICE_ERROR)
+    if (const char* e = std::getenv("RSC_DF_TIMEOUT_US"))
+      df_timeout_ = (unsigned long long)khz * (unsigned long long)std::atoll(e) / 1000ull;
     df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
     std::vector<int> z = {0, 0};
     dapst_.upload(z, st_);

@@ -271,3 +271,26 @@ def test_full_rank_compression_is_exact(name):
     M.solve(x, b)
     e = spla.spsolve(A.to_scipy().tocsc(), b)
     assert relerr(x, e) < 1e-9, relerr(x, e)
+
+
+def test_apply_wait_timeout_is_reported_and_device_recovers(monkeypatch):
+    """A dependency wait that exceeds its bound (forced here with a zero
+    bound) aborts the dataflow launch: every workgroup drains, the apply and
+    the PCG solve report a device error, and the next factor works."""
+    A, xyz = R.poisson3d(16, 1.0, 1234)
+    b = R.uniform_rhs(A._nrow, 5678)
+    monkeypatch.setenv("RSC_DF_TIMEOUT_US", "0")
+    M = _factor(A, xyz, max_dense_block=63)
+    y = np.empty_like(b)
+    with pytest.raises(_lib.RscError) as e:
+        M.solve(y, b)
+    assert e.value.status == _lib.RSC_DEVICE_ERROR
+    assert "timed out" in str(e.value)
+    with pytest.raises(_lib.RscError):
+        R.PCG_SOLVER(A, 1e-8, 50).solvePCG(np.zeros_like(b), b, M)
+    del M
+    monkeypatch.delenv("RSC_DF_TIMEOUT_US")
+    M2 = _factor(A, xyz, max_dense_block=63)
+    x = np.zeros_like(b)
+    res = R.PCG_SOLVER(A, 1e-8, 500).solvePCG(x, b, M2)
+    assert res["true_relres"] < 1e-7

@@ -102,6 +102,13 @@ inline bool inside_subtree(const Supernode& s, int q, int sub) {
   return false;
 }
 
+// One dataflow task list on the device.
+struct DfList {
+  DBuf tasks, gv, gwoff, gw, need, cnt;
+  int ntasks = 0, ngroups = 0, grid = 1;
+  std::vector<dev::DfTask> host;  // kept for the optional timing trace
+};
+
 template <typename T>
 class Engine final : public FactorEngine {
  public:
@@ -179,7 +186,7 @@ class Engine final : public FactorEngine {
   DBuf dpart_;
   std::vector<int64_t> fv_off_;
   // dataflow apply (one persistent launch per application)
-  DBuf ddf_, ddfgv_, dgwoff_, dgw_, dneed_, dcnt_, dapst_, dtrace_;
+  DBuf dapst_, dtrace_;
   DBuf dstptr_, dstnodes_, dpanel_;
   int nsub_ = 0;
   dev::SubtreeArgs subtree_args(const double* B, int ldb) const {
@@ -197,15 +204,18 @@ class Engine final : public FactorEngine {
     return a;
   }
   SolveLayout sl_;
+  // dataflow task lists: [0] with subtree tasks (single right-hand side),
+  // [1] without (multi-right-hand-side apply)
+  DfList df_[2];
+  int df_last_ = 0;  // list of the last apply (for the trace)
   std::thread df_thread_;
-  std::unique_ptr<DataflowTasks> df_pending_;
+  std::unique_ptr<DataflowTasks> df_pending_, df_pending2_;
   std::exception_ptr df_err_;
   bool solve_ready_ = false;
   int df_resident_ = 1;
   void finish_solve_setup();
-  std::vector<dev::DfTask> df_host_;  // kept for the optional timing trace
   std::string df_trace_path_;
-  int df_ntasks_ = 0, df_ngroups_ = 0, df_grid_ = 1;
+
   unsigned long long df_timeout_ = 0;
   bool use_dataflow_ = false;
   int64_t fv_stride_ = 0;
@@ -1296,24 +1306,30 @@ void Engine<T>::finish_solve_setup() {
     df_err_ = nullptr;
     std::rethrow_exception(e);
   }
-  DataflowTasks& DT = *df_pending_;
-  df_ntasks_ = (int)DT.tasks.size();
-  df_ngroups_ = (int)DT.need.size();
-  ddf_.upload(DT.tasks, st_);
   const char* tp = std::getenv("RSC_DF_TRACE");
-  if (tp && tp[0]) {
-    df_trace_path_ = tp;
-    df_host_ = DT.tasks;
-    dtrace_.alloc((size_t)4 * DT.tasks.size() * sizeof(unsigned long long));
+  if (tp && tp[0]) df_trace_path_ = tp;
+  int64_t slots = 0, maxtasks = 0;
+  for (int q = 0; q < 2; ++q) {
+    DataflowTasks& DT = q == 0 ? *df_pending_ : *df_pending2_;
+    DfList& D = df_[q];
+    D.ntasks = (int)DT.tasks.size();
+    D.ngroups = (int)DT.need.size();
+    D.tasks.upload(DT.tasks, st_);
+    D.gv.upload(DT.gv, st_);
+    D.gwoff.upload(DT.gwoff, st_);
+    D.gw.upload(DT.gw, st_);
+    D.need.upload(DT.need, st_);
+    D.cnt.alloc((size_t)(D.ngroups + 1) * sizeof(unsigned));
+    D.grid = std::max(1, std::min(D.ntasks, df_resident_));
+    if (!df_trace_path_.empty()) D.host = DT.tasks;
+    slots = std::max(slots, DT.slots);
+    maxtasks = std::max<int64_t>(maxtasks, D.ntasks);
   }
-  ddfgv_.upload(DT.gv, st_);
-  dgwoff_.upload(DT.gwoff, st_);
-  dgw_.upload(DT.gw, st_);
-  dneed_.upload(DT.need, st_);
-  dcnt_.alloc((size_t)(df_ngroups_ + 1) * sizeof(unsigned));
-  part_stride_ = std::max<int64_t>(DT.slots * 32, 1);
-  df_grid_ = std::max(1, std::min(df_ntasks_, df_resident_));
+  if (!df_trace_path_.empty())
+    dtrace_.alloc((size_t)4 * maxtasks * sizeof(unsigned long long));
+  part_stride_ = std::max<int64_t>(slots * 32, 1);
   df_pending_.reset();
+  df_pending2_.reset();
   RSC_HIP(hipStreamSynchronize(st_));
   solve_ready_ = true;
 }
@@ -1388,6 +1404,8 @@ void Engine<T>::setup_static() {
       try {
         df_pending_.reset(new DataflowTasks());
         build_dataflow_tasks(plan_, sl_, *df_pending_);
+        df_pending2_.reset(new DataflowTasks());
+        build_dataflow_tasks(plan_, sl_, *df_pending2_, false);
       } catch (...) {
         df_err_ = std::current_exception();
       }
@@ -1723,14 +1741,16 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
                   live, dz_.as<double>(), multi_min_k};
   if (use_dataflow_) {
     dev::DfArgs a{};
-    a.tasks = ddf_.as<dev::DfTask>();
-    a.ntasks = df_ntasks_;
-    a.ngroups = df_ngroups_;
-    a.gv = ddfgv_.as<dev::GvTask>();
-    a.gwait_off = dgwoff_.as<int>();
-    a.gwait = dgw_.as<int>();
-    a.need = dneed_.as<unsigned>();
-    a.cnt = dcnt_.as<unsigned>();
+    df_last_ = k >= multi_min_k ? 1 : 0;
+    DfList& D = df_[df_last_];
+    a.tasks = D.tasks.as<dev::DfTask>();
+    a.ntasks = D.ntasks;
+    a.ngroups = D.ngroups;
+    a.gv = D.gv.as<dev::GvTask>();
+    a.gwait_off = D.gwoff.as<int>();
+    a.gwait = D.gw.as<int>();
+    a.need = D.need.as<unsigned>();
+    a.cnt = D.cnt.as<unsigned>();
     a.status = dapst_.as<int>();
     a.timeout_ticks = df_timeout_;
     a.nodes = dvnodes_.as<dev::VecNode>();
@@ -1745,8 +1765,8 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     a.ldx = ldx;
     a.trace = dtrace_.p ? dtrace_.as<unsigned long long>() : nullptr;
     a.sub = subtree_args(B, ldb);
-    dev::launch_df_reset(dcnt_.as<unsigned>(), df_ngroups_ + 1, s);
-    dev::launch_apply_dataflow<T>(F(), a, vb, k, df_grid_, s);
+    dev::launch_df_reset(D.cnt.as<unsigned>(), D.ngroups + 1, s);
+    dev::launch_apply_dataflow<T>(F(), a, vb, k, D.grid, s);
     return;
   }
   dev::launch_gather(B, ldb, vb.w, dperm_.as<int>(), n, k, live, s);
@@ -1790,6 +1810,7 @@ void Engine<T>::check_apply(hipStream_t s) {
   }
   if (dtrace_.p) {
     // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each
+    const std::vector<dev::DfTask>& df_host_ = df_[df_last_].host;
     std::vector<unsigned long long> tr(4 * df_host_.size());
     RSC_HIP(hipMemcpy(tr.data(), dtrace_.p, tr.size() * sizeof(unsigned long long),
                       hipMemcpyDeviceToHost));

@@ -418,7 +418,8 @@ void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out) {
 // waits on the parent's last C group (nothing for a root) and on the node's
 // own chain B.  The scatter waits on every node's last C group.  Tasks are
 // emitted level by level, so the list is in topological order.
-void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out) {
+void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out,
+                          bool use_subtrees) {
   const int ns = (int)P.nodes.size();
   const int VCH = kSolveChunk, SCH = kScatterChunk;
   std::vector<dev::GvTask> gv;
@@ -474,7 +475,10 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   auto chunks = [&](int j) { return (P.nodes[j].ncols + P.nodes[j].R() + VCH - 1) / VCH; };
   std::vector<NodeStages> st(ns);
   for (int j = 0; j < ns; ++j) st[j] = node_stages(P, L, j);
-  const Subtrees& SB = L.sub;
+  Subtrees none;
+  none.of.assign(ns, -1);
+  none.ptr.assign(1, 0);
+  const Subtrees& SB = use_subtrees ? L.sub : none;
   const int nsub = (int)SB.root.size();
   std::vector<std::vector<int>> lnodes(P.levels.size());
   for (size_t lv = 0; lv < P.levels.size(); ++lv)

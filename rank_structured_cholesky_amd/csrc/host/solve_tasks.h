@@ -125,6 +125,10 @@ struct DataflowTasks {
 // Vector layout; subtree_elems = 0 disables subtree tasks.
 void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems);
 void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
-void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out);
+// use_subtrees = false: every supernode as ordinary tasks (the list of the
+// multi-right-hand-side apply, whose matrix-core bodies do better without
+// subtree tasks)
+void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out,
+                          bool use_subtrees = true);
 
 }  // namespace rsc

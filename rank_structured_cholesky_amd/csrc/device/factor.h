@@ -95,8 +95,8 @@ class FactorEngine {
   // X = M^{-1} B (device pointers).  live: optional device flag; when it
   // reads 0 at run time the apply does nothing (converged PCG columns).
   // multi_min_k: matrix-vector tasks use the sixteen-right-hand-side matrix
-  // core bodies when k >= multi_min_k (1: always, so a column's result does
-  // not depend on how many others share the apply)
+  // core bodies for k > 1, and for k = 1 when multi_min_k <= 1 (so a
+  // column's result does not depend on how many others share the apply)
   virtual void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
                      const int* live = nullptr, int multi_min_k = 2) = 0;
   // Synchronizes s and throws DeviceError if an apply on it failed.

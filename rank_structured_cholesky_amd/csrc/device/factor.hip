@@ -104,7 +104,7 @@ inline bool inside_subtree(const Supernode& s, int q, int sub) {
 
 // One dataflow task list on the device.
 struct DfList {
-  DBuf tasks, gv, gwoff, gw, need, cnt;
+  DBuf tasks, gv, gw, cnt;
   int ntasks = 0, ngroups = 0, grid = 1;
   std::vector<dev::DfTask> host;  // kept for the optional timing trace
 };
@@ -179,7 +179,7 @@ class Engine final : public FactorEngine {
   DBuf factor_, work_, dAx_, dlowp_, da_dst_, da_src_, dfronts_, dchildren_,
       drelmap_, dtasks_, dgjobs_, dgtiles_, dpjobs_, dtjobs_, dttiles_, dcjobs_,
       dctiles_, dsjobs_, dstatus_, dejobs_, detiles_, dkjobs_, dvjobs_;
-  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dfv_, dtv_, dw_, dz_, dranges_;
+  DBuf dvnodes_, dgv_, drows_, dlevnodes_, dperm_, dpinv_, dfv_, dtv_, dw_, dz_, dranges_;
   std::vector<std::vector<SolvePhase>> fwd_, bwd_;
   std::vector<std::vector<int>> level_nodes_;  // per level, nodes outside subtrees
   int64_t tv_stride_ = 0, ntasks_ = 0, part_stride_ = 1;
@@ -1316,9 +1316,9 @@ void Engine<T>::finish_solve_setup() {
     D.ngroups = (int)DT.need.size();
     D.tasks.upload(DT.tasks, st_);
     D.gv.upload(DT.gv, st_);
-    D.gwoff.upload(DT.gwoff, st_);
-    D.gw.upload(DT.gw, st_);
-    D.need.upload(DT.need, st_);
+    std::vector<dev::DfWait> waits(DT.gw.size());
+    for (size_t q = 0; q < DT.gw.size(); ++q) waits[q] = {DT.gw[q], DT.need[DT.gw[q]]};
+    D.gw.upload(waits, st_);
     D.cnt.alloc((size_t)(D.ngroups + 1) * sizeof(unsigned));
     D.grid = std::max(1, std::min(D.ntasks, df_resident_));
     if (!df_trace_path_.empty()) D.host = DT.tasks;
@@ -1387,6 +1387,8 @@ void Engine<T>::setup_static() {
   drows_.upload(SL.rows, st_);
   std::vector<int> permv(P.perm);
   dperm_.upload(permv, st_);
+  std::vector<int> pinvv(P.pinv);
+  dpinv_.upload(pinvv, st_);
   fv_off_ = SL.fv_off;
   fv_stride_ = SL.fv_stride;
   tv_stride_ = SL.tv_stride;
@@ -1398,17 +1400,39 @@ void Engine<T>::setup_static() {
     use_dataflow_ = !(lv && lv[0] == '1');
   }
   if (use_dataflow_) {
-    // the dataflow task list is built on a host thread while the device
-    // factors; finish_solve_setup() joins it before the first apply
-    df_thread_ = std::thread([this]() {
+    int dev = 0, cus = 1;
+    RSC_HIP(hipGetDevice(&dev));
+    RSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
+    // ticket order: a list schedule on the resident workgroups (costs for one
+    // and for sixteen right-hand sides); RSC_TICKET_ORDER=0 keeps level order
+    TicketModel one, many;
+    const char* to = std::getenv("RSC_TICKET_ORDER");
+    const int workers = (to && to[0] == '0') ? 0 : df_resident_;
+    one.workers = many.workers = workers;
+    one.elem_bytes = many.elem_bytes = (int)sizeof(T);
+    one.nrhs = 1;
+    many.nrhs = 16;
+    // the dataflow task lists are built on host threads while the device
+    // factors; finish_solve_setup() joins them before the first apply
+    df_thread_ = std::thread([this, one, many]() {
+      std::exception_ptr err2;
+      std::thread second([&]() {
+        try {
+          df_pending2_.reset(new DataflowTasks());
+          build_dataflow_tasks(plan_, sl_, *df_pending2_, false, many);
+        } catch (...) {
+          err2 = std::current_exception();
+        }
+      });
       try {
         df_pending_.reset(new DataflowTasks());
-        build_dataflow_tasks(plan_, sl_, *df_pending_);
-        df_pending2_.reset(new DataflowTasks());
-        build_dataflow_tasks(plan_, sl_, *df_pending2_, false);
+        build_dataflow_tasks(plan_, sl_, *df_pending_, true, one);
       } catch (...) {
         df_err_ = std::current_exception();
       }
+      second.join();
+      if (!df_err_ && err2) df_err_ = err2;
     });
   } else {
     LevelTasks LT;
@@ -1435,7 +1459,7 @@ void Engine<T>::setup_static() {
     // RSC_DEVICE_ERROR)
     if (const char* e = std::getenv("RSC_DF_TIMEOUT_US"))
       df_timeout_ = (unsigned long long)khz * (unsigned long long)std::atoll(e) / 1000ull;
-    df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
+    if (!use_dataflow_) df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
     std::vector<int> z = {0, 0};
     dapst_.upload(z, st_);
   }
@@ -1741,15 +1765,13 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
                   live, dz_.as<double>(), multi_min_k};
   if (use_dataflow_) {
     dev::DfArgs a{};
-    df_last_ = k >= multi_min_k ? 1 : 0;
+    df_last_ = (k > 1 || multi_min_k <= 1) ? 1 : 0;
     DfList& D = df_[df_last_];
     a.tasks = D.tasks.as<dev::DfTask>();
     a.ntasks = D.ntasks;
     a.ngroups = D.ngroups;
     a.gv = D.gv.as<dev::GvTask>();
-    a.gwait_off = D.gwoff.as<int>();
-    a.gwait = D.gw.as<int>();
-    a.need = D.need.as<unsigned>();
+    a.gwait = D.gw.as<dev::DfWait>();
     a.cnt = D.cnt.as<unsigned>();
     a.status = dapst_.as<int>();
     a.timeout_ticks = df_timeout_;
@@ -1759,6 +1781,7 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     a.ranges = dranges_.as<int>();
     a.rows = drows_.as<int>();
     a.perm = dperm_.as<int>();
+    a.pinv = dpinv_.as<int>();
     a.B = B;
     a.ldb = ldb;
     a.X = X;

@@ -26,8 +26,8 @@ struct VecBufs {
   int64_t n, fvs, tvs, ps;
   const int* live;  // when non-null and *live == 0 every kernel of the apply returns
   double* z;        // z = L^{-T} y per supernode, stride n per column
-  // dataflow matrix-vector tasks use the matrix-core bodies (sixteen
-  // right-hand sides per matrix pass) when k >= multi_min_k
+  // the dataflow apply runs the matrix-core variant (sixteen right-hand
+  // sides per matrix pass) for k > 1, and for k = 1 when multi_min_k <= 1
   int multi_min_k = 2;
 };
 
@@ -64,9 +64,7 @@ struct DfArgs {
   const DfTask* tasks;
   int ntasks, ngroups;
   const GvTask* gv;
-  const int* gwait_off;  // ngroups + 1
-  const int* gwait;      // groups waited on, per group
-  const unsigned* need;  // tasks per group
+  const DfWait* gwait;   // wait list (DfTask::wbeg / wend index it)
   unsigned* cnt;         // ngroups completion counters, then the ticket counter
   int* status;           // [0] = 1 after a wait timed out, [1] = abort request
   unsigned long long timeout_ticks;  // wall-clock ticks before a wait gives up
@@ -76,6 +74,7 @@ struct DfArgs {
   const int* ranges;  // child row ranges per assembly chunk (VecNode::range_off)
   const int* rows;
   const int* perm;
+  const int* pinv;  // inverse permutation: x[j] = w[pinv[j]] (final scatter)
   const double* B;
   int64_t ldb;
   double* X;

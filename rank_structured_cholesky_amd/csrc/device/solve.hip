@@ -71,11 +71,18 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Every field is read unconditionally and the choice made by selects on the
+// values: a choice between the fields' addresses would keep the argument
+// block in private (scratch) memory.
 __device__ inline double* vsel(const VecBufs& b, int sel, int col) {
-  if (sel == 0) return b.w + (int64_t)col * b.n;
-  if (sel == 1) return b.fv + (int64_t)col * b.fvs;
-  if (sel == 2) return b.tv + (int64_t)col * b.tvs;
-  return b.z + (int64_t)col * b.n;
+  double* const w = b.w;
+  double* const fv = b.fv;
+  double* const tv = b.tv;
+  double* const z = b.z;
+  const int64_t n = b.n, fvs = b.fvs, tvs = b.tvs;
+  double* const base = sel == 0 ? w : (sel == 1 ? fv : (sel == 2 ? tv : z));
+  const int64_t st = sel == 1 ? fvs : (sel == 2 ? tvs : n);
+  return base + (int64_t)col * st;
 }
 
 // Task descriptors are the same for every lane of a workgroup; copying them
@@ -613,9 +620,27 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       const bool rok = r < t.rows;
       const bool wact = wave * 16 < t.rows;
       const T* __restrict__ mrow = M + (rok ? r : 0);
-      d4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+      d4v acc = {0, 0, 0, 0};
+      // columns jb + 4u + kq of this lane's row, u < 8 (a quarter chunk);
+      // the lane's part of the address is fixed, the column offset uniform
+      const T* __restrict__ mlane = mrow + (int64_t)kq * t.ld;
+      auto load_q = [&](int jb, int nb, double (&m)[8]) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          m[u] = (rok && kq < nb - 4 * u) ? (double)mlane[(int64_t)(jb + 4 * u) * t.ld] : 0.0;
+      };
+      // one accumulator, columns in increasing groups of four
+      auto mul_q = [&](const double (&m)[8], const double* xr) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = mfma64(m[u], xr[4 * u], acc);
+      };
       for (int jb = 0; jb < t.cols; jb += kRowChunk) {
         const int nb = min(kRowChunk, t.cols - jb);
+        // the chunk's matrix block is loaded a quarter ahead: the first
+        // quarter while x is staged, each next one while the previous is
+        // multiplied
+        double ma[8], mb[8];
+        if (wact) load_q(jb, nb, ma);
         for (int idx = tid; idx < kMfmaRhs * kRowChunk; idx += 256) {
           const int cc = idx / kRowChunk, jj = idx - cc * kRowChunk;
           sm[cc * LD + jj] = (cc < kb && jj < nb)
@@ -625,14 +650,13 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
         __syncthreads();
         if (wact) {
           const double* xr = sm + li * LD + kq;
-#pragma unroll 2
-          for (int j = 0; j < nb; j += 8) {
-            const int ja = j + kq, jc = j + 4 + kq;
-            const double a0 = (rok && ja < nb) ? (double)mrow[(int64_t)(jb + ja) * t.ld] : 0.0;
-            const double a1 = (rok && jc < nb) ? (double)mrow[(int64_t)(jb + jc) * t.ld] : 0.0;
-            acc0 = mfma64(a0, xr[j], acc0);
-            acc1 = mfma64(a1, xr[j + 4], acc1);
-          }
+          load_q(jb + 32, nb - 32, mb);
+          mul_q(ma, xr);
+          load_q(jb + 64, nb - 64, ma);
+          mul_q(mb, xr + 32);
+          load_q(jb + 96, nb - 96, mb);
+          mul_q(ma, xr + 64);
+          mul_q(mb, xr + 96);
         }
         __syncthreads();
       }
@@ -641,7 +665,7 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
         for (int v = 0; v < 4; ++v) {
           const int row = wave * 16 + kq + 4 * v;
           if (row < t.rows && li < kb) {
-            const double val = acc0[v] + acc1[v];
+            const double val = acc[v];
             if (t.part >= 0) {
               vst<COH>(b.part + (int64_t)(c0 + li) * b.ps +
                            (int64_t)(t.part + (row >> 5)) * 32 + (row & 31), val);
@@ -717,86 +741,27 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
     else gemv_multi<T, COH, 4>(F, t, b, k, red, xl);
     return;
   }
-  const T* __restrict__ M = F + t.mat;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int col = 0; col < k; ++col) {
-    double* y = vsel(b, t.ysel, col) + t.yoff;
-    if (t.kind == 2) {
-      const double* pp = b.part + (int64_t)col * b.ps + (int64_t)t.part * 32;
-      const int64_t rs = 32 * (int64_t)(t.ld > 0 ? t.ld : 1);
-      if (tid < t.cols) {
-        double acc = 0;
-        for (int s2 = 0; s2 < t.rows; ++s2) acc += vld<COH>(pp + (int64_t)s2 * rs + tid);
-        vst<COH>(y + tid, t.op ? vld<COH>(y + tid) - acc : acc);
-      }
-      continue;
+  const int tid = threadIdx.x;
+  // reduce: y[j] (op)= sum_s part[s][j] for every (right-hand side, j)
+  // pair at once, eight partial loads in flight per item; the sum runs
+  // over s in order, so the result does not depend on k
+  const int64_t rs = 32 * (int64_t)(t.ld > 0 ? t.ld : 1);
+  const int items = k * t.cols;
+  for (int it = tid; it < items; it += 256) {
+    const int col = it / t.cols, j = it - col * t.cols;
+    const double* pp = b.part + (int64_t)col * b.ps + (int64_t)t.part * 32 + j;
+    double acc = 0;
+    int s2 = 0;
+    for (; s2 + 4 <= t.rows; s2 += 4) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = vld<COH>(pp + (int64_t)(s2 + u) * rs);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += v[u];
     }
-    const double* x = vsel(b, t.xsel, col) + t.xoff;
-    if (t.kind == 0) {
-      // x staged in LDS (coalesced, once per task); 16 matrix loads per
-      // lane in flight per batch
-      const bool act = lane < t.rows;
-      const T* rp = M + (act ? lane : 0);
-      const int64_t ld4 = 4 * (int64_t)t.ld;
-      double acc0 = 0, acc1 = 0;
-      for (int jb = 0; jb < t.cols; jb += kXlds) {
-        const int nb = min(kXlds, t.cols - jb);
-        for (int i = tid; i < nb; i += 256) xl[i] = vld<COH>(x + jb + i);
-        __syncthreads();
-        int j = wave;
-        for (; j + 60 < nb; j += 64) {
-          const T* p = rp + (int64_t)(jb + j) * t.ld;
-          double m[16];
-#pragma unroll
-          for (int u = 0; u < 16; ++u) m[u] = (double)p[u * ld4];
-#pragma unroll
-          for (int u = 0; u < 16; u += 2) {
-            acc0 += m[u] * xl[j + 4 * u];
-            acc1 += m[u + 1] * xl[j + 4 * u + 4];
-          }
-        }
-        for (; j < nb; j += 4) acc0 += (double)rp[(int64_t)(jb + j) * t.ld] * xl[j];
-        __syncthreads();
-      }
-      red[wave][lane] = act ? acc0 + acc1 : 0.0;
-      __syncthreads();
-      if (wave == 0 && act) {
-        const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-        vst<COH>(y + lane, t.op ? vld<COH>(y + lane) - v : v);
-      }
-      __syncthreads();
-    } else {
-      const int q0 = wave * 8;
-      const int nq = min(8, t.cols - q0);
-      double acc[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc[u] = 0;
-      if (nq > 0) {
-        const T* cbase = M + (int64_t)q0 * t.ld;
-        for (int i = lane; i < t.rows; i += 128) {
-          const int i2 = i + 64;
-          const bool ok2 = i2 < t.rows;
-          const double x1 = vld<COH>(x + i), x2 = ok2 ? vld<COH>(x + i2) : 0.0;
-          double m1[8], m2[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
-            m1[u] = (double)cp[i];
-            m2[u] = ok2 ? (double)cp[i2] : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) acc[u] += m1[u] * x1 + m2[u] * x2;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const double v = wave_sum(acc[u]);
-        if (lane == 0 && u < nq) {
-          if (t.part >= 0) vst<COH>(b.part + (int64_t)col * b.ps + (int64_t)t.part * 32 + q0 + u, v);
-          else vst<COH>(y + q0 + u, t.op ? vld<COH>(y + q0 + u) - v : v);
-        }
-      }
-    }
+    for (; s2 < t.rows; ++s2) acc += vld<COH>(pp + (int64_t)s2 * rs);
+    double* y = vsel(b, t.ysel, col) + t.yoff + j;
+    vst<COH>(y, t.op ? vld<COH>(y) - acc : acc);
   }
 }
 
@@ -986,13 +951,13 @@ template void launch_subtrees<double>(const double*, const SubtreeArgs&, int, bo
 template void launch_subtrees<float>(const float*, const SubtreeArgs&, int, bool, VecBufs, int,
                                      hipStream_t);
 
-constexpr int DF_SCATTER_CHUNK = 4096;
+constexpr int DF_SCATTER_CHUNK = 4096;  // x entries per scatter task (16 per thread)
+static_assert(DF_SCATTER_CHUNK % (8 * 256) == 0, "scatter chunk: eight entries per thread per pass");
 
 // Thread 0 waits until group g has completed need tasks.  Returns false when
 // the wait gave up (timeout here or in another workgroup): the kernel then
 // runs to its end without further waiting and the host reports the failure.
-__device__ bool df_wait(const DfArgs& a, int g) {
-  const unsigned need = a.need[g];
+__device__ bool df_wait(const DfArgs& a, int g, unsigned need) {
   if (__hip_atomic_load(a.cnt + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need)
     return true;
   const unsigned long long t0 = wall_clock64();
@@ -1014,10 +979,16 @@ __device__ bool df_wait(const DfArgs& a, int g) {
 // One tid == 0 region per loop iteration, with a barrier on both sides: the
 // completion signal and the next ticket are taken together, so the compiler
 // cannot merge a tid == 0 region of one iteration with one of the next (a
-// merged region lets the other lanes run the loop without thread 0).
+// merged region lets the other lanes run the loop without thread 0).  A task's
+// descriptors (task entry, its first wait inline, and the matrix-vector entry
+// at the same index) depend only on the ticket, so they arrive together.
+// MULTI = false is the single-right-hand-side kernel: k is 1 by
+// construction (the launcher sends every k > 1 to the other variant), and
+// the constant lets the compiler drop the multi-column code paths.
 template <typename T, bool MULTI>
 __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, DfArgs a,
-                                                  VecBufs b, int k) {
+                                                  VecBufs b, int k_arg) {
+  const int k = MULTI ? k_arg : 1;
   __shared__ double red[4][64];
   __shared__ union {
     SubtreeLds sl;
@@ -1040,13 +1011,19 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     const int ticket = uni(s_ticket);
     if (ticket >= a.ntasks) break;
     const DfTask& tk = a.tasks[ticket];
+    const GvTask gt = uniform_task(a.gv[ticket]);
     const int type = uni(tk.type), ta = uni(tk.a), tb = uni(tk.b);
     const int group = uni(tk.group), group2 = uni(tk.group2);
-    const int w0 = uni(a.gwait_off[group]), w1 = uni(a.gwait_off[group + 1]);
+    const int w0 = uni(tk.wbeg), w1 = uni(tk.wend);
     if (w1 > w0) {
+      const int wg0 = uni(tk.wg0);
+      const unsigned wn0 = (unsigned)uni((int)tk.wn0);
       if (tid == 0) {
-        bool ok = s_ok != 0;
-        for (int q = w0; ok && q < w1; ++q) ok = df_wait(a, a.gwait[q]);
+        bool ok = s_ok != 0 && df_wait(a, wg0, wn0);
+        for (int q = w0 + 1; ok && q < w1; ++q) {
+          const DfWait w = a.gwait[q];
+          ok = df_wait(a, w.group, w.need);
+        }
         s_ok = ok ? 1 : 0;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       }
@@ -1063,17 +1040,33 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     } else if (type == 1) {
       gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
-      gemv_lite<T, true, MULTI>(F, a.gv[ta], b, k, red, lds.mf);
+      gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.mf);
     } else if (type == 4) {
       subtree_forward<T, true>(F, a.sub, ta, b, k, lds.sl);
     } else if (type == 5) {
       subtree_backward<T, true>(F, a.sub, ta, b, k, lds.sl);
     } else {
-      const int i0 = ta * DF_SCATTER_CHUNK;
-      const int i1 = min(i0 + DF_SCATTER_CHUNK, (int)b.n);
-      for (int col = 0; col < k; ++col)
-        for (int i = i0 + tid; i < i1; i += 256)
-          a.X[a.perm[i] + col * a.ldx] = vld<true>(b.w + i + col * b.n);
+      // x = P^T w over a chunk of x in its own order, x[j] = w[pinv[j]]: the
+      // stores are coalesced (a scattered 8-byte store costs a memory line);
+      // eight source indices of a thread, then per right-hand side their
+      // eight entries, are loaded together
+      const int n = (int)b.n;
+      for (int h = 0; h < DF_SCATTER_CHUNK; h += 8 * 256) {
+        const int j0 = ta * DF_SCATTER_CHUNK + h + tid;
+        int src[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) src[u] = j0 + 256 * u < n ? a.pinv[j0 + 256 * u] : 0;
+        for (int col = 0; col < k; ++col) {
+          const double* wc = b.w + (int64_t)col * b.n;
+          double* xc = a.X + (int64_t)col * a.ldx + j0;
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = vld<true>(wc + src[u]);
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (j0 + 256 * u < n) xc[256 * u] = v[u];
+        }
+      }
     }
     // every thread: its device-scope stores have completed before the count
     __builtin_amdgcn_s_waitcnt(0);
@@ -1114,8 +1107,8 @@ void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, i
   if (a.ntasks <= 0) return;
   // the matrix-core variant is a kernel of its own, so its registers do not
   // weigh on the single-right-hand-side kernel
-  if (k >= b.multi_min_k) k_apply_df<T, true><<<grid, 256, 0, s>>>(factor, a, b, k);
-  else k_apply_df<T, false><<<grid, 256, 0, s>>>(factor, a, b, k);
+  if (k > 1 || b.multi_min_k <= 1) k_apply_df<T, true><<<grid, 256, 0, s>>>(factor, a, b, k);
+  else k_apply_df<T, false><<<grid, 256, 0, s>>>(factor, a, b, 1);
 }
 
 template <typename T>

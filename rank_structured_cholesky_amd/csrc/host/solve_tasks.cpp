@@ -4,7 +4,10 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <queue>
 #include <stdexcept>
+#include <utility>
 
 namespace rsc {
 
@@ -418,8 +421,170 @@ void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out) {
 // waits on the parent's last C group (nothing for a root) and on the node's
 // own chain B.  The scatter waits on every node's last C group.  Tasks are
 // emitted level by level, so the list is in topological order.
+namespace {
+
+// Estimated duration of one task (ns) on one of many busy workgroups, fitted
+// to per-task timings of the dataflow kernel on MI355X (64^3 Poisson, one
+// right-hand side, scalar bodies; 128^3, sixteen, matrix-core bodies): a fixed
+// part (ticket, waits, barriers, descriptor and first-data round trips, larger
+// for the matrix-core bodies) plus terms in the matrix bytes, the 64-row /
+// 128-column blocks of a matrix-vector task, the vector entries moved and the
+// children or supernodes visited.
+struct CostModel {
+  const Plan& P;
+  const SolveLayout& L;
+  const Subtrees& SB;
+  const std::vector<dev::GvTask>& gv;
+  const TicketModel& m;
+  double us(const dev::DfTask& t) const {
+    const bool multi = m.nrhs > 1;
+    const double k = std::max(1, m.nrhs), e = m.elem_bytes;
+    const double passes = multi ? std::ceil(k / 16.0) : 1.0;
+    switch (t.type) {
+      case 0: {
+        const dev::VecNode& v = L.vn[t.a];
+        double ent = std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)t.b * kSolveChunk);
+        for (int q = 0; q < v.nchild; ++q) {
+          const int64_t r = 2 * ((int64_t)v.range_off + (int64_t)t.b * v.nchild + q);
+          ent += L.asm_ranges[r + 1] - L.asm_ranges[r];
+        }
+        return multi ? 5.5 + 2.7e-3 * ent * k + 8.2 * v.nchild
+                     : 0.8 + 4.1e-3 * ent * k + 1.5 * v.nchild;
+      }
+      case 1: {
+        const dev::VecNode& v = L.vn[t.a];
+        const double ent =
+            std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)t.b * kSolveChunk);
+        return multi ? 5.4 + 4.5e-3 * ent * k : 2.7 + 2.5e-3 * ent * k;
+      }
+      case 2: {
+        const dev::GvTask& g = gv[t.a];
+        if (g.kind == 2) return multi ? 9.0 : 3.4;
+        const double kb = (double)g.rows * g.cols * e / 1000.0;
+        if (g.kind == 0)
+          return multi ? 11.2 + 0.141 * kb * passes + 2.0 * std::ceil(g.cols / 128.0) * passes
+                       : 6.6 + 0.07 * kb + 1.0 * std::ceil(g.cols / 128.0);
+        return multi ? 10.5 + 0.198 * kb * passes + 3.1 * std::ceil(g.rows / 64.0) * passes
+                     : 5.4 + 0.074 * kb + 0.45 * std::ceil(g.rows / 64.0);
+      }
+      case 3:
+        return multi ? 12.5 * k : 10.0;
+      default: {
+        double kb = 0, nodes = 0;
+        for (int q = SB.ptr[t.a]; q < SB.ptr[t.a + 1]; ++q) {
+          const Supernode& s = P.nodes[SB.nodes[q]];
+          kb += (double)s.ncols * (s.ncols + s.R()) * e / 1000.0;
+          nodes += 1;
+        }
+        const double sp = std::ceil(k / kSubtreeRhs);
+        return t.type == 4 ? 10.9 + (1.35 * kb + 10.7 * nodes) * sp
+                           : 3.8 + (0.8 * kb + 4.3 * nodes) * sp;
+      }
+    }
+  }
+  int64_t ns(const dev::DfTask& t) const { return std::max<int64_t>(1, (int64_t)(1000.0 * us(t))); }
+};
+
+// Ticket order by list scheduling.  Workgroups take tickets in list order,
+// so a ticket whose task is not ready holds a workgroup idle; in level
+// order, the long chains of HODLR products that only the backward pass needs
+// fill the ticket window ahead of the next level's critical assembly.  The
+// list is replayed on `workers` simulated workgroups with the estimated
+// costs: among the ready tasks the one with the longest estimated path to
+// the end goes first, and the tickets follow the simulated start order.  A
+// task starts only after every group it waits on has completed, so the
+// order stays topological (every wait ends).
+double order_by_list_schedule(std::vector<dev::DfTask>& tasks, const std::vector<unsigned>& need,
+                              const std::vector<int>& gwoff, const std::vector<int>& gw,
+                              const CostModel& cm, int workers) {
+  const int ng = (int)need.size();
+  const int nt = (int)tasks.size();
+  std::vector<int64_t> cost(nt);
+  for (int i = 0; i < nt; ++i) cost[i] = cm.ns(tasks[i]);
+  // tasks per primary group, in list order
+  std::vector<int> tptr(ng + 1, 0), tlist(nt);
+  for (const auto& t : tasks) ++tptr[t.group + 1];
+  for (int g = 0; g < ng; ++g) tptr[g + 1] += tptr[g];
+  {
+    std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+    for (int i = 0; i < nt; ++i) tlist[fill[tasks[i].group]++] = i;
+  }
+  // groups waiting on each group (one entry per wait, counting sort)
+  std::vector<int> wptr(ng + 1, 0), wlist(gw.size());
+  for (int q : gw) ++wptr[q + 1];
+  for (int g = 0; g < ng; ++g) wptr[g + 1] += wptr[g];
+  {
+    std::vector<int> fill(wptr.begin(), wptr.end() - 1);
+    for (int h = 0; h < ng; ++h)
+      for (int q = gwoff[h]; q < gwoff[h + 1]; ++q) wlist[fill[gw[q]]++] = h;
+  }
+  // second group of a group's tasks (the same for all of them), -1 if none
+  std::vector<int> g2(ng, -1);
+  for (const auto& t : tasks)
+    if (t.group2 >= 0) g2[t.group] = t.group2;
+  // bottom levels: every group waits only on groups created before it, and
+  // second groups are created after the groups that feed them
+  std::vector<int64_t> bl(ng, 0);
+  for (int g = ng - 1; g >= 0; --g) {
+    int64_t c = 0, below = g2[g] >= 0 ? bl[g2[g]] : 0;
+    for (int q = tptr[g]; q < tptr[g + 1]; ++q) c = std::max(c, cost[tlist[q]]);
+    for (int q = wptr[g]; q < wptr[g + 1]; ++q) below = std::max(below, bl[wlist[q]]);
+    bl[g] = c + below;
+  }
+  // waits outstanding per group; a group without tasks of its own (the
+  // second group) completes when its count of signalling tasks is reached
+  std::vector<int> pending(ng);
+  for (int h = 0; h < ng; ++h) pending[h] = gwoff[h + 1] - gwoff[h];
+  std::vector<unsigned> done(ng, 0u);
+  using Ready = std::pair<int64_t, int>;  // (-bottom level, group), smallest first
+  std::priority_queue<Ready, std::vector<Ready>, std::greater<Ready>> ready;
+  std::vector<int> cursor(tptr.begin(), tptr.end() - 1);
+  for (int g = 0; g < ng; ++g)
+    if (pending[g] == 0 && tptr[g + 1] > tptr[g]) ready.push({-bl[g], g});
+  using Run = std::pair<int64_t, int>;  // (finish time, task)
+  std::priority_queue<Run, std::vector<Run>, std::greater<Run>> running;
+  std::vector<int> order;
+  order.reserve(nt);
+  int64_t now = 0;
+  int free_w = std::max(1, workers);
+  auto finish = [&](int g) {
+    if (++done[g] != need[g]) return;
+    for (int q = wptr[g]; q < wptr[g + 1]; ++q) {
+      const int h = wlist[q];
+      if (--pending[h] == 0 && tptr[h + 1] > tptr[h]) ready.push({-bl[h], h});
+    }
+  };
+  while ((int)order.size() < nt) {
+    while (free_w > 0 && !ready.empty()) {
+      const int g = ready.top().second;
+      const int i = tlist[cursor[g]++];
+      if (cursor[g] == tptr[g + 1]) ready.pop();
+      order.push_back(i);
+      running.push({now + cost[i], i});
+      --free_w;
+    }
+    if (running.empty()) throw std::logic_error("dataflow apply: task graph has a cycle");
+    const Run r = running.top();
+    running.pop();
+    now = r.first;
+    ++free_w;
+    finish(tasks[r.second].group);
+    if (tasks[r.second].group2 >= 0) finish(tasks[r.second].group2);
+  }
+  while (!running.empty()) {
+    now = std::max(now, running.top().first);
+    running.pop();
+  }
+  std::vector<dev::DfTask> out(nt);
+  for (int q = 0; q < nt; ++q) out[q] = tasks[order[q]];
+  tasks.swap(out);
+  return now / 1000.0;
+}
+
+}  // namespace
+
 void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out,
-                          bool use_subtrees) {
+                          bool use_subtrees, const TicketModel& tm) {
   const int ns = (int)P.nodes.size();
   const int VCH = kSolveChunk, SCH = kScatterChunk;
   std::vector<dev::GvTask> gv;
@@ -443,8 +608,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     t.b = b;
     t.group = g;
     t.group2 = -1;
-    t.node = cur_node;
-    t.stage = cur_stage;
+    t.tag = ((cur_node + 1) << 8) | std::min(cur_stage, 255);
     tasks.push_back(t);
     need[g] += 1u;
   };
@@ -569,6 +733,26 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   cur_node = -1;
   cur_stage = 2;
   for (int c = 0; c * SCH < P.n; ++c) add(3, c, 0, gsc);
+  out.est_span_us = 0;
+  if (tm.workers > 0)
+    out.est_span_us =
+        order_by_list_schedule(tasks, need, gwoff, gw, CostModel{P, L, SB, gv, tm}, tm.workers);
+  for (auto& t : tasks) {
+    t.wbeg = gwoff[t.group];
+    t.wend = gwoff[t.group + 1];
+    t.wg0 = t.wend > t.wbeg ? gw[t.wbeg] : -1;
+    t.wn0 = t.wg0 >= 0 ? need[t.wg0] : 0u;
+  }
+  {
+    // matrix-vector descriptors parallel to the task list
+    std::vector<dev::GvTask> gvt(tasks.size(), dev::GvTask{});
+    for (size_t i = 0; i < tasks.size(); ++i)
+      if (tasks[i].type == 2) {
+        gvt[i] = gv[tasks[i].a];
+        tasks[i].a = (int)i;
+      }
+    gv.swap(gvt);
+  }
   // every group waited on must complete before the waiting task's ticket
   std::vector<int> last_task(need.size(), -1);
   for (size_t i = 0; i < tasks.size(); ++i) {

@@ -53,15 +53,27 @@ struct VecNode {
 // group on its wait list has completed all of its tasks.  A workgroup only
 // ever waits on groups whose tasks hold smaller tickets, which are already
 // running or done, so the launch drains whatever the occupancy.
+// Task t of the list is matrix-vector task gv[t] when its type is 2 (the
+// GvTask array runs parallel to the task list), so a workgroup fetches both
+// descriptors of its next task at once.
 struct DfTask {
   int type;    // 0 forward assembly chunk, 1 backward gather chunk, 2 matrix-vector
                // task, 3 scatter chunk, 4 forward subtree, 5 backward subtree
-  int a, b;    // type 0/1: node, chunk; type 2: GvTask index; type 3: chunk; 4/5: subtree
+  int a, b;    // type 0/1: node, chunk; type 2: own index; type 3: chunk; 4/5: subtree
   int group;   // group signalled on completion
   int group2;  // second group signalled (-1: none)
-  int node;    // supernode the task belongs to (-1: scatter)
-  int stage;   // 0 assembly, 1 gather, 2 scatter, 10+k chain A, 20+k chain B, 30+k chain C
-  int pad;
+  int wbeg, wend;  // the group's waits: entries [wbeg, wend) of the wait list
+  int wg0;         // first wait, inline: group (-1: none)
+  unsigned wn0;    // and the task count that completes it
+  int tag;     // (supernode + 1) << 8 | stage, for traces (supernode -1: scatter;
+               // stage 0 assembly, 1 gather, 2 scatter, 3 / 4 subtree forward /
+               // backward, 10+k chain A, 20+k chain B, 30+k chain C)
+};
+
+// One entry of the wait list: a group and the task count that completes it.
+struct DfWait {
+  int group;
+  unsigned need;
 };
 
 }  // namespace dev
@@ -116,19 +128,29 @@ struct LevelTasks {
 
 struct DataflowTasks {
   std::vector<dev::DfTask> tasks;  // topological order
-  std::vector<dev::GvTask> gv;
+  std::vector<dev::GvTask> gv;     // one per task (used for type 2)
   std::vector<unsigned> need;      // tasks per group
   std::vector<int> gwoff, gw;      // groups waited on, per group
   int64_t slots = 0;
+  double est_span_us = 0;          // makespan of the list schedule (0: level order)
 };
 
 // Vector layout; subtree_elems = 0 disables subtree tasks.
 void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems);
 void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
+// Ticket order of a dataflow list.  workers = 0 keeps the level order;
+// otherwise the list is ordered by a list schedule on that many workgroups
+// (resident workgroups of the launch), with task costs estimated for nrhs
+// right-hand sides and a factor of elem_bytes per element.
+struct TicketModel {
+  int workers = 0;
+  int nrhs = 1;
+  int elem_bytes = 8;
+};
 // use_subtrees = false: every supernode as ordinary tasks (the list of the
 // multi-right-hand-side apply, whose matrix-core bodies do better without
 // subtree tasks)
 void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out,
-                          bool use_subtrees = true);
+                          bool use_subtrees = true, const TicketModel& tm = TicketModel());
 
 }  // namespace rsc

@@ -13,7 +13,9 @@
 // dependency or out-of-range index shows up.
 //
 //   emulate_apply <problem> <N> <max_dense_block> <max_diagonal_block> <nrhs> <threads>
-//                 [subtree_elems]
+//                 [subtree_elems [workers]]
+//   workers: ticket order by a list schedule on that many workgroups
+//   (default 1024; 0 keeps the level order)
 //   problem: p3d (7-point log-normal Poisson), p2d, el (elasticity)
 // Prints "ok ..." and exits 0 on success.
 #include <atomic>
@@ -258,7 +260,8 @@ void run_df_task(State& S, const DataflowTasks& D, const dev::DfTask& t) {
 
 int main(int argc, char** argv) {
   if (argc < 7) {
-    std::fprintf(stderr, "usage: %s p3d|p2d|el N mdb mdiag nrhs threads\n", argv[0]);
+    std::fprintf(stderr, "usage: %s p3d|p2d|el N mdb mdiag nrhs threads [subtree_elems [workers]]\n",
+                 argv[0]);
     return 2;
   }
   const std::string prob = argv[1];
@@ -288,7 +291,13 @@ int main(int argc, char** argv) {
   LevelTasks LT;
   build_level_tasks(P, L, LT);
   DataflowTasks D;
-  build_dataflow_tasks(P, L, D);
+  TicketModel tm;
+  tm.workers = argc > 8 ? std::atoi(argv[8]) : 1024;
+  tm.nrhs = k;
+  const auto tb0 = std::chrono::steady_clock::now();
+  build_dataflow_tasks(P, L, D, true, tm);
+  const double build_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb0).count();
 
   State S;
   S.P = &P;
@@ -397,21 +406,28 @@ int main(int argc, char** argv) {
     for (auto& c : cnt) c.store(0u);
     std::atomic<int> ticket{0};
     std::atomic<bool> stuck{false};
+    auto wait_for = [&](int wg, unsigned need) {
+      const auto t0 = std::chrono::steady_clock::now();
+      while (cnt[wg].load(std::memory_order_acquire) < need) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) return false;
+        std::this_thread::yield();
+      }
+      return true;
+    };
     auto worker = [&]() {
       for (;;) {
         const int i = ticket.fetch_add(1);
         if (i >= (int)D.tasks.size()) return;
         const dev::DfTask& t = D.tasks[i];
-        for (int q = D.gwoff[t.group]; q < D.gwoff[t.group + 1]; ++q) {
-          const int wg = D.gw[q];
-          const auto t0 = std::chrono::steady_clock::now();
-          while (cnt[wg].load(std::memory_order_acquire) < D.need[wg]) {
-            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
-              stuck = true;
-              return;
-            }
-            std::this_thread::yield();
-          }
+        if (t.wbeg != D.gwoff[t.group] || t.wend != D.gwoff[t.group + 1])
+          fail("task wait range differs from its group's");
+        if (t.wend > t.wbeg && (t.wg0 != D.gw[t.wbeg] || t.wn0 != D.need[t.wg0]))
+          fail("inline first wait differs from the wait list");
+        bool ok = t.wend <= t.wbeg || wait_for(t.wg0, t.wn0);
+        for (int q = t.wbeg + 1; ok && q < t.wend; ++q) ok = wait_for(D.gw[q], D.need[D.gw[q]]);
+        if (!ok) {
+          stuck = true;
+          return;
         }
         run_df_task(S, D, t);
         cnt[t.group].fetch_add(1u, std::memory_order_release);
@@ -424,8 +440,10 @@ int main(int argc, char** argv) {
     if (stuck) fail("threaded dataflow run stalled");
   }
   if (!agrees(S.X.v)) fail("threaded dataflow result differs from the level path");
-  std::printf("ok%s n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d\n",
-              exact ? "" : " (split rows)", S.n, (int)P.nodes.size(), nlev, L.sub.root.size(),
-              LT.gv.size(), D.tasks.size(), ng);
+  std::printf(
+      "ok%s n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d "
+      "df_build_ms=%.1f\n",
+      exact ? "" : " (split rows)", S.n, (int)P.nodes.size(), nlev, L.sub.root.size(),
+      LT.gv.size(), D.tasks.size(), ng, build_ms);
   return 0;
 }

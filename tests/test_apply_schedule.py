@@ -2,7 +2,8 @@
 dataflow list run in ticket order and on threads gives the result of the
 level-scheduled list (bitwise, or to rounding when the dataflow list splits
 long rows into partial sums), every wait is satisfied when its task starts,
-and every buffer access is in range; with and without subtree tasks."""
+and every buffer access is in range; with and without subtree tasks, in
+level order and in list-schedule ticket order."""
 import os
 import subprocess
 
@@ -33,6 +34,9 @@ def built():
     "p3d 20 63 50 1 8 100000000",  # subtree tasks as large as allowed
     "p3d 36 2000 400 2 8",   # uncompressed separators over 1024 columns: split rows
     "p2d 1100 1000000 4000 1 8 0",  # 1100-column top separator, no subtree tasks
+    "p3d 20 63 50 1 8 0 0",  # level order (no list schedule)
+    "p3d 20 63 50 16 8 0 3",  # list schedule on three workgroups, 16 RHS costs
+    "p2d 40 10 16 1 8 1000 1",  # list schedule on one workgroup
 ])
 def test_dataflow_schedule_matches_level_schedule(args):
     r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)

@@ -212,7 +212,7 @@ class Engine final : public FactorEngine {
   std::unique_ptr<DataflowTasks> df_pending_, df_pending2_;
   std::exception_ptr df_err_;
   bool solve_ready_ = false;
-  int df_resident_ = 1;
+  int df_resident_[2] = {1, 1};  // resident workgroups: single- / multi-RHS kernel
   void finish_solve_setup();
   std::string df_trace_path_;
 
@@ -1320,7 +1320,7 @@ void Engine<T>::finish_solve_setup() {
     for (size_t q = 0; q < DT.gw.size(); ++q) waits[q] = {DT.gw[q], DT.need[DT.gw[q]]};
     D.gw.upload(waits, st_);
     D.cnt.alloc((size_t)(D.ngroups + 1) * sizeof(unsigned));
-    D.grid = std::max(1, std::min(D.ntasks, df_resident_));
+    D.grid = std::max(1, std::min(D.ntasks, df_resident_[q]));
     if (!df_trace_path_.empty()) D.host = DT.tasks;
     slots = std::max(slots, DT.slots);
     maxtasks = std::max<int64_t>(maxtasks, D.ntasks);
@@ -1403,13 +1403,20 @@ void Engine<T>::setup_static() {
     int dev = 0, cus = 1;
     RSC_HIP(hipGetDevice(&dev));
     RSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
-    // ticket order: a list schedule on the resident workgroups (costs for one
-    // and for sixteen right-hand sides); RSC_TICKET_ORDER=0 keeps level order
+    // RSC_DF_BLOCKS_PER_CU caps the workgroups per compute unit (default 4:
+    // more co-resident workgroups lengthen every task on the critical path)
+    int cap = 4;
+    if (const char* e = std::getenv("RSC_DF_BLOCKS_PER_CU")) cap = std::max(1, std::atoi(e));
+    df_resident_[0] = cus * std::min(cap, dev::dataflow_blocks_per_cu<T>(false));
+    df_resident_[1] = cus * std::min(cap, dev::dataflow_blocks_per_cu<T>(true));
+    // ticket order: a list schedule on the resident workgroups of each kernel
+    // (costs for one and for sixteen right-hand sides); RSC_TICKET_ORDER=0
+    // keeps level order
     TicketModel one, many;
     const char* to = std::getenv("RSC_TICKET_ORDER");
-    const int workers = (to && to[0] == '0') ? 0 : df_resident_;
-    one.workers = many.workers = workers;
+    const bool level_order = to && to[0] == '0';
+    one.workers = level_order ? 0 : df_resident_[0];
+    many.workers = level_order ? 0 : df_resident_[1];
     one.elem_bytes = many.elem_bytes = (int)sizeof(T);
     one.nrhs = 1;
     many.nrhs = 16;
@@ -1459,7 +1466,6 @@ void Engine<T>::setup_static() {
     // RSC_DEVICE_ERROR)
     if (const char* e = std::getenv("RSC_DF_TIMEOUT_US"))
       df_timeout_ = (unsigned long long)khz * (unsigned long long)std::atoll(e) / 1000ull;
-    if (!use_dataflow_) df_resident_ = cus * dev::dataflow_blocks_per_cu<T>();
     std::vector<int> z = {0, 0};
     dapst_.upload(z, st_);
   }
@@ -1829,7 +1835,8 @@ void Engine<T>::check_apply(hipStream_t s) {
     std::vector<int> z = {0, 0};
     dapst_.upload(z, s);
     RSC_HIP(hipStreamSynchronize(s));
-    throw DeviceError("preconditioner apply: a dependency wait timed out");
+    throw DeviceError(st[0] == 2 ? "preconditioner apply: subtree task in the multi-column list"
+                                 : "preconditioner apply: a dependency wait timed out");
   }
   if (dtrace_.p) {
     // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each

@@ -66,7 +66,8 @@ struct DfArgs {
   const GvTask* gv;
   const DfWait* gwait;   // wait list (DfTask::wbeg / wend index it)
   unsigned* cnt;         // ngroups completion counters, then the ticket counter
-  int* status;           // [0] = 1 after a wait timed out, [1] = abort request
+  int* status;           // [0] = 1 after a wait timed out, 2 on a task the kernel
+                         // variant does not run; [1] = abort request
   unsigned long long timeout_ticks;  // wall-clock ticks before a wait gives up
   const VecNode* nodes;
   const int* children;
@@ -90,7 +91,7 @@ void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, i
                            hipStream_t s);
 // Workgroups of the dataflow kernel resident per compute unit.
 template <typename T>
-int dataflow_blocks_per_cu();
+int dataflow_blocks_per_cu(bool multi);
 
 // CSR (= CSC of the symmetric matrix) y = A x for k columns.
 void launch_spmv(int n, const int* p, const int* idx, const double* val,

@@ -774,17 +774,20 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
 //   backward: x_R = w[rows];  x_c = z - Zp^T x_R
 // Only the subtree root's fv_R, z and x leave the workgroup; with COH they
 // are written with device-scope stores.
-struct SubtreeLds {
-  double c[kSubtreeRhs][kSubtreeMaxCols];
-  double y[kSubtreeRhs][kSubtreeMaxCols];
-  double x[kSubtreeRhs][kSubtreeMaxRows];
+// LDS vectors of a subtree task for KS right-hand sides per pass.
+template <int KS>
+struct SubtreeLdsK {
+  double c[KS][kSubtreeMaxCols];
+  double y[KS][kSubtreeMaxCols];
+  double x[KS][kSubtreeMaxRows];
 };
+using SubtreeLds = SubtreeLdsK<kSubtreeRhs>;
 
 // Right-hand sides in groups of KS (1 or kSubtreeRhs) per pass over the
 // panels; each right-hand side's sums run in the same order for any KS.
 template <typename T, bool COH, int KS>
 __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a, int sb,
-                                  const VecBufs& b, int k, SubtreeLds& L) {
+                                  const VecBufs& b, int k, SubtreeLdsK<KS>& L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   const int rootn = uni(a.nodes[q1 - 1]);
@@ -874,7 +877,7 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
 
 template <typename T, bool COH, int KS>
 __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a, int sb,
-                                   const VecBufs& b, int k, SubtreeLds& L) {
+                                   const VecBufs& b, int k, SubtreeLdsK<KS>& L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   for (int qi = q1 - 1; qi >= q0; --qi) {
@@ -919,14 +922,14 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
 template <typename T, bool COH>
 __device__ void subtree_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                                 const VecBufs& b, int k, SubtreeLds& L) {
-  if (k == 1) subtree_forward_g<T, COH, 1>(F, a, sb, b, k, L);
+  if (k == 1) subtree_forward_g<T, COH, 1>(F, a, sb, b, k, reinterpret_cast<SubtreeLdsK<1>&>(L));
   else subtree_forward_g<T, COH, kSubtreeRhs>(F, a, sb, b, k, L);
 }
 
 template <typename T, bool COH>
 __device__ void subtree_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                                  const VecBufs& b, int k, SubtreeLds& L) {
-  if (k == 1) subtree_backward_g<T, COH, 1>(F, a, sb, b, k, L);
+  if (k == 1) subtree_backward_g<T, COH, 1>(F, a, sb, b, k, reinterpret_cast<SubtreeLdsK<1>&>(L));
   else subtree_backward_g<T, COH, kSubtreeRhs>(F, a, sb, b, k, L);
 }
 
@@ -976,6 +979,29 @@ __device__ bool df_wait(const DfArgs& a, int g, unsigned need) {
   return true;
 }
 
+// LDS of the two dataflow kernels.  The single-right-hand-side kernel runs
+// subtree tasks (one right-hand side per pass) and scalar task bodies; the
+// multi-right-hand-side kernel the matrix-core bodies and four-column
+// assembly, and no subtree tasks (its list has none).  The smaller LDS of the
+// first lets more of its workgroups share a compute unit.
+struct DfLdsScalar {
+  union {
+    SubtreeLdsK<1> sl;
+    double xl[kXlds];
+  };
+};
+struct DfLdsMulti {
+  double mf[kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH];
+};
+template <bool MULTI>
+struct DfLds {
+  using type = DfLdsScalar;
+};
+template <>
+struct DfLds<true> {
+  using type = DfLdsMulti;
+};
+
 // One tid == 0 region per loop iteration, with a barrier on both sides: the
 // completion signal and the next ticket are taken together, so the compiler
 // cannot merge a tid == 0 region of one iteration with one of the next (a
@@ -990,11 +1016,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
                                                   VecBufs b, int k_arg) {
   const int k = MULTI ? k_arg : 1;
   __shared__ double red[4][64];
-  __shared__ union {
-    SubtreeLds sl;
-    double xl[kXlds];
-    double mf[kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH];
-  } lds;
+  __shared__ typename DfLds<MULTI>::type lds;
   __shared__ int s_ticket;
   __shared__ int s_ok;
   __shared__ unsigned long long s_tgrab, s_tready;
@@ -1031,7 +1053,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     }
     if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
-      if (MULTI)
+      if constexpr (MULTI)
         assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
                              a.ranges, b, k, a.B, a.ldb, a.perm, lds.mf);
       else
@@ -1040,11 +1062,18 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     } else if (type == 1) {
       gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {
-      gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.mf);
-    } else if (type == 4) {
-      subtree_forward<T, true>(F, a.sub, ta, b, k, lds.sl);
-    } else if (type == 5) {
-      subtree_backward<T, true>(F, a.sub, ta, b, k, lds.sl);
+      if constexpr (MULTI) gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.mf);
+      else gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.xl);
+    } else if (type == 4 || type == 5) {
+      if constexpr (MULTI) {
+        // not in the multi-right-hand-side list: report instead of running
+        if (tid == 0)
+          __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else if (type == 4) {
+        subtree_forward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
+      } else {
+        subtree_backward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
+      }
     } else {
       // x = P^T w over a chunk of x in its own order, x[j] = w[pinv[j]]: the
       // stores are coalesced (a scattered 8-byte store costs a memory line);
@@ -1112,13 +1141,15 @@ void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, i
 }
 
 template <typename T>
-int dataflow_blocks_per_cu() {
-  // both variants have the same launch bounds (256 threads, 4 per CU); a
-  // workgroup beyond the resident ones finds the tickets taken and returns
+int dataflow_blocks_per_cu(bool multi) {
+  // resident workgroups per compute unit of each variant (registers and LDS
+  // differ); every workgroup of a launch is resident, the grid being sized
+  // from this count
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_apply_df<T, false>, 256, 0) != hipSuccess ||
-      nb < 1)
-    nb = 1;
+  const hipError_t e =
+      multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_apply_df<T, true>, 256, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_apply_df<T, false>, 256, 0);
+  if (e != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
 
@@ -1126,8 +1157,8 @@ template void launch_apply_dataflow<double>(const double*, const DfArgs&, VecBuf
                                             hipStream_t);
 template void launch_apply_dataflow<float>(const float*, const DfArgs&, VecBufs, int, int,
                                            hipStream_t);
-template int dataflow_blocks_per_cu<double>();
-template int dataflow_blocks_per_cu<float>();
+template int dataflow_blocks_per_cu<double>(bool);
+template int dataflow_blocks_per_cu<float>(bool);
 
 // ---- SpMV ------------------------------------------------------------------
 // LPR lanes cooperate on one row; the row's partial sums are combined with

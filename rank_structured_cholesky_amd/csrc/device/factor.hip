@@ -212,6 +212,7 @@ class Engine final : public FactorEngine {
   std::unique_ptr<DataflowTasks> df_pending_, df_pending2_;
   std::exception_ptr df_err_;
   bool solve_ready_ = false;
+  int df_backoff_ = 16;
   int df_resident_[2] = {1, 1};  // resident workgroups: single- / multi-RHS kernel
   void finish_solve_setup();
   std::string df_trace_path_;
@@ -1420,6 +1421,8 @@ void Engine<T>::setup_static() {
     one.elem_bytes = many.elem_bytes = (int)sizeof(T);
     one.nrhs = 1;
     many.nrhs = 16;
+    if (const char* e = std::getenv("RSC_DF_COL_ROWS")) one.col_rows = many.col_rows = std::atoi(e);
+    if (const char* e = std::getenv("RSC_DF_ROW_COLS")) one.row_cols = many.row_cols = std::atoi(e);
     // the dataflow task lists are built on host threads while the device
     // factors; finish_solve_setup() joins them before the first apply
     df_thread_ = std::thread([this, one, many]() {
@@ -1464,6 +1467,8 @@ void Engine<T>::setup_static() {
     // RSC_DF_TIMEOUT_US shortens the bound of every dependency wait (for
     // tests of the abort path: the launch drains and the apply reports
     // RSC_DEVICE_ERROR)
+    // RSC_DF_BACKOFF: longest pause between polls of a dependency wait
+    if (const char* e = std::getenv("RSC_DF_BACKOFF")) df_backoff_ = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("RSC_DF_TIMEOUT_US"))
       df_timeout_ = (unsigned long long)khz * (unsigned long long)std::atoll(e) / 1000ull;
     std::vector<int> z = {0, 0};
@@ -1781,6 +1786,7 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
     a.cnt = D.cnt.as<unsigned>();
     a.status = dapst_.as<int>();
     a.timeout_ticks = df_timeout_;
+    a.backoff_max = df_backoff_;
     a.nodes = dvnodes_.as<dev::VecNode>();
     a.children = dchildren_.as<int>();
     a.relmap = drelmap_.as<int>();

@@ -69,6 +69,7 @@ struct DfArgs {
   int* status;           // [0] = 1 after a wait timed out, 2 on a task the kernel
                          // variant does not run; [1] = abort request
   unsigned long long timeout_ticks;  // wall-clock ticks before a wait gives up
+  int backoff_max;       // longest pause between polls of a wait, in s_sleep(4) units
   const VecNode* nodes;
   const int* children;
   const int* relmap;

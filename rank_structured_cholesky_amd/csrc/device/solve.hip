@@ -974,7 +974,7 @@ __device__ bool df_wait(const DfArgs& a, int g, unsigned need) {
       return false;
     }
     for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(4);
-    if (backoff < 16) backoff *= 2;
+    if (backoff < a.backoff_max) backoff *= 2;
   }
   return true;
 }

@@ -245,7 +245,7 @@ NodeStages node_stages(const Plan& P, const SolveLayout& L, int j) {
 // ROW: 64 rows per task.
 struct TaskEmitter {
   int64_t slot = 0;
-  static constexpr int CHUNK = 512;
+  int CHUNK = 512;     // rows per COL task (longer columns: partial sums)
   int row_split = 0;  // > 0: ROW tasks longer than this many columns are split
   void emit(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
     if (m.col) {
@@ -593,7 +593,8 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   std::vector<int> gwoff{0}, gw;
   std::vector<int> lastA(ns, -1), lastB(ns, -1), lastC(ns, -1), a0(ns, -1);
   TaskEmitter em;
-  em.row_split = kRowSplitCols;
+  em.row_split = tm.row_cols > 0 ? tm.row_cols : kRowSplitCols;
+  if (tm.col_rows > 0) em.CHUNK = tm.col_rows;
   auto new_group = [&](const std::vector<int>& waits) {
     need.push_back(0u);
     gw.insert(gw.end(), waits.begin(), waits.end());

@@ -146,6 +146,8 @@ struct TicketModel {
   int workers = 0;
   int nrhs = 1;
   int elem_bytes = 8;
+  int col_rows = 0;  // rows per COL task before partial sums (0: 512)
+  int row_cols = 0;  // columns per ROW task before partial sums (0: kRowSplitCols)
 };
 // use_subtrees = false: every supernode as ordinary tasks (the list of the
 // multi-right-hand-side apply, whose matrix-core bodies do better without

@@ -105,6 +105,8 @@ __device__ inline GvTask uniform_task(const GvTask& t) {
   u.xsel = uni(t.xsel);
   u.ysel = uni(t.ysel);
   u.part = uni(t.part);
+  u.xpart = uni(t.xpart);
+  u.xnsub = uni(t.xnsub);
   u.xoff = uni64(t.xoff);
   u.yoff = uni64(t.yoff);
   return u;
@@ -138,44 +140,23 @@ __device__ inline void vst(double* p, double v) {
   else *p = v;
 }
 
+// Entry j of a ROW task's x window for right-hand side col: the vector
+// itself or, when the producing column task's reduction is fused here
+// (xpart >= 0), the sum of its partial slots in slot order.
+template <bool COH>
+__device__ inline double x_entry(const VecBufs& b, const GvTask& t, int col, int j) {
+  if (t.xpart < 0) return vld<COH>(vsel(b, t.xsel, col) + t.xoff + j);
+  const double* pp = b.part + (int64_t)col * b.ps +
+                     ((int64_t)t.xpart + (int64_t)(j >> 5) * t.xnsub) * 32 + (j & 31);
+  double acc = 0;
+#pragma unroll 4
+  for (int s2 = 0; s2 < t.xnsub; ++s2) acc += vld<COH>(pp + (int64_t)s2 * 32);
+  return acc;
+}
+
 // Forward assembly of one 1024-entry chunk of node S's front vector:
 // fv_c = source[perm[cols]] (perm may be null: source indexed directly),
 // fv_R = 0, then the children's update vectors added through their relmaps.
-// Front-vector assembly of one chunk straight in device memory (one
-// right-hand side at a time): own entries, then child by child.
-template <bool COH>
-__device__ void assemble_chunk_direct(const VecNode& S, int chunk,
-                                      const VecNode* __restrict__ nodes,
-                                      const int* __restrict__ children,
-                                      const int* __restrict__ relmap,
-                                      const int* __restrict__ ranges, const VecBufs& b,
-                                      int k, const double* __restrict__ src, int64_t lds,
-                                      const int* __restrict__ perm) {
-  const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
-  if (p0 >= c + R) return;
-  for (int col = 0; col < k; ++col) {
-    double* f = b.fv + S.fv_off + (int64_t)col * b.fvs;
-    const double* sc = src + (int64_t)col * lds;
-    for (int i = p0 + tid; i < p1; i += 256)
-      vst<COH>(f + i, i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0);
-    __syncthreads();
-    for (int q = 0; q < S.nchild; ++q) {
-      const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
-      const int* rm = relmap + C.relmap_off;
-      const double* fc = b.fv + C.fv_off + (int64_t)col * b.fvs + C.ncols;
-      // the child rows that land in this chunk (precomputed range)
-      const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * S.nchild + q);
-      const int a0 = uni(rg[0]), a1 = uni(rg[1]);
-      for (int a = a0 + tid; a < a1; a += 256) {
-        const int p = rm[a];
-        vst<COH>(f + p, vld<COH>(f + p) + vld<COH>(fc + a));
-      }
-      __syncthreads();
-    }
-  }
-}
-
 template <bool COH>
 __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
                                const int* __restrict__ children,
@@ -495,7 +476,7 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         const int nb = min(NB, t.cols - jb);
         for (int idx = tid; idx < kb * nb; idx += 256) {
           const int cc = idx / nb, jj = idx - cc * nb;
-          xl[cc * NB + jj] = vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + jb + jj);
+          xl[cc * NB + jj] = x_entry<COH>(b, t, c0 + cc, jb + jj);
         }
         __syncthreads();
         int j = wave;
@@ -643,9 +624,7 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
         if (wact) load_q(jb, nb, ma);
         for (int idx = tid; idx < kMfmaRhs * kRowChunk; idx += 256) {
           const int cc = idx / kRowChunk, jj = idx - cc * kRowChunk;
-          sm[cc * LD + jj] = (cc < kb && jj < nb)
-                                 ? vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + jb + jj)
-                                 : 0.0;
+          sm[cc * LD + jj] = (cc < kb && jj < nb) ? x_entry<COH>(b, t, c0 + cc, jb + jj) : 0.0;
         }
         __syncthreads();
         if (wact) {
@@ -987,9 +966,10 @@ __device__ bool df_wait(const DfArgs& a, int g, unsigned need) {
 struct DfLdsScalar {
   union {
     SubtreeLdsK<1> sl;
-    double xl[kXlds];
+    double xl[kXlds];  // also the one-right-hand-side assembly chunk (VCH)
   };
 };
+static_assert(kXlds >= VCH, "the assembly chunk fits the x staging area");
 struct DfLdsMulti {
   double mf[kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH];
 };
@@ -1053,12 +1033,14 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     }
     if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
+      // the chunk is summed in LDS (one right-hand side: the first VCH
+      // entries of the x staging area), then stored once
       if constexpr (MULTI)
         assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
                              a.ranges, b, k, a.B, a.ldb, a.perm, lds.mf);
       else
-        assemble_chunk_direct<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children,
-                                    a.relmap, a.ranges, b, k, a.B, a.ldb, a.perm);
+        assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
+                             a.ranges, b, 1, a.B, a.ldb, a.perm, lds.xl);
     } else if (type == 1) {
       gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
     } else if (type == 2) {

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
 #include <queue>
 #include <stdexcept>
 #include <utility>
@@ -247,12 +248,21 @@ struct TaskEmitter {
   int64_t slot = 0;
   int CHUNK = 512;     // rows per COL task (longer columns: partial sums)
   int row_split = 0;  // > 0: ROW tasks longer than this many columns are split
+  // fuse_tv: a split COL task that sets a rank-sized vector (tv) gets no
+  // reduce tasks; the ROW tasks that read the vector next sum its partial
+  // slots (first slot and slots per 32-entry block, by tv offset)
+  bool fuse_tv = false;
+  std::map<int64_t, std::pair<int, int>> tv_parts;
   void emit(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
     if (m.col) {
+      const bool fuse = fuse_tv && m.ys == TV && m.op == 0 && !m.upper;
+      if (fuse) tv_parts.erase(m.yo);
+      const int64_t first_slot = slot;
       for (int j0 = 0; j0 < m.cols; j0 += 32) {
         const int rr = m.upper ? std::min(m.rows, j0 + 32) : m.rows;
         if (rr <= 0) continue;
         dev::GvTask t{};
+        t.xpart = -1;
         t.ld = m.ld;
         t.cols = std::min(32, m.cols - j0);
         t.kind = 1;
@@ -277,7 +287,14 @@ struct TaskEmitter {
           u.part = (int)(slot + s2);
           main.push_back(u);
         }
+        if (fuse) {
+          // every 32-column block of this vector is split alike (same rows)
+          tv_parts[m.yo] = {(int)first_slot, nsub};
+          slot += nsub;
+          continue;
+        }
         dev::GvTask rd{};
+        rd.xpart = -1;
         rd.kind = 2;
         rd.part = (int)slot;
         rd.rows = nsub;
@@ -289,8 +306,18 @@ struct TaskEmitter {
         slot += nsub;
       }
     } else {
+      int xpart = -1, xnsub = 0;
+      if (fuse_tv && m.xs == TV && !m.upper && row_split % 32 == 0) {
+        const auto it = tv_parts.find(m.xo);
+        if (it != tv_parts.end()) {
+          xpart = it->second.first;
+          xnsub = it->second.second;
+        }
+      }
       for (int i0 = 0; i0 < m.rows; i0 += 64) {
         dev::GvTask t{};
+        t.xpart = xpart;
+        t.xnsub = xnsub;
         t.mat = m.mat + i0 + (m.upper ? (int64_t)i0 * m.ld : 0);
         t.ld = m.ld;
         t.rows = std::min(64, m.rows - i0);
@@ -316,6 +343,7 @@ struct TaskEmitter {
           u.mat = t.mat + (int64_t)c0 * t.ld;
           u.cols = std::min(row_split, t.cols - c0);
           u.xoff = t.xoff + c0;
+          if (xpart >= 0) u.xpart = xpart + (c0 / 32) * xnsub;
           u.part = (int)(slot + 2 * s2);
           main.push_back(u);
         }
@@ -323,6 +351,7 @@ struct TaskEmitter {
           const int nr = std::min(32, t.rows - 32 * h);
           if (nr <= 0) continue;
           dev::GvTask rd{};
+          rd.xpart = -1;
           rd.kind = 2;
           rd.part = (int)(slot + h);
           rd.rows = nsub;
@@ -594,6 +623,10 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   std::vector<int> lastA(ns, -1), lastB(ns, -1), lastC(ns, -1), a0(ns, -1);
   TaskEmitter em;
   em.row_split = tm.row_cols > 0 ? tm.row_cols : kRowSplitCols;
+  // fused reductions: every consumer task re-sums the partial slots, cheap
+  // for one right-hand side; for sixteen the extra loads cost more than the
+  // saved reduce tasks (measured)
+  em.fuse_tv = tm.nrhs <= 1;
   if (tm.col_rows > 0) em.CHUNK = tm.col_rows;
   auto new_group = [&](const std::vector<int>& waits) {
     need.push_back(0u);
@@ -746,7 +779,9 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   }
   {
     // matrix-vector descriptors parallel to the task list
-    std::vector<dev::GvTask> gvt(tasks.size(), dev::GvTask{});
+    dev::GvTask none{};
+    none.xpart = -1;
+    std::vector<dev::GvTask> gvt(tasks.size(), none);
     for (size_t i = 0; i < tasks.size(); ++i)
       if (tasks[i].type == 2) {
         gvt[i] = gv[tasks[i].a];

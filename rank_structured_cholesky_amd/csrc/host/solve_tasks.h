@@ -20,7 +20,10 @@ namespace dev {
 // (stride fvs) or the rank-sized buffers tv (stride tvs): sel 0 / 1 / 2.
 // A COL task over many rows is split into row chunks that write partial sums
 // (part >= 0: slot of 32 partials in the partial buffer); a reduce task then
-// adds a group's partials in fixed order and applies op to y.
+// adds a group's partials in fixed order and applies op to y.  In the
+// dataflow list a split COL task that sets a rank-sized vector has no reduce
+// task: the ROW tasks that read that vector sum the partials themselves
+// (xpart >= 0), in the reduce task's order.
 struct GvTask {
   int64_t mat;
   int ld, rows, cols;
@@ -31,6 +34,8 @@ struct GvTask {
   int op;    // 0 set, 1 subtract
   int xsel, ysel;
   int part;
+  int xpart;  // ROW, >= 0: x[j] = sum over s < xnsub of the partial slot
+  int xnsub;  //   xpart + (j / 32) * xnsub + s, entry j % 32 (x itself unused)
   int64_t xoff, yoff;
 };
 

@@ -157,10 +157,24 @@ void gemv(State& S, const dev::GvTask& t) {
     const int64_t xb = (int64_t)col * S.stride(t.xsel) + t.xoff;
     if (t.kind == 0) {
       if (t.rows > 64) fail("ROW task with more than 64 rows");
+      // x window: the vector, or the partial slots of a fused reduction
+      // summed in slot order
+      std::vector<double> xw(t.cols);
+      for (int j = 0; j < t.cols; ++j) {
+        if (t.xpart < 0) {
+          xw[j] = Xv.at(xb + j);
+          continue;
+        }
+        double acc = 0;
+        for (int s2 = 0; s2 < t.xnsub; ++s2)
+          acc += S.part.at((int64_t)col * S.ps +
+                           ((int64_t)t.xpart + (int64_t)(j >> 5) * t.xnsub + s2) * 32 + (j & 31));
+        xw[j] = acc;
+      }
       for (int i = 0; i < t.rows; ++i) {
         double acc = 0;
         for (int j = 0; j < t.cols; ++j)
-          acc += S.F.at(t.mat + i + (int64_t)j * t.ld) * Xv.at(xb + j);
+          acc += S.F.at(t.mat + i + (int64_t)j * t.ld) * xw[j];
         if (t.part >= 0) S.part.at((int64_t)col * S.ps + ((int64_t)t.part + (i >> 5)) * 32 + (i & 31)) = acc;
         else if (t.op) Y.at(yb + i) -= acc;
         else Y.at(yb + i) = acc;
@@ -440,10 +454,13 @@ int main(int argc, char** argv) {
     if (stuck) fail("threaded dataflow run stalled");
   }
   if (!agrees(S.X.v)) fail("threaded dataflow result differs from the level path");
+  int fused = 0;
+  for (size_t i = 0; i < D.tasks.size(); ++i)
+    if (D.tasks[i].type == 2 && D.gv[i].kind == 0 && D.gv[i].xpart >= 0) ++fused;
   std::printf(
       "ok%s n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d "
-      "df_build_ms=%.1f\n",
+      "fused_row_tasks=%d df_build_ms=%.1f\n",
       exact ? "" : " (split rows)", S.n, (int)P.nodes.size(), nlev, L.sub.root.size(),
-      LT.gv.size(), D.tasks.size(), ng, build_ms);
+      LT.gv.size(), D.tasks.size(), ng, fused, build_ms);
   return 0;
 }

@@ -29,7 +29,8 @@ def built():
     "p2d 40 10 16 1 8",      # 2D, many compressed levels
     "el 5 30 40 1 8",        # elasticity (3 dof per node)
     "p2d 48 1000000 400 1 8",  # no compression
-    "p3d 24 63 128 3 8",     # split column tasks (partial sums)
+    "p3d 24 63 128 3 8",     # split column tasks (partial sums), fused tv reductions
+    "p3d 32 63 64 1 8",      # many fused tv reductions (separators over 512 columns)
     "p3d 20 63 50 1 8 0",    # no subtree tasks
     "p3d 20 63 50 1 8 100000000",  # subtree tasks as large as allowed
     "p3d 36 2000 400 2 8",   # uncompressed separators over 1024 columns: split rows

@@ -778,11 +778,11 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
     const T* __restrict__ Lt = F + P;
     for (int c0 = 0; c0 < k; c0 += KS) {
       const int kb = min(KS, k - c0);
+      // fv_c in L.c and fv_R in L.x, both summed in LDS
       for (int cc = 0; cc < kb; ++cc) {
-        double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
         const double* bc = a.B + (int64_t)(c0 + cc) * a.ldb;
         for (int i = tid; i < c; i += 256) L.c[cc][i] = bc[a.perm[S.first + i]];
-        for (int i = tid; i < R; i += 256) fvR[i] = 0.0;
+        for (int i = tid; i < R; i += 256) L.x[cc][i] = 0.0;
       }
       __syncthreads();
       for (int q = 0; q < S.nchild; ++q) {
@@ -790,44 +790,74 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
         const int* rm = a.relmap + C.relmap_off;
         for (int cc = 0; cc < kb; ++cc) {
           const double* fc = b.fv + C.fv_off + (int64_t)(c0 + cc) * b.fvs + C.ncols;
-          double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
           for (int t = tid; t < C.R; t += 256) {
             const int p = rm[t];
-            if (p < c) L.c[cc][p] += fc[t];
-            else fvR[p - c] += fc[t];
+            const double v = fc[t];
+            if (p < c) L.c[cc][p] += v;
+            else L.x[cc][p - c] += v;
           }
         }
         __syncthreads();
       }
-      // y_i = sum_{j <= i} Lt[j, i] fv_c[j]   (one wave per column)
-      for (int i = wave; i < c; i += 4) {
-        double acc[KS];
+      // y_i = sum_{j <= i} Lt[j, i] fv_c[j]: wave w takes columns 8g..8g+7
+      // together (lane: rows lane, lane + 64, ...; a column's order and
+      // wave reduction are those of one column at a time)
+      for (int i0 = wave * 8; i0 < c; i0 += 32) {
+        const int nu = min(8, c - i0);
+        double acc[8][KS];
 #pragma unroll
-        for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
-        const T* cp = Lt + (int64_t)i * ld;
-        for (int jj = lane; jj <= i; jj += 64) {
-          const double m = (double)cp[jj];
+        for (int u = 0; u < 8; ++u)
 #pragma unroll
-          for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.c[cc][jj];
+          for (int cc = 0; cc < KS; ++cc) acc[u][cc] = 0;
+        for (int jj = lane; jj < i0 + nu; jj += 64) {
+          double cv[KS];
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) cv[cc] = L.c[cc][jj];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const double m =
+                (u < nu && jj <= i0 + u) ? (double)Lt[jj + (int64_t)(i0 + u) * ld] : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += m * cv[cc];
+          }
         }
 #pragma unroll
-        for (int cc = 0; cc < KS; ++cc) {
-          const double v = wave_sum(acc[cc]);
-          if (lane == 0) L.y[cc][i] = v;
-        }
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) {
+            const double v = wave_sum(acc[u][cc]);
+            if (lane == 0 && u < nu) L.y[cc][i0 + u] = v;
+          }
       }
-      // fv_R -= Zp fv_c   (one thread per row)
+      // fv_R -= Zp fv_c   (one thread per row), stored once; the subtree
+      // root's fv_R leaves the workgroup (device-scope stores)
       for (int r = tid; r < R; r += 256) {
         const T* rp = Lt + c + r;
         double acc[KS];
 #pragma unroll
         for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
-        for (int jj = 0; jj < c; ++jj) {
+        // eight loads in flight, then their products in column order
+        int jj = 0;
+        for (; jj + 8 <= c; jj += 8) {
+          double m[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) m[u] = (double)rp[(int64_t)(jj + u) * ld];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) acc[cc] += m[u] * L.c[cc][jj + u];
+        }
+        for (; jj < c; ++jj) {
           const double m = (double)rp[(int64_t)jj * ld];
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.c[cc][jj];
         }
-        for (int cc = 0; cc < kb; ++cc) b.fv[S.fv_off + (int64_t)(c0 + cc) * b.fvs + c + r] -= acc[cc];
+        for (int cc = 0; cc < kb; ++cc) {
+          double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
+          const double v = L.x[cc][r] - acc[cc];
+          if (COH && j == rootn) vst<true>(fvR + r, v);
+          else fvR[r] = v;
+        }
       }
       __syncthreads();
       // z_i = sum_{j >= i} Lt[i, j] y_j   (one thread per row)
@@ -836,7 +866,17 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
         double acc[KS];
 #pragma unroll
         for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
-        for (int jj = i; jj < c; ++jj) {
+        int jj = i;
+        for (; jj + 8 <= c; jj += 8) {
+          double m[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) m[u] = (double)rp[(int64_t)(jj + u) * ld];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) acc[cc] += m[u] * L.y[cc][jj + u];
+        }
+        for (; jj < c; ++jj) {
           const double m = (double)rp[(int64_t)jj * ld];
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.y[cc][jj];
@@ -844,11 +884,6 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
         for (int cc = 0; cc < kb; ++cc)
           vst<COH>(b.z + (int64_t)(c0 + cc) * b.n + S.first + i, acc[cc]);
       }
-      if (COH && j == rootn)
-        for (int cc = 0; cc < kb; ++cc) {
-          double* fvR = b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + c;
-          for (int r = tid; r < R; r += 256) vst<true>(fvR + r, fvR[r]);
-        }
       __syncthreads();
     }
   }
@@ -873,25 +908,37 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
         for (int t = tid; t < R; t += 256) L.x[cc][t] = vld<COH>(wc + rw[t]);
       }
       __syncthreads();
-      // x_i = z_i - sum_r Zp[r, i] x_R[r]   (one wave per column)
-      for (int i = wave; i < c; i += 4) {
-        double acc[KS];
+      // x_i = z_i - sum_r Zp[r, i] x_R[r]: wave w takes columns 8g..8g+7
+      // together (lane: rows lane, lane + 64, ...; a column's order and wave
+      // reduction are those of one column at a time)
+      for (int i0 = wave * 8; i0 < c; i0 += 32) {
+        const int nu = min(8, c - i0);
+        double acc[8][KS];
 #pragma unroll
-        for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
-        const T* cp = Zp + (int64_t)i * ld;
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) acc[u][cc] = 0;
         for (int r = lane; r < R; r += 64) {
-          const double m = (double)cp[r];
+          double xv[KS];
 #pragma unroll
-          for (int cc = 0; cc < KS; ++cc) acc[cc] += m * L.x[cc][r];
-        }
+          for (int cc = 0; cc < KS; ++cc) xv[cc] = L.x[cc][r];
 #pragma unroll
-        for (int cc = 0; cc < KS; ++cc) {
-          const double v = wave_sum(acc[cc]);
-          if (lane == 0 && cc < kb) {
-            const int64_t o = (int64_t)(c0 + cc) * b.n + S.first + i;
-            vst<COH>(b.w + o, vld<COH>(b.z + o) - v);
+          for (int u = 0; u < 8; ++u) {
+            const double m = u < nu ? (double)Zp[r + (int64_t)(i0 + u) * ld] : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += m * xv[cc];
           }
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int cc = 0; cc < KS; ++cc) {
+            const double v = wave_sum(acc[u][cc]);
+            if (lane == 0 && u < nu && cc < kb) {
+              const int64_t o = (int64_t)(c0 + cc) * b.n + S.first + i0 + u;
+              vst<COH>(b.w + o, vld<COH>(b.z + o) - v);
+            }
+          }
       }
       __syncthreads();
     }

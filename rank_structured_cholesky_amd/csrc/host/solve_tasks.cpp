@@ -133,13 +133,13 @@ void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems) {
 // diagonal (leaves L^{-T}, W/X helpers), V and X_off = L_D^{-T} U.  Per
 // supernode (b = front vector fv = [fv_c; fv_R], w = permuted vector,
 // z = L^{-T} y):
-//   A  (forward, towards the parent)
-//      A0  y_c = Lc^{-1} fv_c  (uncompressed: COL on Lc^{-T}; compressed:
-//          HODLR leaves), fv_R -= Zp fv_c (ROW) or tv = X_off^T fv_c (COL)
+//   A  (forward, towards the parent: only what the parent's assembly needs)
+//      A0  fv_R -= Zp fv_c (ROW) or tv = X_off^T fv_c (COL)
 //      A1  compressed: fv_R -= V tv (ROW)
-//   B  (forward, off the path to the parent) compressed: HODLR heights of
-//      y_c; then z_c = Lc^{-T} y_c (ROW on the stored upper inverse; HODLR
-//      leaves and heights with V_M / X_M)
+//   B  (forward, off the path to the parent; starts after the assembly)
+//      y_c = Lc^{-1} fv_c (uncompressed: COL on Lc^{-T}; compressed: HODLR
+//      leaves, then heights); then z_c = Lc^{-T} y_c (ROW on the stored upper
+//      inverse; HODLR leaves and heights with V_M / X_M)
 //   C  (backward, after the gather fv_R = x[rows], x_c = z_c)
 //      uncompressed: x_c -= Zp^T fv_R (COL); compressed: tv = V^T fv_R (COL),
 //      then x_c -= X_off tv (ROW)
@@ -170,10 +170,8 @@ NodeStages node_stages(const Plan& P, const SolveLayout& L, int j) {
   int maxh = 0;
   for (auto& h : s.hnodes) maxh = std::max(maxh, h.height);
   if (!s.compress) {
-    Stage a0;
-    a0.push_back({true, s.off_panel, c + R, c, c, 0, FV, fo, W, s.first, true});
-    if (R > 0) a0.push_back({false, s.off_panel + c, c + R, R, c, 1, FV, fo, FV, fo + c, false});
-    ns.A.push_back(a0);
+    if (R > 0) ns.A.push_back({{false, s.off_panel + c, c + R, R, c, 1, FV, fo, FV, fo + c, false}});
+    ns.B.push_back({{true, s.off_panel, c + R, c, c, 0, FV, fo, W, s.first, true}});
     ns.B.push_back({{false, s.off_panel, c + R, c, c, 0, W, s.first, ZB, s.first, true}});
     if (R > 0)
       ns.C.push_back({{true, s.off_panel + c, c + R, R, c, 1, FV, fo + c, W, s.first, false}});
@@ -181,22 +179,23 @@ NodeStages node_stages(const Plan& P, const SolveLayout& L, int j) {
   }
   if (s.off_dinv >= 0) {
     // dense L_D^{-T} (root separator, no off-diagonal rows)
-    ns.A.push_back({{true, s.off_dinv, c, c, c, 0, FV, fo, W, s.first, true}});
+    ns.B.push_back({{true, s.off_dinv, c, c, c, 0, FV, fo, W, s.first, true}});
     ns.B.push_back({{false, s.off_dinv, c, c, c, 0, W, s.first, ZB, s.first, true}});
     return ns;
   }
   const int64_t tvn = L.tv_node[j];
   const std::vector<int64_t>& tvh = L.tv_h[j];
-  Stage a0;
+  if (R > 0 && s.rank > 0) {
+    ns.A.push_back({{true, s.off_Xoff, c, c, s.rank, 0, FV, fo, TV, tvn, false}});
+    ns.A.push_back({{false, s.off_V, R, R, s.rank, 1, TV, tvn, FV, fo + c, false}});
+  }
+  // y_c: HODLR leaves, then heights (forward)
+  Stage yl;
   for (auto& h : s.hnodes)
     if (h.leaf())
-      a0.push_back({true, h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fo + h.r0,
+      yl.push_back({true, h.off_leaf, h.r1 - h.r0, h.r1 - h.r0, h.r1 - h.r0, 0, FV, fo + h.r0,
                     W, s.first + h.r0, true});
-  if (R > 0 && s.rank > 0) a0.push_back({true, s.off_Xoff, c, c, s.rank, 0, FV, fo, TV, tvn, false});
-  ns.A.push_back(a0);
-  if (R > 0 && s.rank > 0)
-    ns.A.push_back({{false, s.off_V, R, R, s.rank, 1, TV, tvn, FV, fo + c, false}});
-  // y_c: HODLR heights (forward)
+  ns.B.push_back(yl);
   for (int hh = 1; hh <= maxh; ++hh) {
     Stage c1, c2;
     for (size_t q = 0; q < s.hnodes.size(); ++q) {
@@ -446,7 +445,7 @@ void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out) {
 
 // Dataflow form: per supernode three chains of groups (A towards the
 // parent, B for y and z, C backward).  A node's assembly waits on its
-// children's last A group; chain B starts after A0; the backward gather
+// children's last A group; chain B starts after the assembly; the backward gather
 // waits on the parent's last C group (nothing for a root) and on the node's
 // own chain B.  The scatter waits on every node's last C group.  Tasks are
 // emitted level by level, so the list is in topological order.
@@ -620,7 +619,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   std::vector<dev::DfTask> tasks;
   std::vector<unsigned> need;
   std::vector<int> gwoff{0}, gw;
-  std::vector<int> lastA(ns, -1), lastB(ns, -1), lastC(ns, -1), a0(ns, -1);
+  std::vector<int> lastA(ns, -1), lastB(ns, -1), lastC(ns, -1), asmg(ns, -1);
   TaskEmitter em;
   em.row_split = tm.row_cols > 0 ? tm.row_cols : kRowSplitCols;
   // fused reductions: every consumer task re-sums the partial slots, cheap
@@ -698,7 +697,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
       cur_stage = 0;
       const int g = new_group(waits);
       for (int c = 0; c < chunks(j); ++c) add(0, j, c, g);
-      lastA[j] = g;
+      lastA[j] = asmg[j] = g;
     }
     size_t nA = 0, nB = 0;
     for (int j : nodes) {
@@ -711,9 +710,8 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
           cur_node = j;
           cur_stage = 10 + (int)k;
           lastA[j] = stage(st[j].A[k], lastA[j]);
-          if (k == 0) a0[j] = lastA[j];
         }
-    for (int j : nodes) lastB[j] = a0[j] >= 0 ? a0[j] : lastA[j];
+    for (int j : nodes) lastB[j] = asmg[j];
     for (size_t k = 0; k < nB; ++k)
       for (int j : nodes)
         if (k < st[j].B.size()) {

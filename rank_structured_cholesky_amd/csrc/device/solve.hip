@@ -493,10 +493,18 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
               acc1[cc] += m[u + 1] * xl[cc * NB + j + 4 * u + 4];
             }
         }
-        for (; j < nb; j += 4) {
-          const double m = (double)rp[(int64_t)(jb + j) * t.ld];
+        if (j < nb) {
+          // the last (at most sixteen) columns of this wave: loads together,
+          // products in column order
+          double m[16];
 #pragma unroll
-          for (int cc = 0; cc < KRHS; ++cc) acc0[cc] += m * xl[cc * NB + j];
+          for (int u = 0; u < 16; ++u)
+            m[u] = j + 4 * u < nb ? (double)rp[(int64_t)(jb + j + 4 * u) * t.ld] : 0.0;
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (j + 4 * u < nb)
+#pragma unroll
+              for (int cc = 0; cc < KRHS; ++cc) acc0[cc] += m[u] * xl[cc * NB + j + 4 * u];
         }
         __syncthreads();
       }
@@ -516,53 +524,57 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         __syncthreads();
       }
     } else {
-      for (int h = 0; h < 2; ++h) {
-        const int q0 = wave * 8 + 4 * h;
-        const int nq = min(4, t.cols - q0);
-        double acc[4][KRHS];
+      // x (at most kXlds / KRHS rows) staged in LDS once; wave w takes
+      // columns 8w..8w+7 together, lane l rows l and l + 64 of every
+      // 128-row block (a column's sum order is that of four at a time)
+      for (int idx = tid; idx < kb * t.rows; idx += 256) {
+        const int cc = idx / t.rows, i = idx - cc * t.rows;
+        xl[cc * NB + i] = vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + i);
+      }
+      __syncthreads();
+      const int q0 = wave * 8;
+      const int nq = min(8, t.cols - q0);
+      double acc[8][KRHS];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 8; ++u)
 #pragma unroll
-          for (int cc = 0; cc < KRHS; ++cc) acc[u][cc] = 0;
-        if (nq > 0) {
-          const T* cbase = M + (int64_t)q0 * t.ld;
-          for (int i = lane; i < t.rows; i += 128) {
-            const int i2 = i + 64;
-            const bool ok2 = i2 < t.rows;
-            double x1[KRHS], x2[KRHS], m1[4], m2[4];
+        for (int cc = 0; cc < KRHS; ++cc) acc[u][cc] = 0;
+      if (nq > 0) {
+        const T* cbase = M + (int64_t)q0 * t.ld;
+        for (int i = lane; i < t.rows; i += 128) {
+          const int i2 = i + 64;
+          const bool ok2 = i2 < t.rows;
+          double m1[8], m2[8];
 #pragma unroll
-            for (int cc = 0; cc < KRHS; ++cc) {
-              const double* xc = vsel(b, t.xsel, c0 + (cc < kb ? cc : 0)) + t.xoff;
-              x1[cc] = cc < kb ? vld<COH>(xc + i) : 0.0;
-              x2[cc] = (cc < kb && ok2) ? vld<COH>(xc + i2) : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
-              m1[u] = (double)cp[i];
-              m2[u] = ok2 ? (double)cp[i2] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-              for (int cc = 0; cc < KRHS; ++cc) acc[u][cc] += m1[u] * x1[cc] + m2[u] * x2[cc];
+          for (int u = 0; u < 8; ++u) {
+            const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
+            m1[u] = (double)cp[i];
+            m2[u] = ok2 ? (double)cp[i2] : 0.0;
           }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
 #pragma unroll
           for (int cc = 0; cc < KRHS; ++cc) {
-            const double v = wave_sum(acc[u][cc]);
-            if (lane == 0 && u < nq && cc < kb) {
-              if (t.part >= 0)
-                vst<COH>(b.part + (int64_t)(c0 + cc) * b.ps + (int64_t)t.part * 32 + q0 + u, v);
-              else {
-                double* y = vsel(b, t.ysel, c0 + cc) + t.yoff + q0 + u;
-                vst<COH>(y, t.op ? vld<COH>(y) - v : v);
-              }
+            const double x1 = cc < kb ? xl[cc * NB + i] : 0.0;
+            const double x2 = (cc < kb && ok2) ? xl[cc * NB + i2] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u][cc] += m1[u] * x1 + m2[u] * x2;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int cc = 0; cc < KRHS; ++cc) {
+          const double v = wave_sum(acc[u][cc]);
+          if (lane == 0 && u < nq && cc < kb) {
+            if (t.part >= 0)
+              vst<COH>(b.part + (int64_t)(c0 + cc) * b.ps + (int64_t)t.part * 32 + q0 + u, v);
+            else {
+              double* y = vsel(b, t.ysel, c0 + cc) + t.yoff + q0 + u;
+              vst<COH>(y, t.op ? vld<COH>(y) - v : v);
             }
           }
-      }
+        }
+      __syncthreads();
     }
   }
 }
@@ -1017,6 +1029,7 @@ struct DfLdsScalar {
   };
 };
 static_assert(kXlds >= VCH, "the assembly chunk fits the x staging area");
+static_assert(kXlds >= kColRowsMax, "a column task's x fits the staging area");
 struct DfLdsMulti {
   double mf[kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH];
 };

@@ -626,7 +626,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   // for one right-hand side; for sixteen the extra loads cost more than the
   // saved reduce tasks (measured)
   em.fuse_tv = tm.nrhs <= 1;
-  if (tm.col_rows > 0) em.CHUNK = tm.col_rows;
+  if (tm.col_rows > 0) em.CHUNK = std::min(tm.col_rows, kColRowsMax);
   auto new_group = [&](const std::vector<int>& waits) {
     need.push_back(0u);
     gw.insert(gw.end(), waits.begin(), waits.end());

@@ -24,6 +24,7 @@ namespace dev {
 
 constexpr int MAXR = 1024;  // largest rank handled in LDS by the solves
 constexpr int kXlds = 2048;  // x entries staged in LDS per ROW batch (dataflow tasks)
+constexpr int kColGroup = 8;  // columns per pass of a single-RHS column task's wave
 
 __global__ void k_gather(const double* __restrict__ b, int ldb, double* __restrict__ w,
                          const int* __restrict__ perm, int n, int k,
@@ -532,11 +533,12 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         xl[cc * NB + i] = vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + i);
       }
       __syncthreads();
-      const int q0 = wave * 8;
-      const int nq = min(8, t.cols - q0);
-      double acc[8][KRHS];
+      for (int h = 0; h < 8; h += kColGroup) {
+      const int q0 = wave * 8 + h;
+      const int nq = min(kColGroup, t.cols - q0);
+      double acc[kColGroup][KRHS];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < kColGroup; ++u)
 #pragma unroll
         for (int cc = 0; cc < KRHS; ++cc) acc[u][cc] = 0;
       if (nq > 0) {
@@ -544,9 +546,9 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         for (int i = lane; i < t.rows; i += 128) {
           const int i2 = i + 64;
           const bool ok2 = i2 < t.rows;
-          double m1[8], m2[8];
+          double m1[kColGroup], m2[kColGroup];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
+          for (int u = 0; u < kColGroup; ++u) {
             const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
             m1[u] = (double)cp[i];
             m2[u] = ok2 ? (double)cp[i2] : 0.0;
@@ -556,12 +558,12 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
             const double x1 = cc < kb ? xl[cc * NB + i] : 0.0;
             const double x2 = (cc < kb && ok2) ? xl[cc * NB + i2] : 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc[u][cc] += m1[u] * x1 + m2[u] * x2;
+            for (int u = 0; u < kColGroup; ++u) acc[u][cc] += m1[u] * x1 + m2[u] * x2;
           }
         }
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < kColGroup; ++u)
 #pragma unroll
         for (int cc = 0; cc < KRHS; ++cc) {
           const double v = wave_sum(acc[u][cc]);
@@ -574,6 +576,7 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
             }
           }
         }
+      }
       __syncthreads();
     }
   }
@@ -765,6 +768,8 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
 //   backward: x_R = w[rows];  x_c = z - Zp^T x_R
 // Only the subtree root's fv_R, z and x leave the workgroup; with COH they
 // are written with device-scope stores.
+constexpr int kSubCols = 4;  // columns per wave in a subtree task's column products
+
 // LDS vectors of a subtree task for KS right-hand sides per pass.
 template <int KS>
 struct SubtreeLdsK {
@@ -814,11 +819,11 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
       // y_i = sum_{j <= i} Lt[j, i] fv_c[j]: wave w takes columns 8g..8g+7
       // together (lane: rows lane, lane + 64, ...; a column's order and
       // wave reduction are those of one column at a time)
-      for (int i0 = wave * 8; i0 < c; i0 += 32) {
-        const int nu = min(8, c - i0);
-        double acc[8][KS];
+      for (int i0 = wave * kSubCols; i0 < c; i0 += 4 * kSubCols) {
+        const int nu = min(kSubCols, c - i0);
+        double acc[kSubCols][KS];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kSubCols; ++u)
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) acc[u][cc] = 0;
         for (int jj = lane; jj < i0 + nu; jj += 64) {
@@ -826,7 +831,7 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) cv[cc] = L.c[cc][jj];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
+          for (int u = 0; u < kSubCols; ++u) {
             const double m =
                 (u < nu && jj <= i0 + u) ? (double)Lt[jj + (int64_t)(i0 + u) * ld] : 0.0;
 #pragma unroll
@@ -834,7 +839,7 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
           }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kSubCols; ++u)
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) {
             const double v = wave_sum(acc[u][cc]);
@@ -923,11 +928,11 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
       // x_i = z_i - sum_r Zp[r, i] x_R[r]: wave w takes columns 8g..8g+7
       // together (lane: rows lane, lane + 64, ...; a column's order and wave
       // reduction are those of one column at a time)
-      for (int i0 = wave * 8; i0 < c; i0 += 32) {
-        const int nu = min(8, c - i0);
-        double acc[8][KS];
+      for (int i0 = wave * kSubCols; i0 < c; i0 += 4 * kSubCols) {
+        const int nu = min(kSubCols, c - i0);
+        double acc[kSubCols][KS];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kSubCols; ++u)
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) acc[u][cc] = 0;
         for (int r = lane; r < R; r += 64) {
@@ -935,14 +940,14 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) xv[cc] = L.x[cc][r];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
+          for (int u = 0; u < kSubCols; ++u) {
             const double m = u < nu ? (double)Zp[r + (int64_t)(i0 + u) * ld] : 0.0;
 #pragma unroll
             for (int cc = 0; cc < KS; ++cc) acc[u][cc] += m * xv[cc];
           }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kSubCols; ++u)
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) {
             const double v = wave_sum(acc[u][cc]);

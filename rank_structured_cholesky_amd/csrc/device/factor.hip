@@ -187,12 +187,13 @@ class Engine final : public FactorEngine {
   std::vector<int64_t> fv_off_;
   // dataflow apply (one persistent launch per application)
   DBuf dapst_, dtrace_;
-  DBuf dstptr_, dstnodes_, dpanel_;
+  DBuf dstptr_, dstnodes_, dstcols_, dpanel_;
   int nsub_ = 0;
   dev::SubtreeArgs subtree_args(const double* B, int ldb) const {
     dev::SubtreeArgs a{};
     a.ptr = dstptr_.as<int>();
     a.nodes = dstnodes_.as<int>();
+    a.cols = dstcols_.as<int>();
     a.panel = dpanel_.as<int64_t>();
     a.vn = dvnodes_.as<dev::VecNode>();
     a.children = dchildren_.as<int>();
@@ -1381,6 +1382,7 @@ void Engine<T>::setup_static() {
   nsub_ = (int)SL.sub.root.size();
   dstptr_.upload(SL.sub.ptr, st_);
   dstnodes_.upload(SL.sub.nodes.empty() ? std::vector<int>{0} : SL.sub.nodes, st_);
+  dstcols_.upload(SL.sub.cols.empty() ? std::vector<int>{0, -1} : SL.sub.cols, st_);
   dpanel_.upload(SL.panel, st_);
   dchildren_.upload(SL.children, st_);
   drelmap_.upload(SL.relmap, st_);

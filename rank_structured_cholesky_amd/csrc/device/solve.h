@@ -47,6 +47,7 @@ void launch_gemv_tasks(const T* factor, const GvTask* tasks, int count, VecBufs 
 struct SubtreeArgs {
   const int* ptr;        // nodes of subtree s: nodes[ptr[s] .. ptr[s+1])
   const int* nodes;
+  const int* cols;       // subtree s: first column cols[2s], count cols[2s+1] (-1: scattered)
   const int64_t* panel;  // factor offset of every uncompressed panel
   const VecNode* vn;
   const int* children;

@@ -769,6 +769,7 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
 // Only the subtree root's fv_R, z and x leave the workgroup; with COH they
 // are written with device-scope stores.
 constexpr int kSubCols = 4;  // columns per wave in a subtree task's column products
+constexpr int kSubtreeOwn = 1024;  // subtree columns preloaded by a one-RHS task
 
 // LDS vectors of a subtree task for KS right-hand sides per pass.
 template <int KS>
@@ -776,6 +777,10 @@ struct SubtreeLdsK {
   double c[KS][kSubtreeMaxCols];
   double y[KS][kSubtreeMaxCols];
   double x[KS][kSubtreeMaxRows];
+  // one right-hand side: the subtree's own entries loaded once at the start
+  // (forward: b[perm[cols]]; backward: z[cols]) when its columns are
+  // contiguous and at most kSubtreeOwn
+  double own[KS == 1 ? kSubtreeOwn : 1];
 };
 using SubtreeLds = SubtreeLdsK<kSubtreeRhs>;
 
@@ -787,6 +792,13 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   const int rootn = uni(a.nodes[q1 - 1]);
+  // one right-hand side: every own entry b[perm[col]] of the subtree at once
+  const int col_lo = uni(a.cols[2 * sb]), col_n = uni(a.cols[2 * sb + 1]);
+  const bool pre = KS == 1 && col_n >= 0 && col_n <= kSubtreeOwn;
+  if (pre) {
+    for (int i = tid; i < col_n; i += 256) L.own[i] = a.B[a.perm[col_lo + i]];
+    __syncthreads();
+  }
   for (int qi = q0; qi < q1; ++qi) {
     const int j = uni(a.nodes[qi]);
     const VecNode S = uniform_node(a.vn[j]);
@@ -798,7 +810,10 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
       // fv_c in L.c and fv_R in L.x, both summed in LDS
       for (int cc = 0; cc < kb; ++cc) {
         const double* bc = a.B + (int64_t)(c0 + cc) * a.ldb;
-        for (int i = tid; i < c; i += 256) L.c[cc][i] = bc[a.perm[S.first + i]];
+        if (pre)
+          for (int i = tid; i < c; i += 256) L.c[cc][i] = L.own[S.first - col_lo + i];
+        else
+          for (int i = tid; i < c; i += 256) L.c[cc][i] = bc[a.perm[S.first + i]];
         for (int i = tid; i < R; i += 256) L.x[cc][i] = 0.0;
       }
       __syncthreads();
@@ -911,6 +926,13 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
                                    const VecBufs& b, int k, SubtreeLdsK<KS>& L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
+  // one right-hand side: z of every subtree column at once
+  const int col_lo = uni(a.cols[2 * sb]), col_n = uni(a.cols[2 * sb + 1]);
+  const bool pre = KS == 1 && col_n >= 0 && col_n <= kSubtreeOwn;
+  if (pre) {
+    for (int i = tid; i < col_n; i += 256) L.own[i] = vld<COH>(b.z + col_lo + i);
+    __syncthreads();
+  }
   for (int qi = q1 - 1; qi >= q0; --qi) {
     const int j = uni(a.nodes[qi]);
     const VecNode S = uniform_node(a.vn[j]);
@@ -953,7 +975,8 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
             const double v = wave_sum(acc[u][cc]);
             if (lane == 0 && u < nu && cc < kb) {
               const int64_t o = (int64_t)(c0 + cc) * b.n + S.first + i0 + u;
-              vst<COH>(b.w + o, vld<COH>(b.z + o) - v);
+              const double zv = pre ? L.own[S.first + i0 + u - col_lo] : vld<COH>(b.z + o);
+              vst<COH>(b.w + o, zv - v);
             }
           }
       }

@@ -18,6 +18,7 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S) {
   S.ptr.assign(1, 0);
   S.nodes.clear();
   S.root.clear();
+  S.cols.clear();
   if (max_elems <= 0) return;
   // children are numbered before their parents (postordered supernodes)
   std::vector<char> ok(ns, 0);
@@ -52,6 +53,16 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S) {
     for (int q : mem) S.of[q] = id;
     S.nodes.insert(S.nodes.end(), mem.begin(), mem.end());
     S.ptr.push_back((int)S.nodes.size());
+    // column range of the subtree when its supernodes' columns are contiguous
+    int lo = INT_MAX, hi = 0;
+    int64_t tot = 0;
+    for (int q : mem) {
+      lo = std::min(lo, P.nodes[q].first);
+      hi = std::max(hi, P.nodes[q].first + P.nodes[q].ncols);
+      tot += P.nodes[q].ncols;
+    }
+    S.cols.push_back(lo);
+    S.cols.push_back(tot == (int64_t)hi - lo ? hi - lo : -1);
   }
 }
 

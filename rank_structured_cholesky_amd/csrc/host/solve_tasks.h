@@ -98,6 +98,8 @@ struct Subtrees {
   std::vector<int> of;          // per supernode: subtree id, -1 if none
   std::vector<int> ptr, nodes;  // nodes of subtree s (children before parents)
   std::vector<int> root;        // root supernode of subtree s
+  std::vector<int> cols;        // per subtree: first column, column count
+                                // (-1 when the columns are not contiguous)
 };
 // Maximal subtrees whose panels hold at most max_elems elements in total.
 void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S);

@@ -263,7 +263,43 @@ struct TaskEmitter {
   // slots (first slot and slots per 32-entry block, by tv offset)
   bool fuse_tv = false;
   std::map<int64_t, std::pair<int, int>> tv_parts;
+  // A product of at least wide_elems matrix elements is split finer (down to
+  // 128 rows per COL task, 256 columns per ROW task) while it still yields at
+  // most half as many tasks as there are workgroups: the few huge supernodes
+  // at the top of the tree run nearly alone, and there a stage's time is its
+  // longest task's.
+  int workers = 0;
+  int64_t wide_elems = int64_t(1) << 20;
   void emit(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
+    const int chunk0 = CHUNK, split0 = row_split;
+    if (workers > 0 && (int64_t)m.rows * m.cols >= wide_elems) {
+      if (m.col) {
+        auto count = [&](int ch) {
+          int64_t n = 0;
+          for (int j0 = 0; j0 < m.cols; j0 += 32) {
+            const int rr = m.upper ? std::min(m.rows, j0 + 32) : m.rows;
+            n += (rr + ch - 1) / ch;
+          }
+          return n;
+        };
+        while (CHUNK > 128 && 2 * count(CHUNK / 2) <= workers) CHUNK /= 2;
+      } else if (row_split > 0) {
+        auto count = [&](int rs) {
+          int64_t n = 0;
+          for (int i0 = 0; i0 < m.rows; i0 += 64) {
+            const int cc = m.upper ? m.cols - i0 : m.cols;
+            n += (cc + rs - 1) / rs;
+          }
+          return n;
+        };
+        while (row_split > 256 && 2 * count(row_split / 2) <= workers) row_split /= 2;
+      }
+    }
+    emit_split(m, main, red);
+    CHUNK = chunk0;
+    row_split = split0;
+  }
+  void emit_split(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
     if (m.col) {
       const bool fuse = fuse_tv && m.ys == TV && m.op == 0 && !m.upper;
       if (fuse) tv_parts.erase(m.yo);
@@ -638,6 +674,10 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   // saved reduce tasks (measured)
   em.fuse_tv = tm.nrhs <= 1;
   if (tm.col_rows > 0) em.CHUNK = std::min(tm.col_rows, kColRowsMax);
+  // finer splits of huge products: measured faster with sixteen right-hand
+  // sides (matrix-core bodies), not with one
+  em.workers = tm.nrhs > 1 || tm.wide_elems > 0 ? tm.workers : 0;
+  if (tm.wide_elems > 0) em.wide_elems = tm.wide_elems;
   auto new_group = [&](const std::vector<int>& waits) {
     need.push_back(0u);
     gw.insert(gw.end(), waits.begin(), waits.end());

@@ -156,6 +156,8 @@ struct TicketModel {
   int elem_bytes = 8;
   int col_rows = 0;  // rows per COL task before partial sums (0: 512)
   int row_cols = 0;  // columns per ROW task before partial sums (0: kRowSplitCols)
+  int64_t wide_elems = 0;  // products this large are split finer (0: 2^20 elements,
+                           // and only for nrhs > 1)
 };
 // use_subtrees = false: every supernode as ordinary tasks (the list of the
 // multi-right-hand-side apply, whose matrix-core bodies do better without

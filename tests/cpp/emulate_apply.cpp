@@ -13,9 +13,10 @@
 // dependency or out-of-range index shows up.
 //
 //   emulate_apply <problem> <N> <max_dense_block> <max_diagonal_block> <nrhs> <threads>
-//                 [subtree_elems [workers]]
+//                 [subtree_elems [workers [wide_elems]]]
 //   workers: ticket order by a list schedule on that many workgroups
-//   (default 1024; 0 keeps the level order)
+//   (default 1024; 0 keeps the level order); wide_elems: products split
+//   finer from this many elements (default 2^20)
 //   problem: p3d (7-point log-normal Poisson), p2d, el (elasticity)
 // Prints "ok ..." and exits 0 on success.
 #include <atomic>
@@ -307,6 +308,7 @@ int main(int argc, char** argv) {
   DataflowTasks D;
   TicketModel tm;
   tm.workers = argc > 8 ? std::atoi(argv[8]) : 1024;
+  tm.wide_elems = argc > 9 ? std::atoll(argv[9]) : 0;
   tm.nrhs = k;
   const auto tb0 = std::chrono::steady_clock::now();
   build_dataflow_tasks(P, L, D, true, tm);

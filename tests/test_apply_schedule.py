@@ -38,6 +38,8 @@ def built():
     "p3d 20 63 50 1 8 0 0",  # level order (no list schedule)
     "p3d 20 63 50 16 8 0 3",  # list schedule on three workgroups, 16 RHS costs
     "p2d 40 10 16 1 8 1000 1",  # list schedule on one workgroup
+    "p3d 32 2000 400 1 8 16384 1024 20000",  # wide products split finer
+    "p3d 24 63 128 2 8 16384 4096 5000",  # the same with compressed separators
 ])
 def test_dataflow_schedule_matches_level_schedule(args):
     r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)

@@ -224,12 +224,23 @@ __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict_
     }
     return;
   }
-  const int len = p1 - p0;
-  for (int idx = tid; idx < k * len; idx += 256) {
-    const int col = idx / len, i = p0 + (idx - col * len);
-    double* wc = b.w + (int64_t)col * b.n;
-    if (i < c) vst<COH>(wc + S.first + i, vld<COH>(b.z + (int64_t)col * b.n + S.first + i));
-    else vst<COH>(b.fv + S.fv_off + (int64_t)col * b.fvs + i, vld<COH>(wc + rw[i - c]));
+  // a thread's entry index is loaded once for every right-hand side, and
+  // the loads of eight right-hand sides go out together
+  for (int i = p0 + tid; i < p1; i += 256) {
+    const bool own = i < c;
+    const int64_t src = own ? (int64_t)S.first + i : (int64_t)rw[i - c];
+    const double* from = own ? b.z : b.w;
+    double* to = own ? b.w + S.first + i : b.fv + S.fv_off + i;
+    const int64_t ts = own ? b.n : b.fvs;
+    int col = 0;
+    for (; col + 8 <= k; col += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = vld<COH>(from + (int64_t)(col + u) * b.n + src);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vst<COH>(to + (int64_t)(col + u) * ts, v[u]);
+    }
+    for (; col < k; ++col) vst<COH>(to + (int64_t)col * ts, vld<COH>(from + (int64_t)col * b.n + src));
   }
 }
 

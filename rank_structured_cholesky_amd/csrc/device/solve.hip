@@ -158,6 +158,15 @@ __device__ inline double x_entry(const VecBufs& b, const GvTask& t, int col, int
 // Forward assembly of one 1024-entry chunk of node S's front vector:
 // fv_c = source[perm[cols]] (perm may be null: source indexed directly),
 // fv_R = 0, then the children's update vectors added through their relmaps.
+// Children whose descriptors an assembly task keeps in LDS for all its
+// passes (more are read again per pass).
+constexpr int kAsmChildren = 32;
+struct AsmChild {
+  int64_t rm_off;  // relmap offset of the child
+  int64_t fc_off;  // its update vector (fv_R) offset, first right-hand side
+  int a0, cr;      // its rows that land in this chunk: [a0, a0 + cr)
+};
+
 template <bool COH>
 __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
                                const int* __restrict__ children,
@@ -168,29 +177,65 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
   // The chunk of kAsmRhs right-hand sides at a time is summed in LDS (sm:
   // kAsmRhs * VCH doubles): its own entries, then child by child (a child's
   // rows map to distinct positions), then written once.  Each entry's sum
-  // order is the same for any k.
+  // order is the same for any k.  The children's descriptors and the own
+  // entries' source indices are read once for all passes.
+  __shared__ AsmChild s_ch[kAsmChildren];
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
   const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
   if (p0 >= c + R) return;
   const int len = p1 - p0;
+  const int nch = S.nchild;
+  const bool cached = nch <= kAsmChildren;
+  if (cached && tid < nch) {
+    const VecNode C = nodes[children[S.child_off + tid]];
+    const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * nch + tid);
+    s_ch[tid] = AsmChild{C.relmap_off, C.fv_off + C.ncols, rg[0], rg[1] - rg[0]};
+  }
+  constexpr int PER = VCH / 256;
+  int own[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int i = p0 + tid + 256 * u;
+    own[u] = i < p1 && i < c ? (perm ? perm[S.first + i] : S.first + i) : -1;
+  }
   for (int c0 = 0; c0 < k; c0 += kAsmRhs) {
     const int g = min(kAsmRhs, k - c0);
-    for (int idx = tid; idx < g * len; idx += 256) {
-      const int cc = idx / len, i = p0 + (idx - cc * len);
+    for (int cc = 0; cc < g; ++cc) {
       const double* sc = src + (int64_t)(c0 + cc) * lds;
-      sm[cc * VCH + (i - p0)] = i < c ? sc[perm ? perm[S.first + i] : S.first + i] : 0.0;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int i = tid + 256 * u;
+        if (i < len) sm[cc * VCH + i] = own[u] >= 0 ? sc[own[u]] : 0.0;
+      }
     }
     __syncthreads();
-    for (int q = 0; q < S.nchild; ++q) {
-      const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
-      const int* rm = relmap + C.relmap_off;
-      const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * S.nchild + q);
-      const int a0 = uni(rg[0]), cr = uni(rg[1]) - a0;
-      for (int idx = tid; idx < g * cr; idx += 256) {
-        const int cc = idx / cr, a = a0 + (idx - cc * cr);
-        const int p = rm[a];
-        sm[cc * VCH + (p - p0)] +=
-            vld<COH>(b.fv + C.fv_off + (int64_t)(c0 + cc) * b.fvs + C.ncols + a);
+    for (int q = 0; q < nch; ++q) {
+      int64_t rm_off, fc_off;
+      int a0, cr;
+      if (cached) {
+        rm_off = uni64(s_ch[q].rm_off);
+        fc_off = uni64(s_ch[q].fc_off);
+        a0 = uni(s_ch[q].a0);
+        cr = uni(s_ch[q].cr);
+      } else {
+        const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
+        const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * nch + q);
+        rm_off = C.relmap_off;
+        fc_off = C.fv_off + C.ncols;
+        a0 = uni(rg[0]);
+        cr = uni(rg[1]) - a0;
+      }
+      const int* rm = relmap + rm_off;
+      for (int t = tid; t < cr; t += 256) {
+        const int a = a0 + t;
+        const int p = rm[a] - p0;
+        double v[kAsmRhs];
+#pragma unroll
+        for (int cc = 0; cc < kAsmRhs; ++cc)
+          v[cc] = cc < g ? vld<COH>(b.fv + fc_off + (int64_t)(c0 + cc) * b.fvs + a) : 0.0;
+#pragma unroll
+        for (int cc = 0; cc < kAsmRhs; ++cc)
+          if (cc < g) sm[cc * VCH + p] += v[cc];
       }
       __syncthreads();
     }

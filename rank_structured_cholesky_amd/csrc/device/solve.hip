@@ -1076,7 +1076,7 @@ template void launch_subtrees<double>(const double*, const SubtreeArgs&, int, bo
 template void launch_subtrees<float>(const float*, const SubtreeArgs&, int, bool, VecBufs, int,
                                      hipStream_t);
 
-constexpr int DF_SCATTER_CHUNK = 4096;  // x entries per scatter task (16 per thread)
+constexpr int DF_SCATTER_CHUNK = kScatterChunk;  // x entries per scatter task (8 per thread)
 static_assert(DF_SCATTER_CHUNK % (8 * 256) == 0, "scatter chunk: eight entries per thread per pass");
 
 // Thread 0 waits until group g has completed need tasks.  Returns false when

@@ -86,7 +86,7 @@ struct DfWait {
 constexpr int kSolveChunk = 1024;    // front-vector entries per assembly/gather task
 constexpr int kRowSplitCols = 1024;  // dataflow ROW tasks: columns per partial task
 constexpr int kColRowsMax = 2048;    // dataflow COL tasks: most rows (x staged in LDS)
-constexpr int kScatterChunk = 4096;  // vector entries per scatter task
+constexpr int kScatterChunk = 2048;  // vector entries per scatter task
 // Subtree tasks: one workgroup runs every supernode of a small subtree in
 // turn; a node's ncols and off-diagonal rows must fit its LDS vectors.
 constexpr int kSubtreeMaxCols = 256;

@@ -539,29 +539,32 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         int j = wave;
         for (; j + 60 < nb; j += 64) {
           const T* p = rp + (int64_t)(jb + j) * t.ld;
-          double m[16];
+          // loads kept in the factor type (each its own register), widened
+          // at their products
+          T m[16];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) m[u] = (double)p[u * ld4];
+          for (int u = 0; u < 16; ++u) m[u] = p[u * ld4];
 #pragma unroll
           for (int cc = 0; cc < KRHS; ++cc)
 #pragma unroll
             for (int u = 0; u < 16; u += 2) {
-              acc0[cc] += m[u] * xl[cc * NB + j + 4 * u];
-              acc1[cc] += m[u + 1] * xl[cc * NB + j + 4 * u + 4];
+              acc0[cc] += (double)m[u] * xl[cc * NB + j + 4 * u];
+              acc1[cc] += (double)m[u + 1] * xl[cc * NB + j + 4 * u + 4];
             }
         }
         if (j < nb) {
           // the last (at most sixteen) columns of this wave: loads together,
           // products in column order
-          double m[16];
+          T m[16];
 #pragma unroll
           for (int u = 0; u < 16; ++u)
-            m[u] = j + 4 * u < nb ? (double)rp[(int64_t)(jb + j + 4 * u) * t.ld] : 0.0;
+            m[u] = j + 4 * u < nb ? rp[(int64_t)(jb + j + 4 * u) * t.ld] : T(0);
 #pragma unroll
           for (int u = 0; u < 16; ++u)
             if (j + 4 * u < nb)
 #pragma unroll
-              for (int cc = 0; cc < KRHS; ++cc) acc0[cc] += m[u] * xl[cc * NB + j + 4 * u];
+              for (int cc = 0; cc < KRHS; ++cc)
+                acc0[cc] += (double)m[u] * xl[cc * NB + j + 4 * u];
         }
         __syncthreads();
       }
@@ -602,19 +605,20 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         for (int i = lane; i < t.rows; i += 128) {
           const int i2 = i + 64;
           const bool ok2 = i2 < t.rows;
-          double m1[kColGroup], m2[kColGroup];
+          T m1[kColGroup], m2[kColGroup];
 #pragma unroll
           for (int u = 0; u < kColGroup; ++u) {
             const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
-            m1[u] = (double)cp[i];
-            m2[u] = ok2 ? (double)cp[i2] : 0.0;
+            m1[u] = cp[i];
+            m2[u] = ok2 ? cp[i2] : T(0);
           }
 #pragma unroll
           for (int cc = 0; cc < KRHS; ++cc) {
             const double x1 = cc < kb ? xl[cc * NB + i] : 0.0;
             const double x2 = (cc < kb && ok2) ? xl[cc * NB + i2] : 0.0;
 #pragma unroll
-            for (int u = 0; u < kColGroup; ++u) acc[u][cc] += m1[u] * x1 + m2[u] * x2;
+            for (int u = 0; u < kColGroup; ++u)
+              acc[u][cc] += (double)m1[u] * x1 + (double)m2[u] * x2;
           }
         }
       }
@@ -676,22 +680,22 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       // columns jb + 4u + kq of this lane's row, u < 8 (a quarter chunk);
       // the lane's part of the address is fixed, the column offset uniform
       const T* __restrict__ mlane = mrow + (int64_t)kq * t.ld;
-      auto load_q = [&](int jb, int nb, double (&m)[8]) {
+      auto load_q = [&](int jb, int nb, T (&m)[8]) {
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          m[u] = (rok && kq < nb - 4 * u) ? (double)mlane[(int64_t)(jb + 4 * u) * t.ld] : 0.0;
+          m[u] = (rok && kq < nb - 4 * u) ? mlane[(int64_t)(jb + 4 * u) * t.ld] : T(0);
       };
       // one accumulator, columns in increasing groups of four
-      auto mul_q = [&](const double (&m)[8], const double* xr) {
+      auto mul_q = [&](const T (&m)[8], const double* xr) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = mfma64(m[u], xr[4 * u], acc);
+        for (int u = 0; u < 8; ++u) acc = mfma64((double)m[u], xr[4 * u], acc);
       };
       for (int jb = 0; jb < t.cols; jb += kRowChunk) {
         const int nb = min(kRowChunk, t.cols - jb);
         // the chunk's matrix block is loaded a quarter ahead: the first
         // quarter while x is staged, each next one while the previous is
         // multiplied
-        double ma[8], mb[8];
+        T ma[8], mb[8];
         if (wact) load_q(jb, nb, ma);
         for (int idx = tid; idx < kMfmaRhs * kRowChunk; idx += 256) {
           const int cc = idx / kRowChunk, jj = idx - cc * kRowChunk;
@@ -901,13 +905,14 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
           double cv[KS];
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) cv[cc] = L.c[cc][jj];
+          T m[kSubCols];
 #pragma unroll
-          for (int u = 0; u < kSubCols; ++u) {
-            const double m =
-                (u < nu && jj <= i0 + u) ? (double)Lt[jj + (int64_t)(i0 + u) * ld] : 0.0;
+          for (int u = 0; u < kSubCols; ++u)
+            m[u] = (u < nu && jj <= i0 + u) ? Lt[jj + (int64_t)(i0 + u) * ld] : T(0);
 #pragma unroll
-            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += m * cv[cc];
-          }
+          for (int u = 0; u < kSubCols; ++u)
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += (double)m[u] * cv[cc];
         }
 #pragma unroll
         for (int u = 0; u < kSubCols; ++u)
@@ -927,13 +932,13 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
         // eight loads in flight, then their products in column order
         int jj = 0;
         for (; jj + 8 <= c; jj += 8) {
-          double m[8];
+          T m[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) m[u] = (double)rp[(int64_t)(jj + u) * ld];
+          for (int u = 0; u < 8; ++u) m[u] = rp[(int64_t)(jj + u) * ld];
 #pragma unroll
           for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int cc = 0; cc < KS; ++cc) acc[cc] += m[u] * L.c[cc][jj + u];
+            for (int cc = 0; cc < KS; ++cc) acc[cc] += (double)m[u] * L.c[cc][jj + u];
         }
         for (; jj < c; ++jj) {
           const double m = (double)rp[(int64_t)jj * ld];
@@ -956,13 +961,13 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
         for (int cc = 0; cc < KS; ++cc) acc[cc] = 0;
         int jj = i;
         for (; jj + 8 <= c; jj += 8) {
-          double m[8];
+          T m[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) m[u] = (double)rp[(int64_t)(jj + u) * ld];
+          for (int u = 0; u < 8; ++u) m[u] = rp[(int64_t)(jj + u) * ld];
 #pragma unroll
           for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int cc = 0; cc < KS; ++cc) acc[cc] += m[u] * L.y[cc][jj + u];
+            for (int cc = 0; cc < KS; ++cc) acc[cc] += (double)m[u] * L.y[cc][jj + u];
         }
         for (; jj < c; ++jj) {
           const double m = (double)rp[(int64_t)jj * ld];
@@ -1017,12 +1022,13 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
           double xv[KS];
 #pragma unroll
           for (int cc = 0; cc < KS; ++cc) xv[cc] = L.x[cc][r];
+          T m[kSubCols];
 #pragma unroll
-          for (int u = 0; u < kSubCols; ++u) {
-            const double m = u < nu ? (double)Zp[r + (int64_t)(i0 + u) * ld] : 0.0;
+          for (int u = 0; u < kSubCols; ++u) m[u] = u < nu ? Zp[r + (int64_t)(i0 + u) * ld] : T(0);
 #pragma unroll
-            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += m * xv[cc];
-          }
+          for (int u = 0; u < kSubCols; ++u)
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += (double)m[u] * xv[cc];
         }
 #pragma unroll
         for (int u = 0; u < kSubCols; ++u)

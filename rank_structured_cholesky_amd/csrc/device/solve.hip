@@ -25,6 +25,10 @@ namespace dev {
 constexpr int MAXR = 1024;  // largest rank handled in LDS by the solves
 constexpr int kXlds = 2048;  // x entries staged in LDS per ROW batch (dataflow tasks)
 constexpr int kColGroup = 8;  // columns per pass of a single-RHS column task's wave
+// Loads of the single-RHS task bodies: an FP32 factor is loaded
+// unconditionally (a clamped in-range address, the value masked afterwards),
+// since predicated narrow loads were serialized through one register; the
+// FP64 bodies keep predicated loads, which need fewer live registers.
 
 __global__ void k_gather(const double* __restrict__ b, int ldb, double* __restrict__ w,
                          const int* __restrict__ perm, int n, int k,
@@ -610,7 +614,8 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
           for (int u = 0; u < kColGroup; ++u) {
             const T* cp = cbase + (int64_t)(u < nq ? u : 0) * t.ld;
             m1[u] = cp[i];
-            m2[u] = ok2 ? cp[i2] : T(0);
+            if constexpr (sizeof(T) == 4) m2[u] = cp[ok2 ? i2 : i];  // unused when !ok2
+            else m2[u] = ok2 ? cp[i2] : T(0);
           }
 #pragma unroll
           for (int cc = 0; cc < KRHS; ++cc) {
@@ -683,7 +688,8 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       auto load_q = [&](int jb, int nb, T (&m)[8]) {
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          m[u] = (rok && kq < nb - 4 * u) ? mlane[(int64_t)(jb + 4 * u) * t.ld] : T(0);
+          m[u] = mlane[(int64_t)(jb + min(4 * u, nb - 1 - kq)) * t.ld];  // past the
+                                     // end: an in-range column times x = 0
       };
       // one accumulator, columns in increasing groups of four
       auto mul_q = [&](const T (&m)[8], const double* xr) {
@@ -907,12 +913,17 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
           for (int cc = 0; cc < KS; ++cc) cv[cc] = L.c[cc][jj];
           T m[kSubCols];
 #pragma unroll
-          for (int u = 0; u < kSubCols; ++u)
-            m[u] = (u < nu && jj <= i0 + u) ? Lt[jj + (int64_t)(i0 + u) * ld] : T(0);
+          for (int u = 0; u < kSubCols; ++u) {
+            const bool in = u < nu && jj <= i0 + u;
+            if constexpr (sizeof(T) == 4) m[u] = Lt[jj + (int64_t)min(i0 + u, c - 1) * ld];
+            else m[u] = in ? Lt[jj + (int64_t)(i0 + u) * ld] : T(0);
+          }
 #pragma unroll
-          for (int u = 0; u < kSubCols; ++u)
+          for (int u = 0; u < kSubCols; ++u) {
+            const double mv = (u < nu && jj <= i0 + u) ? (double)m[u] : 0.0;
 #pragma unroll
-            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += (double)m[u] * cv[cc];
+            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += mv * cv[cc];
+          }
         }
 #pragma unroll
         for (int u = 0; u < kSubCols; ++u)
@@ -1024,11 +1035,16 @@ __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a
           for (int cc = 0; cc < KS; ++cc) xv[cc] = L.x[cc][r];
           T m[kSubCols];
 #pragma unroll
-          for (int u = 0; u < kSubCols; ++u) m[u] = u < nu ? Zp[r + (int64_t)(i0 + u) * ld] : T(0);
+          for (int u = 0; u < kSubCols; ++u) {
+            if constexpr (sizeof(T) == 4) m[u] = Zp[r + (int64_t)min(i0 + u, c - 1) * ld];
+            else m[u] = u < nu ? Zp[r + (int64_t)(i0 + u) * ld] : T(0);
+          }
 #pragma unroll
-          for (int u = 0; u < kSubCols; ++u)
+          for (int u = 0; u < kSubCols; ++u) {
+            const double mv = u < nu ? (double)m[u] : 0.0;
 #pragma unroll
-            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += (double)m[u] * xv[cc];
+            for (int cc = 0; cc < KS; ++cc) acc[u][cc] += mv * xv[cc];
+          }
         }
 #pragma unroll
         for (int u = 0; u < kSubCols; ++u)

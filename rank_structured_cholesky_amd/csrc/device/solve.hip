@@ -559,16 +559,20 @@ __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBu
         if (j < nb) {
           // the last (at most sixteen) columns of this wave: loads together,
           // products in column order
-          T m[16];
+          // (clamped in-range addresses, loads in batches of eight)
 #pragma unroll
-          for (int u = 0; u < 16; ++u)
-            m[u] = j + 4 * u < nb ? rp[(int64_t)(jb + j + 4 * u) * t.ld] : T(0);
+          for (int h = 0; h < 16; h += 8) {
+            T m[8];
 #pragma unroll
-          for (int u = 0; u < 16; ++u)
-            if (j + 4 * u < nb)
+            for (int u = 0; u < 8; ++u)
+              m[u] = rp[(int64_t)(jb + min(j + 4 * (h + u), nb - 1)) * t.ld];
 #pragma unroll
-              for (int cc = 0; cc < KRHS; ++cc)
-                acc0[cc] += (double)m[u] * xl[cc * NB + j + 4 * u];
+            for (int u = 0; u < 8; ++u)
+              if (j + 4 * (h + u) < nb)
+#pragma unroll
+                for (int cc = 0; cc < KRHS; ++cc)
+                  acc0[cc] += (double)m[u] * xl[cc * NB + j + 4 * (h + u)];
+          }
         }
         __syncthreads();
       }

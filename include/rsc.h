@@ -146,9 +146,20 @@ rsc_status rsc_factor_refactor(rsc_factor* f, const double* values);
 
 /* X = M^{-1} B for nrhs columns; B and X are DEVICE pointers (float64,
  * column-major, leading dimensions ldb/ldx >= n).  stream: hipStream_t or
- * NULL for the factor's own stream. */
+ * NULL for the factor's own stream.  The apply is queued asynchronously, so
+ * RSC_OK means "queued": a device-side failure of the apply (a dependency
+ * wait of the dataflow launch that ran out of time) is reported by
+ * rsc_apply_status, which the caller runs before trusting X. */
 rsc_status rsc_apply(rsc_factor* f, int nrhs, const double* B, int ldb,
                      double* X, int ldx, void* stream);
+
+/* Synchronizes stream (NULL: the factor's own) and returns RSC_DEVICE_ERROR
+ * if an apply queued on it since the last check failed on the device, else
+ * RSC_OK.  Counterpart of the reference's synchronous solve returning
+ * (FactorPreconditioner.cpp:135-162): the result of rsc_apply is valid once
+ * this returns RSC_OK.  The PCG and host-pointer entry points check this
+ * themselves. */
+rsc_status rsc_apply_status(rsc_factor* f, void* stream);
 
 /* PCG on A x = b with the factor as preconditioner.  b, x: DEVICE pointers;
  * x holds the initial guess on entry (the reference driver passes zero).

@@ -130,6 +130,40 @@ class FactorPreconditioner : public PRECONDITIONER {
     }
   }
 
+  // initialize() with a caller-supplied nested dissection in CHOLMOD's
+  // format (perm[new] = old, cparent per component, cmember per original
+  // index), e.g. a reference run's dumps read by readReferenceOrdering
+  // (rschol_amd_io.hpp).  The modes are the reference drivers' (CHOLESKY,
+  // IN_PLACE, DECOMPOSE_TRANSPOSE, DECOMPOSE_FACTOR).
+  void initializeWithOrdering(const SparseColumnMatrix& matrix,
+                              const Vector3Array& meshPositions, const std::vector<int>& perm,
+                              const std::vector<int>& cparent,
+                              const std::vector<int>& cmember, int maxDenseBlock,
+                              int maxDiagonalBlock, bool reorderSeparators) {
+    clearSystem();
+    _options.max_dense_block = maxDenseBlock;
+    _options.max_diagonal_block = maxDiagonalBlock;
+    _options.reorder_separators = reorderSeparators ? 1 : 0;
+    std::vector<Real> xyz;
+    if (!meshPositions.empty()) {
+      xyz.resize(3 * meshPositions.size());
+      for (size_t k = 0; k < meshPositions.size(); ++k)
+        for (int d = 0; d < 3; ++d) xyz[3 * k + d] = meshPositions[k][d];
+    }
+    if ((int)perm.size() != matrix._nrow || (int)cmember.size() != matrix._nrow)
+      throw Error(RSC_BAD_INPUT, "ordering size differs from the matrix");
+    check(rsc_factor_create_with_ordering(
+              matrix._nrow, matrix._p.data(), matrix._i.data(), matrix._x.data(),
+              xyz.empty() ? nullptr : xyz.data(), perm.data(), (int)cparent.size(),
+              cparent.data(), cmember.data(), &_options, &_factor),
+          "rsc_factor_create_with_ordering");
+    _pattern_p = matrix._p;
+    _pattern_i = matrix._i;
+    _values = matrix._x;
+    _symbolicFactorNeeded = false;
+    _numericFactorNeeded = false;
+  }
+
   // x = M^{-1} b
   void solve(VECTOR& x, VECTOR& b) override {
     if (!_factor) throw Error(RSC_BAD_INPUT, "FactorPreconditioner::solve before initialize");

@@ -33,7 +33,8 @@ def lib():
         L = C.CDLL(LIB)
         L.oracle_factor.restype = C.c_int
         L.oracle_factor.argtypes = [C.c_int, _IP, _IP, _DP, _IP, C.c_int, _IP, _IP, _IP,
-                                    _IP, C.POINTER(OracleOpts), C.POINTER(C.c_void_p)]
+                                    _IP, C.POINTER(OracleOpts), C.POINTER(C.c_void_p),
+                                    _IP, _DP]
         L.oracle_attempts.restype = C.c_int
         L.oracle_attempts.argtypes = [C.c_void_p]
         L.oracle_apply.restype = None
@@ -66,6 +67,15 @@ def _d(a):
     return a, a.ctypes.data_as(_DP)
 
 
+class OracleIndefinite(RuntimeError):
+    """Every attempt of the retry met an indefinite diagonal block."""
+
+    def __init__(self, attempts, final_constant):
+        super().__init__("oracle factorization indefinite after %d attempts" % attempts)
+        self.attempts = attempts
+        self.final_constant = final_constant
+
+
 class OracleFactor:
     """FP64 CPU factor for a given ordering (perm, supernodes, diag blocks)."""
 
@@ -86,11 +96,15 @@ class OracleFactor:
                 _i(ptr), _i(offs if offs else [0])]
         self._keep = keep
         h = C.c_void_p()
+        att = C.c_int()
+        fc = C.c_double()
         st = lib().oracle_factor(self.n, keep[0][1], keep[1][1], keep[2][1], keep[3][1],
                                  len(first) - 1, keep[4][1], keep[5][1], keep[6][1],
-                                 keep[7][1], C.byref(o), C.byref(h))
+                                 keep[7][1], C.byref(o), C.byref(h), C.byref(att),
+                                 C.byref(fc))
+        self.final_constant = fc.value
         if st != 0:
-            raise RuntimeError("oracle factorization failed (indefinite after retries)")
+            raise OracleIndefinite(att.value, fc.value)
         self._h = h
 
     def __del__(self):

@@ -604,10 +604,14 @@ struct oracle_opts {
 
 // diag_ptr[nsuper+1] indexes diag_off: node s has offsets
 // diag_off[diag_ptr[s] .. diag_ptr[s+1]) (empty or count+1 entries).
+// Returns 0 and a handle, or 1 when every attempt of the retry hit an
+// indefinite diagonal block.  attempts_out (may be null) receives the number
+// of numeric factorizations run and final_constant_out (may be null) the
+// diagonal rank constant of the last one, in both cases.
 int oracle_factor(int n, const int* colptr, const int* rowind, const double* values,
                   const int* perm, int nsuper, const int* first, const int* compressed,
                   const int* diag_ptr, const int* diag_off, const oracle_opts* o,
-                  void** handle) {
+                  void** handle, int* attempts_out, double* final_constant_out) {
   std::unique_ptr<Oracle> O(new Oracle());
   O->n = n;
   O->Ap.assign(colptr, colptr + n + 1);
@@ -661,6 +665,8 @@ int oracle_factor(int n, const int* colptr, const int* rowind, const double* val
     std::vector<int> offs(diag_off + diag_ptr[s], diag_off + diag_ptr[s + 1]);
     build_tree(x, offs, o->explicit_threshold);
   }
+  // indefinite-diagonal retry (SparseSolver.cpp:174-218): the diagonal rank
+  // constant grows by retry_alpha per failed attempt, ordering kept
   double dconst = o->diagonal_rank_constant;
   int attempt = 0;
   for (;;) {
@@ -669,9 +675,15 @@ int oracle_factor(int n, const int* colptr, const int* rowind, const double* val
     for (auto& x : O->nodes)
       if (x.compress) set_ranks(x, O->opt);
     if (numeric(*O) == 0) break;
-    if (attempt >= o->max_attempts) return 1;
+    if (attempt >= o->max_attempts) {
+      if (attempts_out) *attempts_out = attempt;
+      if (final_constant_out) *final_constant_out = dconst;
+      return 1;
+    }
     dconst *= o->retry_alpha;
   }
+  if (attempts_out) *attempts_out = attempt;
+  if (final_constant_out) *final_constant_out = dconst;
   O->attempts = attempt;
   *handle = O.release();
   return 0;

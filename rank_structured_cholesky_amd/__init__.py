@@ -25,6 +25,7 @@ DECOMPOSE_STANDARD, DECOMPOSE_TRANSPOSE = 0, 1
 DECOMPOSE_SCHUR, DECOMPOSE_FACTOR = 0, 1
 
 __all__ = ["SparseColumnMatrix", "FactorPreconditioner", "PCG_SOLVER", "solve",
+           "read_reference_ordering", "write_reference_ordering", "nested_dissection",
            "poisson2d", "poisson3d", "elasticity3d", "uniform_rhs", "pcg_pysolve",
            "CHOLESKY", "LDL", "EXTENDED_VARIABLE", "IN_PLACE", "DECOMPOSE_STANDARD",
            "DECOMPOSE_TRANSPOSE", "DECOMPOSE_SCHUR", "DECOMPOSE_FACTOR",
@@ -112,6 +113,94 @@ def write_positions(path, xyz):
         f.write(xyz.astype("<f8").tobytes())
 
 
+# The reference's ordering dumps (readVector format: int32 count, then the
+# records): SparseSolver_permutation.vector (SparseSolver.cpp:136, the final
+# permutation, perm[new] = old), separatorParents.vector and
+# separators.vector (NestedDissection.cpp:394-395; one {size, first, last,
+# numChildren} int32 record per component, CHOLMOD_Environment.h:233-240).
+_SEP_RECORD = np.dtype([("size", "<i4"), ("first", "<i4"), ("last", "<i4"),
+                        ("numChildren", "<i4")])
+
+
+def _read_records(path, dtype):
+    with open(path, "rb") as f:
+        (n,) = struct.unpack("<i", f.read(4))
+        data = f.read(n * np.dtype(dtype).itemsize)
+    if len(data) != n * np.dtype(dtype).itemsize:
+        raise ValueError("truncated file " + path)
+    return np.frombuffer(data, dtype=dtype).copy()
+
+
+def _write_records(path, a):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", len(a)))
+        f.write(a.tobytes())
+
+
+def read_reference_ordering(directory="."):
+    """A reference run's ordering dumps -> (perm, cparent, cmember), the
+    nested-dissection triple of FactorPreconditioner.initialize(ordering=)
+    and rsc_factor_create_with_ordering.  cmember is rebuilt from the
+    components' column ranges.  The dumped permutation already carries the
+    reference's separator sub-ordering: factor it with
+    reorderSeparators=False."""
+    import os
+    perm = _read_records(os.path.join(directory, "SparseSolver_permutation.vector"), "<i4")
+    cparent = _read_records(os.path.join(directory, "separatorParents.vector"), "<i4")
+    seps = _read_records(os.path.join(directory, "separators.vector"), _SEP_RECORD)
+    n, nc = len(perm), len(cparent)
+    if len(seps) != nc:
+        raise ValueError("separator and parent counts differ")
+    cmember = np.full(n, -1, np.int32)
+    nxt = 0
+    for c, s in enumerate(seps):
+        if (s["first"] != nxt or s["last"] - s["first"] + 1 != s["size"] or s["last"] >= n
+                or not -1 <= cparent[c] < nc):
+            raise ValueError("separator ranges do not tile the permutation")
+        cols = perm[s["first"]:s["last"] + 1]
+        if np.any(cmember[cols] != -1):
+            raise ValueError("not a permutation")
+        cmember[cols] = c
+        nxt = s["last"] + 1
+    if nxt != n or np.any(cmember < 0):
+        raise ValueError("separator ranges do not cover the permutation")
+    return perm, cparent, cmember
+
+
+def write_reference_ordering(directory, perm, cparent, cmember):
+    """Writes (perm, cparent, cmember) as the reference's three dumps."""
+    import os
+    perm, cparent, cmember = as_i32(perm), as_i32(cparent), as_i32(cmember)
+    comp = cmember[perm]
+    if np.any(np.diff(comp) < 0):
+        raise ValueError("components are not contiguous in perm")
+    nc = len(cparent)
+    seps = np.zeros(nc, _SEP_RECORD)
+    counts = np.bincount(comp, minlength=nc)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    seps["size"] = counts
+    seps["first"] = starts
+    seps["last"] = starts + counts - 1
+    seps["numChildren"] = np.bincount(cparent[cparent >= 0], minlength=nc)
+    _write_records(os.path.join(directory, "SparseSolver_permutation.vector"), perm)
+    _write_records(os.path.join(directory, "separatorParents.vector"), cparent)
+    _write_records(os.path.join(directory, "separators.vector"), seps)
+
+
+def nested_dissection(A, positions=None, nd_small=200):
+    """This library's nested dissection: (perm, cparent, cmember) in
+    CHOLMOD's format (CHOLMOD_Environment.cpp:326-432)."""
+    n = A._nrow
+    perm = np.empty(n, np.int32)
+    cparent = np.empty(n, np.int32)
+    cmember = np.empty(n, np.int32)
+    nc = C.c_int()
+    pos = None if positions is None else as_f64(np.asarray(positions).reshape(-1, 3))
+    check(lib().rsc_nested_dissection(n, iptr(A._p), iptr(A._i), dptr(pos), int(nd_small),
+                                      iptr(perm), C.byref(nc), iptr(cparent), iptr(cmember)))
+    return perm, cparent[:nc.value].copy(), cmember
+
+
 def _matrix_from_handle(h):
     L = lib()
     n = C.c_int()
@@ -187,11 +276,15 @@ class FactorPreconditioner:
     def initialize(self, matrix, meshPositions=None, maxDenseBlock=200,
                    maxDiagonalBlock=400, extFactorType=CHOLESKY,
                    compressionType=IN_PLACE, blockType=DECOMPOSE_TRANSPOSE,
-                   decompType=DECOMPOSE_FACTOR, reorderSeparators=True, **options):
+                   decompType=DECOMPOSE_FACTOR, reorderSeparators=True, ordering=None,
+                   **options):
         """Symbolic + numeric factorization (FactorPreconditioner.cpp:70-131).
 
-        Extra keyword options set fields of rsc_options (e.g.
-        factor_precision=RSC_FP32, max_rank=64, explicit_block_threshold=...)."""
+        ordering: optional (perm, cparent, cmember) nested dissection in
+        CHOLMOD's format (e.g. read_reference_ordering()) used instead of
+        this library's own.  Extra keyword options set fields of rsc_options
+        (e.g. factor_precision=RSC_FP32, max_rank=64,
+        explicit_block_threshold=...)."""
         if not isinstance(matrix, SparseColumnMatrix):
             matrix = SparseColumnMatrix.from_scipy(matrix)
         o = default_options(max_dense_block=int(maxDenseBlock),
@@ -204,7 +297,18 @@ class FactorPreconditioner:
         pos = None
         if meshPositions is not None and len(meshPositions):
             pos = as_f64(np.asarray(meshPositions).reshape(-1, 3))
-        if (self._h and not self._symbolicFactorNeeded and self._numericFactorNeeded
+        if ordering is not None:
+            perm, cparent, cmember = (as_i32(a) for a in ordering)
+            if len(perm) != matrix._nrow or len(cmember) != matrix._nrow:
+                raise ValueError("ordering size differs from the matrix")
+            self.clearSystem()
+            h = C.c_void_p()
+            check(lib().rsc_factor_create_with_ordering(
+                matrix._nrow, iptr(matrix._p), iptr(matrix._i), dptr(matrix._x), dptr(pos),
+                iptr(perm), len(cparent), iptr(cparent), iptr(cmember), C.byref(o),
+                C.byref(h)))
+            self._h = h
+        elif (self._h and not self._symbolicFactorNeeded and self._numericFactorNeeded
                 and self._matrix is not None and self._matrix._nzmax == matrix._nzmax
                 and np.array_equal(self._matrix._p, matrix._p)
                 and np.array_equal(self._matrix._i, matrix._i)):

@@ -55,9 +55,27 @@ def build(jobs=8, verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+CLI = os.path.join(HERE, "pcg_solver")
+
+
+def build_cli():
+    """The pcg_solver driver (reference src/pcg_solver.cpp) over librsc.so."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", inc,
+           os.path.join(CSRC, "..", "cli", "pcg_solver.cpp"), "-o", tmp, "-L", HERE, "-lrsc",
+           "-lz", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"pcg_solver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -103,9 +103,13 @@ rsc_status check_ranks(const rsc::Plan& plan) {
   return RSC_OK;
 }
 
+// Numeric factorization with the indefinite-diagonal retry of
+// SparseSolver::factorNumeric (SparseSolver.cpp:174-218).  Every call starts
+// from the constant the plan currently holds (f->diag_const): a constant
+// raised by an earlier retry is kept for later refactorizations, as the
+// reference keeps _diagonalRankConstant in its SparseSolver.
 rsc_status factor_numeric(rsc_factor* f, const double* values) {
   auto t0 = clk::now();
-  double dconst = f->plan.opt.diagonal_rank_constant;
   int attempt = 0;
   for (;;) {
     ++attempt;
@@ -115,20 +119,21 @@ rsc_status factor_numeric(rsc_factor* f, const double* values) {
     } catch (const rsc::IndefiniteError& e) {
       if (attempt >= f->plan.opt.max_attempts) {
         f->info.attempts = attempt;
+        f->info.final_diagonal_rank_constant = f->diag_const;
         return fail(RSC_INDEFINITE, e.what());
       }
-      dconst *= f->plan.opt.retry_alpha;
+      f->diag_const *= f->plan.opt.retry_alpha;
       if (f->plan.opt.verbose)
-        std::printf("Increasing rank constant to %f\n", dconst);
+        std::printf("Increasing rank constant to %f\n", f->diag_const);
       f->eng.reset();
-      rsc::replan_ranks(f->plan, dconst);
+      rsc::replan_ranks(f->plan, f->diag_const);
       rsc_status st = check_ranks(f->plan);
       if (st != RSC_OK) return st;
       f->eng = rsc::make_engine(f->plan, f->A, f->stream);
     }
   }
   f->info.attempts = attempt;
-  f->info.final_diagonal_rank_constant = dconst;
+  f->info.final_diagonal_rank_constant = f->diag_const;
   f->info.numeric_seconds = std::chrono::duration<double>(clk::now() - t0).count();
   f->info.factor_bytes = f->eng->factor_bytes();
   f->info.workspace_bytes = f->eng->workspace_bytes();
@@ -164,6 +169,7 @@ rsc_status create_common(int n, const int* colptr, const int* rowind,
     }
     auto t0 = clk::now();
     rsc::build_plan(f->A, f->has_xyz ? f->xyz.data() : nullptr, o, nd, f->plan);
+    f->diag_const = o.diagonal_rank_constant;
     f->info.symbolic_seconds = std::chrono::duration<double>(clk::now() - t0).count();
     rsc_status s2 = check_ranks(f->plan);
     if (s2 != RSC_OK) return s2;
@@ -256,6 +262,15 @@ rsc_status rsc_apply(rsc_factor* f, int nrhs, const double* B, int ldb, double* 
     hipStream_t s = stream ? (hipStream_t)stream : f->stream;
     if (nrhs > 0) f->eng->apply(nrhs, B, ldb, X, ldx, s);
     RSC_HIP(hipGetLastError());
+    return RSC_OK;
+  });
+}
+
+rsc_status rsc_apply_status(rsc_factor* f, void* stream) {
+  if (!f) return fail(RSC_BAD_INPUT, "null factor");
+  return guard([&]() -> rsc_status {
+    hipStream_t s = stream ? (hipStream_t)stream : f->stream;
+    f->eng->check_apply(s);
     return RSC_OK;
   });
 }

@@ -22,4 +22,8 @@ struct rsc_factor {
   rsc::DevCsr dA;
   rsc::PcgWork pcgw;
   rsc_factor_info info{};
+  // diagonal rank constant the plan's ranks were built with; it persists
+  // across numeric factorizations, as SparseSolver::_diagonalRankConstant
+  // does (SparseSolver.cpp:174-218)
+  double diag_const = 0.0;
 };

@@ -101,6 +101,11 @@ class FactorEngine {
                      const int* live = nullptr, int multi_min_k = 2) = 0;
   // Synchronizes s and throws DeviceError if an apply on it failed.
   virtual void check_apply(hipStream_t s) = 0;
+  // Device address of the apply's status words (two ints, both 0 while every
+  // apply so far succeeded), for callers that read it asynchronously.
+  virtual const int* apply_status_device() const = 0;
+  // Throws DeviceError for status words read from apply_status_device().
+  virtual void raise_apply_status(const int* words) = 0;
   virtual int64_t factor_bytes() const = 0;
   virtual int64_t workspace_bytes() const = 0;
   virtual int pivot_replacements() const = 0;

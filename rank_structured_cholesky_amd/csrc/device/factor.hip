@@ -128,6 +128,8 @@ class Engine final : public FactorEngine {
   void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
              const int* live, int multi_min_k) override;
   void check_apply(hipStream_t s) override;
+  const int* apply_status_device() const override { return dapst_.as<int>(); }
+  void raise_apply_status(const int* words) override;
   int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
   int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
   int pivot_replacements() const override { return pivot_replacements_; }
@@ -1834,18 +1836,25 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
 }
 
 template <typename T>
+void Engine<T>::raise_apply_status(const int* st) {
+  if (st[0] == 0 && st[1] == 0) return;
+  // clear the words so the next apply starts clean, then report
+  const int z[2] = {0, 0};
+  RSC_HIP(hipMemcpy(dapst_.p, z, sizeof(z), hipMemcpyHostToDevice));
+  throw DeviceError(st[0] == 2 ? "preconditioner apply: subtree task in the multi-column list"
+                               : "preconditioner apply: a dependency wait timed out");
+}
+
+template <typename T>
 void Engine<T>::check_apply(hipStream_t s) {
-  if (!use_dataflow_) return;
+  if (!use_dataflow_) {
+    RSC_HIP(hipStreamSynchronize(s));
+    return;
+  }
   int st[2] = {0, 0};
   RSC_HIP(hipMemcpyAsync(st, dapst_.p, sizeof(st), hipMemcpyDeviceToHost, s));
   RSC_HIP(hipStreamSynchronize(s));
-  if (st[0] != 0 || st[1] != 0) {
-    std::vector<int> z = {0, 0};
-    dapst_.upload(z, s);
-    RSC_HIP(hipStreamSynchronize(s));
-    throw DeviceError(st[0] == 2 ? "preconditioner apply: subtree task in the multi-column list"
-                                 : "preconditioner apply: a dependency wait timed out");
-  }
+  raise_apply_status(st);
   if (dtrace_.p) {
     // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each
     const std::vector<dev::DfTask>& df_host_ = df_[df_last_].host;

@@ -39,6 +39,7 @@ struct Request {
 
 struct Retire {
   hipEvent_t ev = nullptr;
+  int* apst = nullptr;  // pinned: the apply status words at retirement
   std::vector<std::pair<Request*, int>> cols;
 };
 
@@ -58,6 +59,7 @@ struct Lane {
   bool admitted = false;
   std::deque<Retire> retq;
   std::vector<hipEvent_t> evpool;
+  std::vector<int*> stpool;  // pinned pairs of status words
 
   int kact() const {
     for (int s = kcap - 1; s >= 0; --s)
@@ -66,6 +68,16 @@ struct Lane {
   }
   bool idle() const {
     return occupied == 0 && queue.empty() && retq.empty() && steps == harvested;
+  }
+  int* status_words() {
+    if (!stpool.empty()) {
+      int* p = stpool.back();
+      stpool.pop_back();
+      return p;
+    }
+    void* p = nullptr;
+    RSC_HIP(hipHostMalloc(&p, 2 * sizeof(int), hipHostMallocDefault));
+    return (int*)p;
   }
   hipEvent_t event() {
     if (!evpool.empty()) {
@@ -127,6 +139,11 @@ void retire_slots(rsc_server& S, Lane& L, const std::vector<int>& slots) {
       L.slot_req[s] = nullptr;
       --L.occupied;
     }
+    // the apply's status words travel with the retirement: a column is
+    // returned only when every apply that produced it succeeded
+    rt.apst = L.status_words();
+    RSC_HIP(hipMemcpyAsync(rt.apst, L.f->eng->apply_status_device(), 2 * sizeof(int),
+                           hipMemcpyDeviceToHost, L.st));
     rt.ev = L.event();
     RSC_HIP(hipEventRecord(rt.ev, L.st));
     L.retq.push_back(std::move(rt));
@@ -143,6 +160,9 @@ bool rsc_server::service(Lane& L) {
   // finished retirements, in order
   while (!L.retq.empty() && hipEventQuery(L.retq.front().ev) == hipSuccess) {
     Retire& rt = L.retq.front();
+    L.f->eng->raise_apply_status(rt.apst);  // throws: the server fails
+    L.stpool.push_back(rt.apst);
+    rt.apst = nullptr;
     {
       std::lock_guard<std::mutex> lk(mu);
       for (auto& rc : rt.cols) {
@@ -263,8 +283,12 @@ rsc_status rsc_server_create(rsc_factor* const* factors, int nfactors,
   if (!out) return fail(RSC_BAD_INPUT, "out is null");
   *out = nullptr;
   if (!factors || nfactors <= 0) return fail(RSC_BAD_INPUT, "no factors");
-  for (int i = 0; i < nfactors; ++i)
+  for (int i = 0; i < nfactors; ++i) {
     if (!factors[i]) return fail(RSC_BAD_INPUT, "null factor");
+    // two lanes on one factor would share its apply buffers and counters
+    for (int j = 0; j < i; ++j)
+      if (factors[j] == factors[i]) return fail(RSC_BAD_INPUT, "factor passed twice");
+  }
   rsc_server_options o;
   if (opt) o = *opt;
   else rsc_server_options_default(&o);
@@ -388,7 +412,11 @@ void rsc_server_destroy(rsc_server* s) {
   for (auto& L : s->lanes) {
     (void)hipStreamSynchronize(L->st);
     for (hipEvent_t e : L->evpool) (void)hipEventDestroy(e);
-    for (auto& rt : L->retq) (void)hipEventDestroy(rt.ev);
+    for (int* p : L->stpool) (void)hipHostFree(p);
+    for (auto& rt : L->retq) {
+      (void)hipEventDestroy(rt.ev);
+      if (rt.apst) (void)hipHostFree(rt.apst);
+    }
     for (hipEvent_t e : L->sev)
       if (e) (void)hipEventDestroy(e);
     if (L->snap) (void)hipHostFree(L->snap);

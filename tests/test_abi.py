@@ -122,7 +122,14 @@ def test_server_options_and_bad_input():
     assert L.rsc_server_create(None, 0, C.byref(o), C.byref(h)) == _lib.RSC_BAD_INPUT
     none = (C.c_void_p * 1)(None)
     assert L.rsc_server_create(none, 1, C.byref(o), C.byref(h)) == _lib.RSC_BAD_INPUT
+    # the same factor twice: rejected before any handle is dereferenced (the
+    # two lanes would share one factor's apply buffers)
+    fake = C.c_void_p(0x1000)
+    twice = (C.c_void_p * 2)(fake, fake)
+    assert L.rsc_server_create(twice, 2, C.byref(o), C.byref(h)) == _lib.RSC_BAD_INPUT
+    assert b"twice" in L.rsc_last_error()
     t = C.c_int64()
     assert L.rsc_server_submit(None, 0, 1, None, 1, None, 1, C.byref(t)) == _lib.RSC_BAD_INPUT
     assert L.rsc_server_wait(None, 1, None) == _lib.RSC_BAD_INPUT
+    assert L.rsc_apply_status(None, None) == _lib.RSC_BAD_INPUT
     L.rsc_server_destroy(None)

@@ -213,3 +213,26 @@ def test_unpreconditioned_spmv():
     o = oracle_for(A, sym)
     x = np.random.default_rng(5).standard_normal(A._nrow)
     assert relerr(o.spmv(x), A.to_scipy() @ x) < 1e-14
+
+
+def test_oracle_retry_schedule_on_indefinite_matrix():
+    """The oracle's restatement of the indefinite-diagonal retry
+    (SparseSolver.cpp:174-218): on A - 3 I (symmetric indefinite: a pivot
+    fails at every rank) exactly max_attempts factorizations run and the
+    diagonal rank constant of the last one is 0.5 * 1.25^(attempts-1); on
+    the SPD matrix one attempt at 0.5."""
+    from oracle.oracle import OracleIndefinite
+    A, xyz = R.poisson3d(10, 1.0, 1234)
+    x = A._x.copy()
+    for j in range(A._nrow):
+        lo, hi = A._p[j], A._p[j + 1]
+        x[lo + int(np.nonzero(A._i[lo:hi] == j)[0][0])] -= 3.0
+    Ai = R.SparseColumnMatrix(A._nrow, A._ncol, A._p, A._i, x)
+    sym = symbolic(A, xyz, max_dense_block=20, max_diagonal_block=40)
+    for attempts in (1, 4, 10):
+        with pytest.raises(OracleIndefinite) as e:
+            oracle_for(Ai, sym, max_attempts=attempts)
+        assert e.value.attempts == attempts
+        assert e.value.final_constant == pytest.approx(0.5 * 1.25 ** (attempts - 1))
+    O = oracle_for(A, sym)
+    assert O.attempts == 1 and O.final_constant == 0.5

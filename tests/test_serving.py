@@ -19,11 +19,15 @@ EPS = 1e-8
 
 
 def _factors():
+    """Four resident systems of different sizes: three FP64 factors and one
+    mixed (FP32 factor, BF16 sketches)."""
     out = []
-    for N, sigma in ((16, 1.0), (20, 1.5)):
+    for N, sigma, extra in ((16, 1.0, {}), (20, 1.5, {}), (12, 1.0, {}),
+                            (14, 2.0, dict(factor_precision=R.RSC_FP32,
+                                           sketch_precision=R.RSC_BF16))):
         A, xyz = R.poisson3d(N, sigma, 1234)
         M = R.FactorPreconditioner()
-        M.initialize(A, xyz, 63, 400)
+        M.initialize(A, xyz, 63, 400, **extra)
         out.append((A, M))
     return out
 
@@ -47,7 +51,8 @@ def test_served_results_equal_solo_solves():
     fac = _factors()
     # (factor, number of right-hand sides): more columns than slots in total,
     # one request larger than the slot pool
-    reqs = [(0, 3), (1, 5), (0, 12), (1, 1), (0, 2), (1, 7), (0, 1)]
+    reqs = [(0, 3), (1, 5), (2, 4), (0, 12), (3, 6), (1, 1), (2, 9), (0, 2), (3, 1),
+            (1, 7), (0, 1)]
     rng_seed = 100
     data = []
     for f, m in reqs:
@@ -82,7 +87,7 @@ def test_served_results_equal_solo_solves():
     mixed, st = serve(list(range(len(reqs))), stagger=True)
     assert st["max_busy"] > 1
     alone = {}
-    for i in (2, 3, 6):
+    for i in (3, 4, 5, 6, 8):
         o, _ = serve([i], stagger=False)
         alone.update(o)
     reversed_mix, _ = serve(list(reversed(range(len(reqs)))), stagger=False)

@@ -165,6 +165,12 @@ class Engine final : public FactorEngine {
     });
   }
   void upload_rng() {
+    // a compressed supernode may draw nothing (a single diagonal leaf and no
+    // off-diagonal rows): then there is no stream to copy
+    if (plan_.rng_count <= 0) {
+      rng_on_device_ = true;
+      return;
+    }
     start_rng();
     if (rng_thread_.joinable()) rng_thread_.join();
     if (rng_err_) {
@@ -552,6 +558,11 @@ void Engine<T>::trsm_right(const std::vector<TrsmMat>& mats) {
 
 // Thin QR by shifted CholeskyQR3 (Fukaya et al. 2020: one shifted pass, then
 // CholeskyQR2): each Y (m x r) becomes an orthonormal basis of its range.
+// A column that is numerically dependent on the previous ones (a rank-
+// deficient sketch: the block has fewer directions than the rank asks for)
+// comes out zero instead of as amplified rounding noise, so U U^T is always
+// an orthogonal projector (of lower rank there), which keeps every
+// compressed update within the exact one and the factor SPD.
 struct QrMat {
   char* Y;
   int m, r;
@@ -602,7 +613,9 @@ void Engine<T>::qr(const std::vector<QrMat>& jobs0) {
       g.beta = 0.0;
       gram.push_back(g);
       sh.push_back({J.W, J.r, J.r, J.m, 0});
-      ch.push_back({J.W, J.r, J.r, J.r, J.node, 1});
+      // pass 0 is shifted (pivots stay positive); passes 1 and 2 see the
+      // Gram of columns of norm <= 1 and drop dependent ones
+      ch.push_back({J.W, J.r, J.r, J.r, J.node, pass == 0 ? 1 : 3});
       tr.push_back({J.Y, J.m, J.m, J.W, J.r, J.r});
     }
     emit_gemm(OP_GEMM_TN, gram, false);

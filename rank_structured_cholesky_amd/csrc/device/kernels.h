@@ -44,6 +44,11 @@ int plan_gemm_tiles(const GemmJob* jobs, int njobs, bool lower,
 //   1  (Gram matrices of the thin QR) it is replaced by u * max(tile diagonal)
 //   2  (FP32 factor) it is replaced by 1e-5 * max(tile diagonal) and counted
 //      in status[2]; L keeps a positive diagonal, so L L^T stays SPD
+//   3  (Gram matrices of the unshifted thin-QR passes, whose columns have
+//      norm <= 1) a pivot <= 1e-13 marks a dependent column: it is replaced
+//      by a huge value, so that column of L and of the orthonormalized
+//      matrix come out (numerically) zero: the basis drops the direction
+//      instead of amplifying rounding noise into it
 struct PotrfJob {
   void* A;
   int n, lda, node, tolerant;

@@ -297,6 +297,11 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfJob* __restrict__ jobs
             d = dfloor;
             bad += 1;
           }
+        } else if (J.tolerant == 3) {
+          // Gram matrix of columns of norm <= 1: a pivot at rounding level
+          // is a column dependent on the previous ones; an infinite-like
+          // pivot zeroes it in L and in the triangular solve that follows
+          if (!(d > (T)1e-13)) d = sizeof(T) == 8 ? (T)1e300 : (T)1e30;
         } else if (!(d > (T)0)) {
           if (J.tolerant) {
             d = dfloor;

@@ -1,14 +1,20 @@
 """Benchmark: PCG solves/s to 1e-8 relative residual with the rank-structured
-preconditioner resident in HBM, on one MI355X.
+preconditioner resident in HBM, mixed (FP32 dense factor + BF16 sketches)
+against the FP64 factor, on one MI355X.
 
-Workload (default): 3D 7-point variable-coefficient Poisson, 64^3 cells
-(n = 262,144), log-normal face coefficients (mu 0, sigma 1, seed 1234),
-Dirichlet boundary; separators with more than 255 unknowns compressed,
-ranks capped at 64; one uniform[-1,1] right-hand side (seed 5678).
-A step = one PCG solve (recurrence relres <= 1e-8) of that right-hand side
-with the factor and all vectors already in HBM.  The FP64 factor is the
-headline; the mixed variant (FP32 dense factor, BF16 range-finder products,
-FP64 PCG) is reported beside it.
+Default workload (poisson3d_128): 3D 7-point variable-coefficient Poisson,
+128^3 cells (n = 2,097,152), log-normal face coefficients (mu 0, sigma 2,
+seed 1234), Dirichlet boundary; separators with more than 255 unknowns
+compressed at the reference's rank heuristic with oversampling 10; 16
+uniform[-1,1] right-hand sides (seed 5678); FP64 block PCG to 1e-8.
+
+A step = one block PCG solve of the 16 right-hand sides (each column to
+recurrence relres <= 1e-8) with the factor and all vectors already in HBM.
+`value` is the mixed factor's solves/s (right-hand sides per second); the
+FP64 factor's is reported beside it, with both factorization times, both
+end-to-end times (symbolic + numeric factor + one solve) and their ratio,
+and each solution's relative error against an FP64 direct solve (the exact
+Cholesky factor on the device, no compression, refined by PCG to 1e-14).
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: every
 configuration here runs on one GPU.
@@ -19,6 +25,7 @@ import json
 import os
 import statistics
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -27,13 +34,22 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 CONFIGS = {
-    "poisson3d_64": dict(N=64, sigma=1.0, max_dense_block=255, max_rank=64, nrhs=1),
-    "poisson3d_128": dict(N=128, sigma=2.0, max_dense_block=255, max_rank=0, nrhs=16),
-    "poisson2d_256": dict(N2=256, max_dense_block=127, max_rank=32, nrhs=1),
+    # the mixed-precision + level-batched form at 128^3 (headline)
+    "poisson3d_128": dict(N=128, sigma=2.0, max_dense_block=255, max_rank=0, oversampling=10,
+                          nrhs=16, cpu_sample_N=48),
+    # plain FP64 GPU form against the CPU path
+    "poisson3d_64": dict(N=64, sigma=1.0, max_dense_block=255, max_rank=64, oversampling=5,
+                         nrhs=1, cpu_sample_N=48),
+    # plain method, 2D
+    "poisson2d_256": dict(N2=256, max_dense_block=127, max_rank=32, oversampling=5, nrhs=1,
+                          cpu_sample_N=256),
+    # mixed multi-right-hand-side elasticity
+    "elasticity_48": dict(NE=48, sigma=1.0, nu=0.3, max_dense_block=255, max_rank=96,
+                          oversampling=5, nrhs=64, cpu_sample_N=16),
     # request serving over eight resident factors (continuous batching)
     "serving": dict(grids=(32, 48, 64, 80, 96, 96, 112, 128), sigma=1.0,
-                    max_dense_block=255, max_rank=64, requests=24, max_rhs=256,
-                    rate=20.0, slots=32),
+                    max_dense_block=255, max_rank=64, oversampling=5, requests=24,
+                    max_rhs=256, rate=20.0, slots=32),
 }
 
 HBM_PEAK_GBS = 8000.0
@@ -42,12 +58,14 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="poisson3d_64", choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="poisson3d_128", choices=sorted(CONFIGS))
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed FP32/BF16-factor leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor leg")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the error against the exact (direct) factor")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving: number of requests (default from the config)")
     ap.add_argument("--max-rhs", type=int, default=0,
@@ -57,8 +75,8 @@ def parse():
     ap.add_argument("--traffic-run", action="store_true",
                     help="for counter passes: factor, 3 applies, then 3 plain 256 MiB "
                          "copies as the calibration kernel; no timing line")
-    ap.add_argument("--mixed-factor", action="store_true",
-                    help="--traffic-run with the mixed factor (FP32 + BF16 sketches)")
+    ap.add_argument("--fp64-factor", action="store_true",
+                    help="--traffic-run with the FP64 factor (default: the mixed factor)")
     return ap.parse_args()
 
 
@@ -170,12 +188,11 @@ def serving_leg(cfg, eps):
         n=[m._nrow for m in mats])
 
 
-def traffic_run(cfg, precision=0):
+def traffic_run(cfg, precision):
     """The program the FETCH_SIZE / WRITE_SIZE counter passes profile: the
     apply dispatches (k_apply_df) and a copy kernel of known size."""
     import torch
 
-    import rank_structured_cholesky_amd as R  # noqa: F401
     from rank_structured_cholesky_amd import _lib
     L = _lib.lib()
     A, xyz, B = build_problem(cfg)
@@ -188,44 +205,64 @@ def traffic_run(cfg, precision=0):
         _lib.check(L.rsc_apply(M.handle, k, C.c_void_p(dB.data_ptr()), n,
                                C.c_void_p(dY.data_ptr()), n,
                                C.c_void_p(stream.cuda_stream)))
+    _lib.check(L.rsc_apply_status(M.handle, C.c_void_p(stream.cuda_stream)))
     torch.cuda.synchronize()
     src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    vec = 56.0 * n
-    print(json.dumps({"traffic_run": True,
-                      "apply_bytes": M.info()["apply_bytes"] - vec + vec * k,
-                      "nrhs": k}), flush=True)
+    print(json.dumps({"traffic_run": True, "apply_bytes": apply_bytes(M, A, k), "nrhs": k,
+                      "factor": "FP64" if precision == 0 else "mixed"}), flush=True)
 
 
-def build_problem(cfg):
+def build_problem(cfg, N=None):
+    """The workload's matrix, positions and right-hand sides (n x nrhs);
+    N overrides the grid size (the CPU baseline's sample)."""
     import rank_structured_cholesky_amd as R
     if "N2" in cfg:
-        A, xyz = R.poisson2d(cfg["N2"])
+        A, xyz = R.poisson2d(N or cfg["N2"])
+    elif "NE" in cfg:
+        A, xyz = R.elasticity3d(N or cfg["NE"], cfg["sigma"], cfg["nu"], 1234)
     else:
-        A, xyz = R.poisson3d(cfg["N"], cfg["sigma"], 1234)
+        A, xyz = R.poisson3d(N or cfg["N"], cfg["sigma"], 1234)
     B = R.uniform_rhs(A._nrow * cfg["nrhs"], 5678).reshape(cfg["nrhs"], A._nrow).T
     return A, xyz, np.asfortranarray(B)
 
 
-def factor(A, xyz, cfg, precision):
+def options(cfg, precision):
     """precision 0: FP64 factor; 1: the mixed form (FP32 dense factor, BF16
-    range-finder products with FP32 accumulation)."""
+    range-finder products with FP32 accumulation); 2: the exact factor (no
+    compression, FP64): a direct solver."""
+    import rank_structured_cholesky_amd as R
+    o = dict(max_rank=cfg["max_rank"], oversampling=cfg["oversampling"])
+    if precision == 1:
+        o.update(factor_precision=R.RSC_FP32, sketch_precision=R.RSC_BF16)
+    return o
+
+
+def factor(A, xyz, cfg, precision):
     import rank_structured_cholesky_amd as R
     M = R.FactorPreconditioner()
-    extra = dict(sketch_precision=R.RSC_BF16) if precision == 1 else {}
+    mdb = 2 ** 30 if precision == 2 else cfg["max_dense_block"]
     t0 = time.perf_counter()
-    M.initialize(A, xyz, cfg["max_dense_block"], 400, max_rank=cfg["max_rank"],
-                 factor_precision=precision, **extra)
+    M.initialize(A, xyz, mdb, 400, **options(cfg, precision))
     return M, time.perf_counter() - t0
 
 
-def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps):
+def apply_bytes(M, A, k):
+    """Algorithmic bytes of one apply with k right-hand sides: every stored
+    factor element once per traversal (forward and backward) in the factor
+    precision, plus 56 B per row per right-hand side of vector traffic."""
+    vec = 56.0 * A._nrow
+    return M.info()["apply_bytes"] - vec + vec * k
+
+
+def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps, keep_x=False):
+    """Factor, then W untimed and K timed block PCG solves of B, then ten
+    back-to-back applies timed with HIP events on the stream they run on."""
     import torch
 
-    import rank_structured_cholesky_amd as R
     from rank_structured_cholesky_amd import _lib
     L = _lib.lib()
     M, t_factor = factor(A, xyz, cfg, precision)
@@ -233,18 +270,24 @@ def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps):
     n, k = B.shape
     dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")  # k x n rows = columns
     dX = torch.zeros_like(dB)
-    res = (_lib.PcgResult * k)()
     stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())  # dB's upload
+    per_step = []
 
     def solve():
-        dX.zero_()
+        res = (_lib.PcgResult * k)()
+        with torch.cuda.stream(stream):
+            dX.zero_()  # ordered before the solve on the solve's own stream
         st = L.rsc_block_pcg_solve(M.handle, k, C.c_void_p(dB.data_ptr()), n,
                                    C.c_void_p(dX.data_ptr()), n, eps, 10000, res,
                                    C.c_void_p(stream.cuda_stream))
-        if st not in (_lib.RSC_OK, _lib.RSC_NOT_CONVERGED):
-            _lib.check(st)
+        _lib.check(st)  # RSC_NOT_CONVERGED fails the step
+        per_step.append(([r.iterations for r in res], [r.true_relres for r in res]))
 
-    for _ in range(warmup):
+    t0 = time.perf_counter()
+    solve()
+    first_solve = time.perf_counter() - t0
+    for _ in range(max(0, warmup - 1)):
         solve()
     torch.cuda.synchronize()
     lat = []
@@ -255,8 +298,10 @@ def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps):
         lat.append(time.perf_counter() - t1)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    iters = [r.iterations for r in res]
-    relres = [r.true_relres for r in res]
+    iters, relres = per_step[-1]
+    for its, _ in per_step:
+        if its != iters:
+            raise RuntimeError("iteration counts differ between steps: %s vs %s" % (its, iters))
     # preconditioner apply timed with events on the stream it runs on
     dY = torch.empty_like(dB)
     napp = 10
@@ -271,56 +316,116 @@ def gpu_leg(A, xyz, B, cfg, precision, steps, warmup, eps):
                         C.c_void_p(dY.data_ptr()), n, C.c_void_p(stream.cuda_stream))
         e1.record(stream)
     e1.synchronize()
+    _lib.check(L.rsc_apply_status(M.handle, C.c_void_p(stream.cuda_stream)))
     apply_s = e0.elapsed_time(e1) / 1e3 / napp
-    x = dX.cpu().numpy().T
-    err = float(np.linalg.norm(A.multiply(x[:, 0]) - B[:, 0]) / np.linalg.norm(B[:, 0]))
     out = dict(t_factor=t_factor, info=info, total=total, lat=lat, iters=iters,
-               relres=relres, apply_s=apply_s, check_relres=err, k=k)
+               relres=relres, apply_s=apply_s, k=k, first_solve=first_solve,
+               apply_bytes=apply_bytes(M, A, k))
+    if keep_x:
+        out["X"] = dX
     del M
+    torch.cuda.synchronize()
     return out
 
 
-def cpu_leg(A, xyz, B, cfg, eps, full):
-    """Oracle (FP64 CPU restatement of the reference, one thread) on a bounded
-    sample of the same workload family, scaled to the full workload by the
-    ratio of per-solve algorithmic bytes (the solve is memory-bound on the
-    CPU as on the GPU)."""
+def direct_error(A, xyz, B, cfg, sols):
+    """Relative error of each solution column against an FP64 direct solve:
+    the exact supernodal Cholesky factor on the device (no compression),
+    refined by PCG with itself as preconditioner to 1e-14.  sols: name ->
+    device array (k x n).  Returns max/median error per name and the direct
+    solve's own figures."""
+    import torch
+
+    from rank_structured_cholesky_amd import _lib
+    L = _lib.lib()
+    t0 = time.perf_counter()
+    E, t_factor = factor(A, xyz, cfg, 2)
+    n, k = B.shape
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")
+    dXd = torch.zeros_like(dB)
+    res = (_lib.PcgResult * k)()
+    st = L.rsc_block_pcg_solve(E.handle, k, C.c_void_p(dB.data_ptr()), n,
+                               C.c_void_p(dXd.data_ptr()), n, 1e-14, 20, res, None)
+    if st not in (_lib.RSC_OK, _lib.RSC_NOT_CONVERGED):
+        _lib.check(st)
+    torch.cuda.synchronize()
+    info = E.info()
+    del E
+    out = {"direct": dict(factor_s=round(t_factor, 3), factor_gb=round(info["factor_bytes"] / 1e9, 2),
+                          refinement_iterations=max(r.iterations for r in res),
+                          true_relres=max(r.true_relres for r in res),
+                          wall_s=round(time.perf_counter() - t0, 3))}
+    ref = dXd.double()
+    nrm = torch.linalg.norm(ref, dim=1)
+    for name, X in sols.items():
+        e = (torch.linalg.norm(X - ref, dim=1) / nrm).cpu().numpy()
+        out[name] = dict(max=float(e.max()), median=float(np.median(e)))
+    del dXd, ref
+    torch.cuda.synchronize()
+    return out
+
+
+def cpu_leg(cfg, eps, gpu_steps=3):
+    """The oracle (FP64 CPU restatement of the reference method, built for
+    this host with -march=native and OpenMP) on a bounded sample of the
+    workload: the same options and right-hand-side count on a smaller grid
+    (cfg["cpu_sample_N"]).  Timed: symbolic + numeric factor once, then a
+    warm-up and five block solves (median), on all host threads; one
+    column's solve on one thread (the columns are independent, so one
+    thread's block solve is k times that).  The GPU runs the same sample in
+    the same process for a same-size ratio."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import oracle_for, symbolic
-    import rank_structured_cholesky_amd as R
-    N = cfg.get("N", 0)
-    if N:
-        sN = min(N, 40)
-        As, xs = R.poisson3d(sN, cfg["sigma"], 1234)
-        desc = f"poisson3d {sN}^3 (n={As._nrow}), same options"
-    else:
-        sN = min(cfg["N2"], 128)
-        As, xs = R.poisson2d(sN)
-        desc = f"poisson2d {sN}^2 (n={As._nrow}), same options"
-    bs = R.uniform_rhs(As._nrow, 5678)
+    from oracle import build as obuild
+    from oracle import oracle as OR
+    native = obuild.build_native(os.path.join(tempfile.gettempdir(), "rsc_oracle_native"))
+    OR.lib(native)
+    threads = OR.max_threads()
+    sN = cfg["cpu_sample_N"]
+    A, xyz, B = build_problem(cfg, sN)
+    k = B.shape[1]
     t0 = time.perf_counter()
-    sym = symbolic(As, xs, max_dense_block=cfg["max_dense_block"], max_rank=cfg["max_rank"])
-    O = oracle_for(As, sym, max_rank=cfg["max_rank"])
-    tf = time.perf_counter() - t0
-    reps = 3
+    sym = symbolic(A, xyz, max_dense_block=cfg["max_dense_block"], max_rank=cfg["max_rank"],
+                   oversampling=cfg["oversampling"])
+    O = oracle_for(A, sym, max_rank=cfg["max_rank"], oversampling=cfg["oversampling"])
+    t_factor = time.perf_counter() - t0
+    O.block_pcg(B, eps, 10000)
+    ts = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        _, info = O.block_pcg(B, eps, 10000)
+        ts.append(time.perf_counter() - t1)
+    t_solve = statistics.median(ts)
+    OR.set_threads(1)
     t1 = time.perf_counter()
-    for _ in range(reps):
-        _, info = O.pcg(bs, eps, 10000)
-    ts = (time.perf_counter() - t1) / reps
-
-    def spmv_bytes(M):
-        return 12.0 * M._nzmax + 4.0 * (M._nrow + 1) + 16.0 * M._nrow
-
-    def solve_bytes(iters, factor_bytes, Mx):
-        return iters * (2.0 * factor_bytes + spmv_bytes(Mx) + 80.0 * Mx._nrow) + spmv_bytes(Mx)
-
-    w_sample = solve_bytes(info["iterations"], 8.0 * sym["factor_elems"], As)
-    w_full = solve_bytes(max(full["iters"]), float(full["info"]["factor_bytes"]), A)
-    value = (1.0 / ts) * (w_sample / w_full)
-    return dict(value=value, cores=1,
-                sample=desc + "; factor %.1f s, one PCG solve %.3f s (%d it, mean of %d); "
-                "scaled by per-solve algorithmic bytes %.3g / %.3g" % (
-                    tf, ts, info["iterations"], reps, w_sample, w_full))
+    O.block_pcg(B[:, :1], eps, 10000)
+    t_one = time.perf_counter() - t1
+    OR.set_threads(threads)
+    del O
+    # the GPU on the same sample (mixed and FP64 factors)
+    same = {}
+    for prec, name in ((1, "mixed"), (0, "fp64")):
+        g = gpu_leg(A, xyz, B, cfg, prec, gpu_steps, 1, eps)
+        same[name] = dict(solves_per_s=round(gpu_steps * g["k"] / g["total"], 2),
+                          factor_s=round(g["t_factor"], 4),
+                          end_to_end_s=round(g["t_factor"] + g["total"] / gpu_steps, 4))
+    kind = "-march=native" if native else "-march=x86-64-v3 (native build failed)"
+    cpu_e2e = t_factor + t_solve
+    return dict(
+        value=k / t_solve, cores=threads, kind="port",
+        sample=("%s at %d^3 (n=%d), %d right-hand sides, same options; oracle built %s, "
+                "OpenMP %d threads: factor %.2f s, block solve %.3f s (median of 5 after a "
+                "warm-up, %d iterations); 1 thread: one column %.3f s" % (
+                    "poisson3d" if "N" in cfg else ("elasticity3d" if "NE" in cfg
+                                                    else "poisson2d"),
+                    sN, A._nrow, k, kind, threads, t_factor, t_solve,
+                    int(max(info["iterations"])), t_one)),
+        factor_s=round(t_factor, 3), solve_s=round(t_solve, 4), end_to_end_s=round(cpu_e2e, 3),
+        one_thread_solves_per_s=round(1.0 / t_one, 3),
+        gpu_same_sample=same,
+        speedup_same_sample=dict(
+            solves=round(same["mixed"]["solves_per_s"] / (k / t_solve), 1),
+            end_to_end=round(cpu_e2e / same["mixed"]["end_to_end_s"], 1)))
 
 
 def warm_up_device():
@@ -328,24 +433,32 @@ def warm_up_device():
     factor times do not include the process's first HIP module loads."""
     import rank_structured_cholesky_amd as R
     A, xyz = R.poisson3d(12, 1.0, 1)
-    M = R.FactorPreconditioner()
-    M.initialize(A, xyz, 63, 400)
-    x = np.zeros(A._nrow)
-    R.PCG_SOLVER(A, 1e-8, 100).solvePCG(x, R.uniform_rhs(A._nrow, 1), M)
-    del M
+    for extra in ({}, dict(factor_precision=R.RSC_FP32, sketch_precision=R.RSC_BF16)):
+        M = R.FactorPreconditioner()
+        M.initialize(A, xyz, 63, 400, **extra)
+        x = np.zeros((A._nrow, 2))
+        B = np.stack([R.uniform_rhs(A._nrow, 1)] * 2, axis=1)
+        R.PCG_SOLVER(A, 1e-8, 100).solvePCG(x, B, M)
+        del M
 
 
 def measured_traffic(workload, precision):
     """HBM bytes per apply (all right-hand sides of the workload) from the
-    committed counter summary of this workload
-    (profiles/apply_traffic_<workload>.md), FP64 factor only."""
-    path = os.path.join(ROOT, "profiles", "apply_traffic_%s.md" % workload)
-    if precision != 0 or not os.path.exists(path):
+    committed counter summary of this workload and factor
+    (profiles/apply_traffic_<workload>[_mixed].md)."""
+    name = "apply_traffic_%s%s.md" % (workload, "_mixed" if precision == 1 else "")
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
         return None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_apply:"):
             return int(line.split(":")[1]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def pct(v, p):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(round(p * (len(v) - 1))))]
 
 
 def main():
@@ -360,7 +473,7 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     if a.traffic_run:
-        traffic_run(cfg, 1 if a.mixed_factor else 0)
+        traffic_run(cfg, 0 if a.fp64_factor else 1)
         return
     if a.config == "serving":
         cfg = dict(cfg)
@@ -383,70 +496,101 @@ def main():
         return
     A, xyz, B = build_problem(cfg)
     warm_up_device()
-    g64 = gpu_leg(A, xyz, B, cfg, 0, a.steps, a.warmup, a.eps)
-    mixed = None
-    if not a.no_mixed:
-        mixed = gpu_leg(A, xyz, B, cfg, 1, a.steps, a.warmup, a.eps)
-    cpu = None if a.no_cpu else cpu_leg(A, xyz, B, cfg, a.eps, g64)
-    steps = a.steps
-    ms = g64["total"] / steps * 1e3
-    value = steps * g64["k"] / g64["total"]
-    lat = sorted(g64["lat"])
-    p50 = statistics.median(lat) * 1e3
-    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))] * 1e3
-    # algorithmic bytes of one apply with k right-hand sides: the stored
-    # factor once, the vector terms (56 B per row: gather, scatter, front
-    # and rank vectors) k times
-    vec_bytes = 56.0 * A._nrow
-    apply_bytes = (g64["info"]["apply_bytes"] - vec_bytes) + vec_bytes * g64["k"]
-    traffic, traffic_src = measured_traffic(a.config, 0)
-    achieved = apply_bytes / g64["apply_s"] / 1e9
+    print("workload %s: n %d, nnz %d, %d right-hand sides" % (a.config, A._nrow, A._nzmax,
+                                                             B.shape[1]),
+          file=sys.stderr, flush=True)
+    mixed = gpu_leg(A, xyz, B, cfg, 1, a.steps, a.warmup, a.eps, keep_x=True)
+    print("mixed leg: factor %.3f s, %.4f s per solve" % (mixed["t_factor"],
+                                                        mixed["total"] / a.steps),
+          file=sys.stderr, flush=True)
+    g64 = None
+    if not a.no_fp64:
+        g64 = gpu_leg(A, xyz, B, cfg, 0, a.steps, a.warmup, a.eps, keep_x=True)
+        print("fp64 leg: factor %.3f s, %.4f s per solve" % (g64["t_factor"],
+                                                           g64["total"] / a.steps),
+              file=sys.stderr, flush=True)
+    err = None
+    if not a.no_direct:
+        sols = {"mixed": mixed["X"]}
+        if g64 is not None:
+            sols["fp64"] = g64["X"]
+        err = direct_error(A, xyz, B, cfg, sols)
+        print("direct solve: %s" % err["direct"], file=sys.stderr, flush=True)
+    mixed.pop("X", None)
+    if g64 is not None:
+        g64.pop("X", None)
+    torch.cuda.empty_cache()
+    cpu = None if a.no_cpu else cpu_leg(cfg, a.eps)
+    steps, k = a.steps, mixed["k"]
+    value = steps * k / mixed["total"]
+    lat = mixed["lat"]
+    achieved = mixed["apply_bytes"] / mixed["apply_s"] / 1e9
+    traffic, traffic_src = measured_traffic(a.config, 1)
+    e2e_mixed = mixed["t_factor"] + mixed["total"] / steps
     line = {
-        "metric": "PCG solves/s to 1e-8 relres (FP64 factor; mixed FP32/BF16 factor beside)",
+        "metric": "PCG solves/s to 1e-8 relres, mixed vs FP64 factor; factor s; "
+                  "p50/p99 latency",
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(mixed["total"] / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 factor, bf16 sketches (f32 accumulation), f64 PCG",
         "data": "synthetic",
         "config": {"workload": a.config, "n": A._nrow, "nnz": int(A._nzmax),
-                   "max_dense_block": cfg["max_dense_block"], "max_rank": cfg["max_rank"],
-                   "nrhs": g64["k"], "eps": a.eps, "parallelism": "single GPU"},
-        "pcg_iterations": g64["iters"],
-        "true_relres": g64["relres"],
-        "p50_ms": round(p50, 3),
-        "p99_ms": round(p99, 3),
-        "factor_s": round(g64["t_factor"], 4),
-        "numeric_factor_s": round(g64["info"]["numeric_seconds"], 4),
-        "symbolic_s": round(g64["info"]["symbolic_seconds"], 4),
-        "factor_bytes": g64["info"]["factor_bytes"],
-        "roofline": {"bound": "hbm", "kernel": "preconditioner apply (fwd+bwd sweeps)",
+                   "sigma": cfg.get("sigma"), "max_dense_block": cfg["max_dense_block"],
+                   "max_rank": cfg["max_rank"], "oversampling": cfg["oversampling"],
+                   "nrhs": k, "eps": a.eps, "parallelism": "single GPU"},
+        "pcg_iterations": mixed["iters"],
+        "true_relres_max": max(mixed["relres"]),
+        "p50_ms": round(statistics.median(lat) * 1e3, 3),
+        "p99_ms": round(pct(lat, 0.99) * 1e3, 3),
+        "factor_s": round(mixed["t_factor"], 4),
+        "numeric_factor_s": round(mixed["info"]["numeric_seconds"], 4),
+        "symbolic_s": round(mixed["info"]["symbolic_seconds"], 4),
+        "factor_bytes": mixed["info"]["factor_bytes"],
+        "pivot_replacements": mixed["info"]["pivot_replacements"],
+        "end_to_end_s": round(e2e_mixed, 4),
+        "roofline": {"bound": "hbm", "kernel": "preconditioner apply, mixed factor "
+                                               "(k_apply_df<float, true>: fwd+bwd sweeps)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "apply_ms": round(g64["apply_s"] * 1e3, 4),
-                     "apply_bytes": apply_bytes},
+                     "apply_ms": round(mixed["apply_s"] * 1e3, 4),
+                     "apply_bytes": mixed["apply_bytes"]},
     }
-    if mixed is not None:
-        line["mixed"] = {"factor": "FP32 dense blocks, BF16 range-finder products "
-                                   "(FP32 accumulation), FP64 PCG",
-                         "solves_per_s": round(steps * mixed["k"] / mixed["total"], 3),
-                         "factor_s": round(mixed["t_factor"], 4),
-                         "numeric_factor_s": round(mixed["info"]["numeric_seconds"], 4),
-                         "pcg_iterations": mixed["iters"], "true_relres": mixed["relres"],
-                         "apply_ms": round(mixed["apply_s"] * 1e3, 4),
-                         "pivot_replacements": mixed["info"]["pivot_replacements"],
-                         "end_to_end_s": round(mixed["t_factor"] + mixed["total"] / steps, 4)}
-        line["end_to_end_s_fp64"] = round(g64["t_factor"] + g64["total"] / steps, 4)
+    if g64 is not None:
+        e2e64 = g64["t_factor"] + g64["total"] / steps
+        line["fp64"] = {"solves_per_s": round(steps * k / g64["total"], 3),
+                        "factor_s": round(g64["t_factor"], 4),
+                        "numeric_factor_s": round(g64["info"]["numeric_seconds"], 4),
+                        "symbolic_s": round(g64["info"]["symbolic_seconds"], 4),
+                        "pcg_iterations": g64["iters"],
+                        "true_relres_max": max(g64["relres"]),
+                        "apply_ms": round(g64["apply_s"] * 1e3, 4),
+                        "apply_achieved_gbs": round(g64["apply_bytes"] / g64["apply_s"] / 1e9,
+                                                    2),
+                        "factor_bytes": g64["info"]["factor_bytes"],
+                        "end_to_end_s": round(e2e64, 4)}
+        line["speedup_mixed_vs_fp64"] = {"end_to_end": round(e2e64 / e2e_mixed, 3),
+                                         "solve": round(g64["total"] / mixed["total"], 3),
+                                         "numeric_factor": round(
+                                             g64["info"]["numeric_seconds"] /
+                                             mixed["info"]["numeric_seconds"], 3)}
+    if err is not None:
+        line["error_vs_direct_solve"] = err
     if cpu is not None:
         line["cpu_baseline"] = {"value": round(cpu["value"], 4), "unit": "solves/s",
-                                "cores": cpu["cores"], "kind": "port",
-                                "sample": cpu["sample"]}
+                                "cores": cpu["cores"], "kind": cpu["kind"],
+                                "sample": cpu["sample"],
+                                "one_thread_solves_per_s": cpu["one_thread_solves_per_s"],
+                                "end_to_end_s": cpu["end_to_end_s"],
+                                "gpu_same_sample": cpu["gpu_same_sample"],
+                                "speedup_same_sample": cpu["speedup_same_sample"]}
     print(json.dumps(line))
 
 

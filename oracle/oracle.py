@@ -24,13 +24,18 @@ _DP = C.POINTER(C.c_double)
 _lib = None
 
 
-def lib():
+def lib(path=None):
+    """The oracle library (path: another build of the same source, e.g. the
+    -march=native build of the CPU baseline; loaded in its place)."""
     global _lib
+    if path is not None and _lib is not None and getattr(_lib, "_path", None) != path:
+        raise RuntimeError("another oracle build is already loaded")
     if _lib is None:
-        if not os.path.exists(LIB):
+        if path is None and not os.path.exists(LIB):
             from . import build as _b
             _b.build(verbose=False)
-        L = C.CDLL(LIB)
+        L = C.CDLL(path or LIB)
+        L._path = path
         L.oracle_factor.restype = C.c_int
         L.oracle_factor.argtypes = [C.c_int, _IP, _IP, _DP, _IP, C.c_int, _IP, _IP, _IP,
                                     _IP, C.POINTER(OracleOpts), C.POINTER(C.c_void_p),
@@ -51,6 +56,13 @@ def lib():
         L.oracle_destroy.argtypes = [C.c_void_p]
         L.oracle_gaussian.restype = None
         L.oracle_gaussian.argtypes = [C.c_uint32, C.c_int64, _DP]
+        L.oracle_block_pcg.restype = None
+        L.oracle_block_pcg.argtypes = [C.c_void_p, C.c_int, _DP, _DP, C.c_double, C.c_int,
+                                       _IP, _DP, _DP]
+        L.oracle_set_threads.restype = None
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_max_threads.restype = C.c_int
+        L.oracle_max_threads.argtypes = []
         L.oracle_mt_output.restype = C.c_uint32
         L.oracle_mt_output.argtypes = [C.c_uint32, C.c_int64]
         _lib = L
@@ -137,6 +149,19 @@ class OracleFactor:
                               C.byref(rr), C.byref(tr))
         return x, dict(iterations=it, recurrence_relres=rr.value, true_relres=tr.value)
 
+    def block_pcg(self, B, eps, iters):
+        """k independent PCG solves from zero (columns of B, n x k)."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        k = B.shape[1]
+        X = np.zeros_like(B, order="F")
+        it = np.zeros(k, np.int32)
+        rr = np.zeros(k)
+        tr = np.zeros(k)
+        lib().oracle_block_pcg(self._h, k, B.ctypes.data_as(_DP), X.ctypes.data_as(_DP),
+                               eps, iters, it.ctypes.data_as(_IP), rr.ctypes.data_as(_DP),
+                               tr.ctypes.data_as(_DP))
+        return X, dict(iterations=it, recurrence_relres=rr, true_relres=tr)
+
     def offdiag(self, s):
         R = lib().oracle_offdiag(self._h, s, None, 0)
         return R
@@ -156,6 +181,14 @@ def gaussian(seed, count):
     out = np.empty(int(count))
     lib().oracle_gaussian(seed, int(count), out.ctypes.data_as(_DP))
     return out
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def max_threads():
+    return lib().oracle_max_threads()
 
 
 def mt_output(seed, index):

@@ -33,7 +33,9 @@
 //     135-162)
 //   * PCG: PCG_SOLVER::solvePCG (PCG_SOLVER.cpp:226-363)
 // The ordering (permutation, supernode partition, diagonal block ranges) is
-// an input.  Dense kernels are textbook loops (no BLAS/LAPACK here).
+// an input.  Dense kernels are plain loops (no BLAS/LAPACK here), blocked
+// and spread over OpenMP threads without changing any element's summation
+// order, so results do not depend on the thread count.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -44,6 +46,8 @@
 #include <random>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 typedef std::vector<double> Vec;
@@ -51,74 +55,125 @@ typedef std::vector<double> Vec;
 // column-major dense matrix view helpers --------------------------------------
 inline double& at(double* A, int ld, int i, int j) { return A[i + (int64_t)j * ld]; }
 
+// The dense kernels below are blocked for cache reuse and run on OpenMP
+// threads over independent output blocks.  Every output element is
+// accumulated by one thread over the summation index in increasing order,
+// as the textbook triple loop does, so the results do not depend on the
+// number of threads.
+constexpr int64_t kParallelWork = 1 << 15;  // flops below which one thread runs
+
+// C(m x n) (+)= alpha * A(m x k) * op(B), op(B)(t, j) = bt ? B[j + t ldb] :
+// B[t + j ldb]; lower: only elements with i >= j + diag_off.  Each element
+// sums t = 0..k-1 in order; a zero B entry is skipped.
+void gemm_core(int m, int n, int k, double alpha, const double* A, int lda,
+               const double* B, int ldb, bool bt, double* C, int ldc, bool zero,
+               bool lower, int diag_off) {
+  if (m <= 0 || n <= 0) return;
+  constexpr int MB = 256, NB = 16, KB = 128;
+  const int mt = (m + MB - 1) / MB, nt = (n + NB - 1) / NB;
+  const bool par = (int64_t)m * n * std::max(k, 1) > kParallelWork;
+#pragma omp parallel for collapse(2) schedule(dynamic, 1) if (par)
+  for (int jb = 0; jb < nt; ++jb)
+    for (int ib = 0; ib < mt; ++ib) {
+      const int j0 = jb * NB, j1 = std::min(n, j0 + NB);
+      const int i0 = ib * MB, i1 = std::min(m, i0 + MB);
+      if (lower && i1 - 1 < j0 + diag_off) continue;
+      if (zero)
+        for (int j = j0; j < j1; ++j)
+          for (int i = i0; i < i1; ++i) C[i + (int64_t)j * ldc] = 0.0;
+      for (int t0 = 0; t0 < k; t0 += KB) {
+        const int t1 = std::min(k, t0 + KB);
+        for (int j = j0; j < j1; ++j) {
+          double* c = C + (int64_t)j * ldc;
+          const int ilo = lower ? std::max(i0, j + diag_off) : i0;
+          for (int t = t0; t < t1; ++t) {
+            const double b = alpha * (bt ? B[j + (int64_t)t * ldb] : B[t + (int64_t)j * ldb]);
+            if (b == 0.0) continue;
+            const double* a = A + (int64_t)t * lda;
+            for (int i = ilo; i < i1; ++i) c[i] += a[i] * b;
+          }
+        }
+      }
+    }
+}
+
 // C(m x n) += alpha * A(m x k) * B(n x k)^T
 void gemm_nt(int m, int n, int k, double alpha, const double* A, int lda,
              const double* B, int ldb, double* C, int ldc) {
-  for (int j = 0; j < n; ++j)
-    for (int t = 0; t < k; ++t) {
-      double b = alpha * B[j + (int64_t)t * ldb];
-      if (b == 0.0) continue;
-      const double* a = A + (int64_t)t * lda;
-      double* c = C + (int64_t)j * ldc;
-      for (int i = 0; i < m; ++i) c[i] += a[i] * b;
-    }
+  gemm_core(m, n, k, alpha, A, lda, B, ldb, true, C, ldc, false, false, 0);
 }
 // C(m x n) = A(m x k) * B(k x n)
 void gemm_nn(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
              double* C, int ldc) {
-  for (int j = 0; j < n; ++j) {
-    double* c = C + (int64_t)j * ldc;
-    for (int i = 0; i < m; ++i) c[i] = 0.0;
-    for (int t = 0; t < k; ++t) {
-      double b = B[t + (int64_t)j * ldb];
-      const double* a = A + (int64_t)t * lda;
-      for (int i = 0; i < m; ++i) c[i] += a[i] * b;
-    }
-  }
+  gemm_core(m, n, k, 1.0, A, lda, B, ldb, false, C, ldc, true, false, 0);
 }
-// C(m x n) = A(k x m)^T * B(k x n)
+// C(m x n) = A(k x m)^T * B(k x n): A^T is formed once, then each element
+// sums t = 0..k-1 in order as the dot product does
 void gemm_tn(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
              double* C, int ldc) {
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < m; ++i) {
-      const double* a = A + (int64_t)i * lda;
-      const double* b = B + (int64_t)j * ldb;
-      double s = 0;
-      for (int t = 0; t < k; ++t) s += a[t] * b[t];
-      C[i + (int64_t)j * ldc] = s;
-    }
+  std::vector<double> At((size_t)m * k);
+#pragma omp parallel for schedule(static) if ((int64_t)m * k > kParallelWork)
+  for (int i = 0; i < m; ++i)
+    for (int t = 0; t < k; ++t) At[i + (size_t)t * m] = A[t + (int64_t)i * lda];
+  gemm_core(m, n, k, 1.0, At.data(), m, B, ldb, false, C, ldc, true, false, 0);
 }
 
-// in-place lower Cholesky; returns 0 or the 1-based failing column
+// In-place lower Cholesky (left-looking by panels of NB columns: the panel
+// first takes the products of all columns to its left, then is factored
+// column by column; every element subtracts its terms in increasing column
+// order).  Returns 0 or the 1-based failing column.
 int potrf(int n, double* A, int lda) {
-  for (int j = 0; j < n; ++j) {
-    double d = at(A, lda, j, j);
-    for (int t = 0; t < j; ++t) d -= at(A, lda, j, t) * at(A, lda, j, t);
-    if (!(d > 0.0)) return j + 1;
-    d = std::sqrt(d);
-    at(A, lda, j, j) = d;
-    for (int t = 0; t < j; ++t) {
-      double ljt = at(A, lda, j, t);
-      if (ljt == 0.0) continue;
-      for (int i = j + 1; i < n; ++i) at(A, lda, i, j) -= at(A, lda, i, t) * ljt;
+  constexpr int NB = 64;
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int j1 = std::min(n, j0 + NB);
+    // A(j0:n, j0:j1) -= A(j0:n, 0:j0) A(j0:j1, 0:j0)^T, lower part
+    gemm_core(n - j0, j1 - j0, j0, -1.0, A + j0, lda, A + j0, lda, true,
+              A + j0 + (int64_t)j0 * lda, lda, false, true, 0);
+    for (int j = j0; j < j1; ++j) {
+      double d = at(A, lda, j, j);
+      for (int t = j0; t < j; ++t) d -= at(A, lda, j, t) * at(A, lda, j, t);
+      if (!(d > 0.0)) return j + 1;
+      d = std::sqrt(d);
+      at(A, lda, j, j) = d;
+      const int rows = n - j - 1;
+#pragma omp parallel for schedule(static) if ((int64_t)rows * (j - j0 + 1) > kParallelWork)
+      for (int i0 = j + 1; i0 < n; i0 += 256) {
+        const int i1 = std::min(n, i0 + 256);
+        for (int t = j0; t < j; ++t) {
+          const double ljt = at(A, lda, j, t);
+          if (ljt == 0.0) continue;
+          for (int i = i0; i < i1; ++i) at(A, lda, i, j) -= at(A, lda, i, t) * ljt;
+        }
+        for (int i = i0; i < i1; ++i) at(A, lda, i, j) /= d;
+      }
     }
-    for (int i = j + 1; i < n; ++i) at(A, lda, i, j) /= d;
   }
   return 0;
 }
 
-// X(m x n) <- X L^{-T}, L n x n lower
+// X(m x n) <- X L^{-T}, L n x n lower (left-looking by panels of columns;
+// each element subtracts its terms in increasing column order)
 void trsm_rlt(int m, int n, const double* L, int ldl, double* X, int ldx) {
-  for (int j = 0; j < n; ++j) {
-    double* xj = X + (int64_t)j * ldx;
-    for (int t = 0; t < j; ++t) {
-      double l = L[j + (int64_t)t * ldl];
-      if (l == 0.0) continue;
-      const double* xt = X + (int64_t)t * ldx;
-      for (int i = 0; i < m; ++i) xj[i] -= xt[i] * l;
+  constexpr int NB = 64;
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int j1 = std::min(n, j0 + NB);
+    gemm_core(m, j1 - j0, j0, -1.0, X, ldx, L + j0, ldl, true, X + (int64_t)j0 * ldx, ldx,
+              false, false, 0);
+#pragma omp parallel for schedule(static) if ((int64_t)m * NB * NB > kParallelWork)
+    for (int i0 = 0; i0 < m; i0 += 256) {
+      const int i1 = std::min(m, i0 + 256);
+      for (int j = j0; j < j1; ++j) {
+        double* xj = X + (int64_t)j * ldx;
+        for (int t = j0; t < j; ++t) {
+          const double l = L[j + (int64_t)t * ldl];
+          if (l == 0.0) continue;
+          const double* xt = X + (int64_t)t * ldx;
+          for (int i = i0; i < i1; ++i) xj[i] -= xt[i] * l;
+        }
+        const double d = L[j + (int64_t)j * ldl];
+        for (int i = i0; i < i1; ++i) xj[i] /= d;
+      }
     }
-    double d = L[j + (int64_t)j * ldl];
-    for (int i = 0; i < m; ++i) xj[i] /= d;
   }
 }
 
@@ -141,6 +196,7 @@ void householder_q(int m, int r, double* Y, int ldy) {
     for (int i = j + 1; i < m; ++i) a[i] *= scale;
     a[j] = beta;
     tau[j] = t;
+#pragma omp parallel for schedule(static) if ((int64_t)(m - j) * (r - j) > kParallelWork)
     for (int c = j + 1; c < r; ++c) {
       double* b = Y + (int64_t)c * ldy;
       double s = b[j];
@@ -164,6 +220,7 @@ void householder_q(int m, int r, double* Y, int ldy) {
   for (int j = r - 1; j >= 0; --j) {
     if (tau[j] == 0.0) continue;
     const Vec& hv = refl[j];
+#pragma omp parallel for schedule(static) if ((int64_t)(m - j) * (r - j) > kParallelWork)
     for (int c = j; c < r; ++c) {
       double* q = Y + (int64_t)c * ldy;
       double s = 0;
@@ -729,6 +786,22 @@ int oracle_pcg(void* h, const double* b, double* x, double eps, int iters,
   std::copy(xv.begin(), xv.end(), x);
   return i;
 }
+
+// k independent PCG solves (columns of B and X, leading dimension n), one
+// per OpenMP thread at a time: the CPU counterpart of the device's block
+// PCG, whose columns follow solvePCG's arithmetic on their own.
+void oracle_block_pcg(void* h, int k, const double* B, double* X, double eps, int iters,
+                      int* iterations, double* recurrence_relres, double* true_relres) {
+  const int64_t n = ((Oracle*)h)->n;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < k; ++j)
+    iterations[j] = oracle_pcg(h, B + j * n, X + j * n, eps, iters,
+                               recurrence_relres + j, true_relres + j);
+}
+
+// Threads of the OpenMP kernels (the CPU baseline times 1 and all cores).
+void oracle_set_threads(int nthreads) { omp_set_num_threads(nthreads > 0 ? nthreads : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 // Off-diagonal factor block of supernode s as a dense R x c matrix
 // (explicit L21, or V U^T for a compressed supernode); returns R.

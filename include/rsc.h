@@ -161,6 +161,21 @@ rsc_status rsc_apply(rsc_factor* f, int nrhs, const double* B, int ldb,
  * themselves. */
 rsc_status rsc_apply_status(rsc_factor* f, void* stream);
 
+/* rsc_apply without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory during the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results (bit for bit) as rsc_apply; the
+ * per-call scratch (permuted and front vectors, rank vectors, partial sums)
+ * comes from a fixed 64 MiB device array per GPU, held until the process
+ * ends, instead of buffers the library grows.  Applies whose scratch exceeds
+ * it (about 64 MiB / (8 (2n + sum of front sizes)) right-hand sides: e.g.
+ * one right-hand side of a 64^3 Poisson system, not a 128^3 one) return
+ * RSC_BAD_INPUT before any work.  All calls on one GPU share the array:
+ * calls that overlap on one GPU race, so the caller orders them.  Check the
+ * result with rsc_apply_status as for rsc_apply. */
+rsc_status rsc_apply_noalloc(rsc_factor* f, int nrhs, const double* B, int ldb,
+                             double* X, int ldx, void* stream);
+
 /* PCG on A x = b with the factor as preconditioner.  b, x: DEVICE pointers;
  * x holds the initial guess on entry (the reference driver passes zero).
  * Stops when ||r||/||b|| <= eps or after max_iters. */

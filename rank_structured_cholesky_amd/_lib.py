@@ -71,6 +71,7 @@ SIGNATURES = {
     "rsc_block_pcg_solve_host": (_I, [_P, _I, _DP, _I, _DP, _I, _D, _I,
                                       C.POINTER(PcgResult)]),
     "rsc_apply_status": (_I, [_P, _P]),
+    "rsc_apply_noalloc": (_I, [_P, _I, _P, _I, _P, _I, _P]),
     "rsc_spmv": (_I, [_P, _P, _P, _P]),
     "rsc_factor_get_info": (_I, [_P, C.POINTER(FactorInfo)]),
     "rsc_factor_get_permutation": (_I, [_P, _IP]),

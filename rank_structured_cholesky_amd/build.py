@@ -20,7 +20,7 @@ ARCH = "gfx950"
 HOST_SOURCES = ["host/ordering.cpp", "host/generators.cpp", "host/symbolic.cpp",
                 "host/solve_tasks.cpp"]
 HIP_SOURCES = ["device/kernels.hip", "device/factor.hip", "device/solve.hip",
-               "device/pcg.hip", "capi.cpp", "serve.cpp"]
+               "device/pcg.hip", "capi.cpp", "serve.cpp", "noalloc.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

@@ -99,6 +99,14 @@ class FactorEngine {
   // column's result does not depend on how many others share the apply)
   virtual void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
                      const int* live = nullptr, int multi_min_k = 2) = 0;
+  // Doubles of per-call scratch an apply with k right-hand sides uses (the
+  // permuted vectors w and z, front vectors, rank vectors, partial sums).
+  virtual int64_t apply_scratch_doubles(int k) const = 0;
+  // apply() with that scratch supplied by the caller (device memory of at
+  // least apply_scratch_doubles(k) doubles, 256-byte aligned; its contents
+  // need no initialization): no allocation of any kind.
+  virtual void apply_into(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
+                          double* scratch) = 0;
   // Synchronizes s and throws DeviceError if an apply on it failed.
   virtual void check_apply(hipStream_t s) = 0;
   // Device address of the apply's status words (two ints, both 0 while every

@@ -128,6 +128,9 @@ class Engine final : public FactorEngine {
   void apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
              const int* live, int multi_min_k) override;
   void check_apply(hipStream_t s) override;
+  int64_t apply_scratch_doubles(int k) const override;
+  void apply_into(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
+                  double* scratch) override;
   const int* apply_status_device() const override { return dapst_.as<int>(); }
   void raise_apply_status(const int* words) override;
   int64_t factor_bytes() const override { return (int64_t)factor_.bytes; }
@@ -326,6 +329,8 @@ class Engine final : public FactorEngine {
     return p;
   }
   void emit_level_compressed(const std::vector<int>& nodes);
+  void run_apply(int k, const double* B, int ldb, double* X, int ldx, hipStream_t s,
+                 const dev::VecBufs& vb);
   int64_t inv_cursor_ = 0, qr64_cursor_ = 0;  // scratch of the supernode being emitted
   std::vector<int64_t> inv_need_, qr64_need_;  // per supernode (T elements)
   void invert(const std::vector<CholMat>& tris);
@@ -1776,6 +1781,29 @@ void Engine<T>::numeric(const double* Ax) {
 }
 
 template <typename T>
+int64_t Engine<T>::apply_scratch_doubles(int k) const {
+  // each part rounded up to 32 doubles (256 bytes) so every part is aligned
+  auto up = [](int64_t v) { return (v + 31) / 32 * 32; };
+  const int64_t n = plan_.n;
+  return up(fv_stride_ * k) + up(tv_stride_ * k) + up(part_stride_ * k) + 2 * up(n * k);
+}
+
+template <typename T>
+void Engine<T>::apply_into(int k, const double* B, int ldb, double* X, int ldx,
+                           hipStream_t s, double* scratch) {
+  if (!solve_ready_) throw std::logic_error("apply before the factorization finished");
+  auto up = [](int64_t v) { return (v + 31) / 32 * 32; };
+  const int64_t n = plan_.n;
+  double* fv = scratch;
+  double* tv = fv + up(fv_stride_ * k);
+  double* part = tv + up(tv_stride_ * k);
+  double* w = part + up(part_stride_ * k);
+  double* z = w + up(n * k);
+  dev::VecBufs vb{w, fv, tv, part, n, fv_stride_, tv_stride_, part_stride_, nullptr, z, 2};
+  run_apply(k, B, ldb, X, ldx, s, vb);
+}
+
+template <typename T>
 void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
                       hipStream_t s, const int* live, int multi_min_k) {
   finish_solve_setup();
@@ -1791,6 +1819,16 @@ void Engine<T>::apply(int k, const double* B, int ldb, double* X, int ldx,
   dev::VecBufs vb{dw_.as<double>(), dfv_.as<double>(), dtv_.as<double>(),
                   dpart_.as<double>(), (int64_t)n, fv_stride_, tv_stride_, part_stride_,
                   live, dz_.as<double>(), multi_min_k};
+  run_apply(k, B, ldb, X, ldx, s, vb);
+}
+
+// The launches of one apply on given vector buffers (no allocation).
+template <typename T>
+void Engine<T>::run_apply(int k, const double* B, int ldb, double* X, int ldx,
+                          hipStream_t s, const dev::VecBufs& vb) {
+  const int n = plan_.n;
+  const int multi_min_k = vb.multi_min_k;
+  const int* live = vb.live;
   if (use_dataflow_) {
     dev::DfArgs a{};
     df_last_ = (k > 1 || multi_min_k <= 1) ? 1 : 0;

@@ -297,6 +297,19 @@ rsc_status rsc_nested_dissection(int n, const int* colptr, const int* rowind,
  * identity postorder (NestedDissection.cpp:418-429). */
 rsc_status rsc_etree_colcounts(int n, const int* colptr, const int* rowind,
                                const int* perm, int* parent, int* colcount);
+/* Relaxed supernode partition (NestedDissection::BuildRelaxedSupernodes,
+ * NestedDissection.cpp:1266-1667) of an elimination tree with column counts
+ * (parent[n], colcount[n]) and ncomp nested-dissection components given by
+ * their sizes, first/last columns, child counts and parents, with the
+ * reference's relaxation constants.  first[] (n+1 entries of space)
+ * receives each supernode's first column, compressed[] (n entries) its flag,
+ * *nsuper the count. */
+rsc_status rsc_relaxed_supernodes(int n, const int* parent, const int* colcount,
+                                  int ncomp, const int* comp_size,
+                                  const int* comp_first, const int* comp_last,
+                                  const int* comp_nchildren, const int* comp_parent,
+                                  int max_dense_block, int* first, int* compressed,
+                                  int* nsuper);
 /* SeparatorTree leaf order of npts points (xyz), SeparatorTree.cpp:189-589:
  * order[npts] receives the point order, leaf_sizes (<= npts entries) the
  * leaf sizes. */

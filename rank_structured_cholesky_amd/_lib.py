@@ -87,6 +87,8 @@ SIGNATURES = {
     "rsc_symbolic_destroy": (None, [_P]),
     "rsc_nested_dissection": (_I, [_I, _IP, _IP, _DP, _I, _IP, _IP, _IP, _IP]),
     "rsc_etree_colcounts": (_I, [_I, _IP, _IP, _IP, _IP, _IP]),
+    "rsc_relaxed_supernodes": (_I, [_I, _IP, _IP, _I, _IP, _IP, _IP, _IP, _IP, _I, _IP, _IP,
+                                    _IP]),
     "rsc_separator_tree": (_I, [_I, _DP, _I, _IP, _IP, _IP]),
     "rsc_gaussian_stream": (_I, [C.c_uint32, _I64, _DP]),
     "rsc_block_rank": (_I, [_I, _I, _D, _I, _I, _I]),

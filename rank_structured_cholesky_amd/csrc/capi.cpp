@@ -529,6 +529,47 @@ rsc_status rsc_etree_colcounts(int n, const int* colptr, const int* rowind,
   });
 }
 
+rsc_status rsc_relaxed_supernodes(int n, const int* parent, const int* colcount, int ncomp,
+                                  const int* comp_size, const int* comp_first,
+                                  const int* comp_last, const int* comp_nchildren,
+                                  const int* comp_parent, int max_dense_block, int* first,
+                                  int* compressed, int* nsuper) {
+  if (n < 0 || ncomp < 0 || !parent || !colcount || !comp_size || !comp_first ||
+      !comp_last || !comp_nchildren || !comp_parent || !first || !compressed || !nsuper)
+    return fail(RSC_BAD_INPUT, "bad argument");
+  for (int j = 0; j < n; ++j)
+    if (parent[j] < -1 || parent[j] >= n || parent[j] == j)
+      return fail(RSC_BAD_INPUT, "parent out of range");
+  int next = 0;
+  for (int c = 0; c < ncomp; ++c) {
+    if (comp_first[c] != next || comp_last[c] - comp_first[c] + 1 != comp_size[c] ||
+        comp_parent[c] < -1 || comp_parent[c] >= ncomp)
+      return fail(RSC_BAD_INPUT, "components do not tile the columns");
+    next = comp_last[c] + 1;
+  }
+  if (next != n) return fail(RSC_BAD_INPUT, "components do not cover the columns");
+  return guard([&]() -> rsc_status {
+    std::vector<int> par(parent, parent + n), cc(colcount, colcount + n);
+    std::vector<rsc::Separator> seps(ncomp);
+    for (int c = 0; c < ncomp; ++c) {
+      seps[c].size = comp_size[c];
+      seps[c].first = comp_first[c];
+      seps[c].last = comp_last[c];
+      seps[c].nchildren = comp_nchildren[c];
+    }
+    std::vector<int> sp(comp_parent, comp_parent + ncomp);
+    std::vector<rsc::SupernodeSpec> out =
+        rsc::build_relaxed_supernodes(par, cc, seps, sp, max_dense_block);
+    for (size_t q = 0; q < out.size(); ++q) {
+      first[q] = out[q].first;
+      compressed[q] = out[q].compress ? 1 : 0;
+    }
+    first[out.size()] = n;
+    *nsuper = (int)out.size();
+    return RSC_OK;
+  });
+}
+
 rsc_status rsc_separator_tree(int npts, const double* xyz, int max_leaf, int* order,
                               int* nleaves, int* leaf_sizes) {
   if (npts < 0 || !xyz || !order || !nleaves || max_leaf < 0)

@@ -434,136 +434,182 @@ std::vector<int> column_counts(const Csc& A, const std::vector<int>& pinv,
 }
 
 // --------------------------------------------------------------------------
-// Relaxed supernodes: the reference's adaptation of CHOLMOD's supernodal
-// analysis (NestedDissection.cpp:1266-1667).  Compressed separators and all
-// their ancestors are single fundamental supernodes (:1356-1385); other
-// columns form fundamental supernodes (:1387-1433) which are amalgamated by
-// the CHOLMOD relaxation rule (:1476-1574), never across a compressed one.
+// Relaxed supernodes (the partition rule of NestedDissection.cpp:1266-1667,
+// itself CHOLMOD's supernodal analysis with relaxed amalgamation).  Stated
+// as a rule:
+//
+//  1. A nested-dissection component with more than max_block_size columns
+//     and at least one child is compressed, and so is every ancestor of a
+//     compressed component.  A compressed component is one supernode that
+//     nothing merges into or out of.
+//  2. Between compressed components, consecutive uncompressed components
+//     form one run of columns.  The run's first column starts a fundamental
+//     supernode; column j continues the current one exactly when column j-1
+//     is its only etree child with one more nonzero in L (parent[j-1] == j,
+//     count[j-1] == count[j] + 1, j has at most one etree child).
+//  3. Going from the second-to-last fundamental supernode down to the
+//     first, supernode s may absorb s+1 when s+1 is (after earlier merges)
+//     s's parent and neither is compressed.  With c0, c1 the column counts
+//     and k0, k1 the first-column nonzero counts of s and s+1, z1 the
+//     explicit zeros s+1 already carries, and t = c0 + c1:
+//        t <= nrelax0                             -> merge, no new zeros
+//        new = c0 (k1 + c0 - k0) == 0             -> merge, no new zeros
+//        otherwise z = (z1 + new) / (t(t+1)/2 + t (k1 - c1)) and merge if
+//        (t <= nrelax1 and z < zrelax0) or (t <= nrelax2 and z < zrelax1)
+//        or z < zrelax2.
+//     The ratio is evaluated in double and the zero count in integers, as
+//     the reference does, so the decisions are identical.  After a merge s
+//     has c0 + c1 columns, c0 + k1 nonzeros in its first column and z1 + new
+//     zeros.
+//  4. Every supernode not absorbed is emitted with its column range.
 // --------------------------------------------------------------------------
+namespace {
+
+// One fundamental supernode while the amalgamation runs.
+struct FundNode {
+  int first = 0;         // first column
+  int width = 0;         // columns, grown by merges
+  int64_t nnz_first = 0; // nonzeros of L in its first column (diagonal included)
+  int64_t zeros = 0;     // explicit zeros carried from earlier merges
+  int parent = -1;       // fundamental supernode of the parent column, -1 = root
+  bool whole = false;    // compressed component: never merged
+  int taken_by = -1;     // the supernode that absorbed it, -1 while it stands
+};
+
+// Components to compress: large separators with children, and their
+// ancestors (rule 1).
+std::vector<char> components_to_compress(const std::vector<Separator>& seps,
+                                         const std::vector<int>& sep_parent,
+                                         int max_block_size) {
+  std::vector<char> mark(seps.size(), 0);
+  for (size_t c = 0; c < seps.size(); ++c) {
+    if (seps[c].size <= max_block_size || seps[c].nchildren <= 0) continue;
+    for (int a = (int)c; a >= 0 && !mark[a]; a = sep_parent[a]) mark[a] = 1;
+    // an ancestor chain already marked above stops the walk
+  }
+  return mark;
+}
+
+// Fundamental supernodes of the column ranges (rule 2).
+std::vector<FundNode> fundamental_partition(const std::vector<int>& parent,
+                                            const std::vector<int>& count,
+                                            const std::vector<Separator>& seps,
+                                            const std::vector<char>& whole) {
+  const int n = (int)parent.size();
+  std::vector<int> kids(n, 0);
+  for (int j = 0; j < n; ++j)
+    if (parent[j] >= 0) ++kids[parent[j]];
+  auto continues = [&](int j) {
+    return parent[j - 1] == j && count[j - 1] == count[j] + 1 && kids[j] <= 1;
+  };
+  std::vector<FundNode> out;
+  bool in_run = false;
+  for (size_t c = 0; c < seps.size(); ++c) {
+    if (whole[c]) {
+      FundNode f;
+      f.first = seps[c].first;
+      f.whole = true;
+      out.push_back(f);
+      in_run = false;
+      continue;
+    }
+    for (int j = seps[c].first; j <= seps[c].last; ++j) {
+      if (in_run && continues(j)) continue;
+      FundNode f;
+      f.first = j;
+      out.push_back(f);
+      in_run = true;
+    }
+  }
+  return out;
+}
+
+// Representative of fundamental supernode q after the merges so far.
+int standing(std::vector<FundNode>& f, int q) {
+  int r = q;
+  while (f[r].taken_by >= 0) r = f[r].taken_by;
+  // point the visited chain at the representative (a shortcut only: the
+  // representative is the same either way)
+  while (f[q].taken_by >= 0 && f[q].taken_by != r) {
+    const int next = f[q].taken_by;
+    f[q].taken_by = r;
+    q = next;
+  }
+  return r;
+}
+
+struct RelaxParams {
+  int nrelax[3];
+  double zrelax[3];
+};
+
+// Rule 3 for child s (columns first) and its parent p = s + 1; on a merge
+// returns true and the zero count of the merged supernode.
+bool relaxed_merge(const FundNode& s, const FundNode& p, const RelaxParams& rp,
+                   int64_t& zeros_after) {
+  const int t = s.width + p.width;
+  zeros_after = p.zeros;
+  if (t <= rp.nrelax[0]) return true;
+  // zeros the merge adds: the child's columns get the parent's rows
+  const double added =
+      (double)s.width * ((double)p.nnz_first + s.width - (double)s.nnz_first);
+  if (added == 0) return true;
+  const int64_t added_i = (int64_t)s.width * (p.nnz_first + s.width - s.nnz_first);
+  const double tt = (double)t;
+  const double entries = tt * (tt + 1) / 2 + tt * ((double)p.nnz_first - p.width);
+  const double ratio = ((double)p.zeros + added) / entries;
+  zeros_after = p.zeros + added_i;
+  return (t <= rp.nrelax[1] && ratio < rp.zrelax[0]) ||
+         (t <= rp.nrelax[2] && ratio < rp.zrelax[1]) || ratio < rp.zrelax[2];
+}
+
+}  // namespace
+
 std::vector<SupernodeSpec> build_relaxed_supernodes(
-    const std::vector<int>& Parent, const std::vector<int>& cc,
+    const std::vector<int>& parent, const std::vector<int>& count,
     const std::vector<Separator>& seps, const std::vector<int>& sep_parent,
     int max_block_size, int nrelax0, int nrelax1, int nrelax2, double zrelax0,
     double zrelax1, double zrelax2) {
-  const int n = (int)Parent.size();
-  const int EMPTY = -1;
-  std::vector<int> nchild(n, 0);
-  for (int j = 0; j < n; ++j)
-    if (Parent[j] >= 0) nchild[Parent[j]]++;
-
-  int ns = (int)seps.size();
-  std::vector<char> compress(ns, 0);
-  for (int s = 0; s < ns; ++s) {
-    if (seps[s].size > max_block_size && seps[s].nchildren > 0) {
-      compress[s] = 1;
-      for (int a = sep_parent[s]; a >= 0; a = sep_parent[a]) compress[a] = 1;
-    }
+  const int n = (int)parent.size();
+  const std::vector<char> whole = components_to_compress(seps, sep_parent, max_block_size);
+  std::vector<FundNode> f = fundamental_partition(parent, count, seps, whole);
+  const int nf = (int)f.size();
+  // widths, first-column counts and parents; owner[] maps a column to its
+  // fundamental supernode
+  std::vector<int> owner(n, -1);
+  for (int q = 0; q < nf; ++q) {
+    const int end = q + 1 < nf ? f[q + 1].first : n;
+    f[q].width = end - f[q].first;
+    f[q].nnz_first = f[q].first < n ? count[f[q].first] : 0;
+    for (int j = f[q].first; j < end; ++j) owner[j] = q;
   }
-
-  std::vector<int> Super;
-  std::vector<char> IsSep;
-  Super.reserve(n + 1);
-  int col = 0;
-  bool newSegment = true;
-  for (int s = 0; s < ns; ++s) {
-    const Separator& sp = seps[s];
-    if (compress[s]) {
-      Super.push_back(sp.first);
-      IsSep.push_back(1);
-      newSegment = true;
-    } else {
-      if (newSegment) {
-        Super.push_back(sp.first);
-        IsSep.push_back(0);
-        col = sp.first + 1;
-        newSegment = false;
-      }
-      for (; col <= sp.last; ++col) {
-        if (Parent[col - 1] != col || cc[col - 1] != cc[col] + 1 ||
-            nchild[col] > 1) {
-          Super.push_back(col);
-          IsSep.push_back(0);
-        }
-      }
-    }
+  for (int q = 0; q < nf; ++q) {
+    const int last = (q + 1 < nf ? f[q + 1].first : n) - 1;
+    const int pc = last >= 0 ? parent[last] : -1;
+    f[q].parent = pc >= 0 ? owner[pc] : -1;
   }
-  int nfsuper = (int)Super.size();
-  Super.push_back(n);
-
-  std::vector<int> SuperMap(n);
-  for (int s = 0; s < nfsuper; ++s)
-    for (int k = Super[s]; k < Super[s + 1]; ++k) SuperMap[k] = s;
-  std::vector<int> Sparent(nfsuper);
-  for (int s = 0; s < nfsuper; ++s) {
-    int par = Parent[Super[s + 1] - 1];
-    Sparent[s] = (par == EMPTY) ? EMPTY : SuperMap[par];
+  const RelaxParams rp{{nrelax0, nrelax1, nrelax2}, {zrelax0, zrelax1, zrelax2}};
+  for (int s = nf - 2; s >= 0; --s) {
+    if (f[s].parent < 0 || f[s].whole || f[s + 1].whole) continue;
+    if (standing(f, f[s].parent) != s + 1) continue;
+    int64_t zeros = 0;
+    if (!relaxed_merge(f[s], f[s + 1], rp, zeros)) continue;
+    f[s].zeros = zeros;
+    f[s].nnz_first = f[s].width + f[s + 1].nnz_first;
+    f[s].width += f[s + 1].width;
+    f[s + 1].taken_by = s;
   }
-
-  std::vector<int> Merged(nfsuper, EMPTY), Nscol(nfsuper), Zeros(nfsuper, 0),
-      Snz(nfsuper);
-  for (int s = 0; s < nfsuper; ++s) {
-    Nscol[s] = Super[s + 1] - Super[s];
-    Snz[s] = cc[Super[s]];
-  }
-  for (int s = nfsuper - 2; s >= 0; --s) {
-    if (Sparent[s] == EMPTY) continue;
-    if (IsSep[s] || IsSep[s + 1]) continue;
-    int ss;
-    for (ss = Sparent[s]; Merged[ss] != EMPTY; ss = Merged[ss]) {
-    }
-    int sparent = ss;
-    for (ss = Sparent[s]; Merged[ss] != EMPTY;) {
-      int snext = Merged[ss];
-      Merged[ss] = sparent;
-      ss = snext;
-    }
-    if (sparent != s + 1) continue;
-    int nscol0 = Nscol[s], nscol1 = Nscol[s + 1];
-    int nsc = nscol0 + nscol1;
-    int totzeros = Zeros[s + 1];
-    bool merge;
-    if (nsc <= nrelax0) {
-      merge = true;
-    } else {
-      double lnz0 = Snz[s], lnz1 = Snz[s + 1];
-      double xnewzeros = nscol0 * (lnz1 + nscol0 - lnz0);
-      int newzeros = nscol0 * (Snz[s + 1] + nscol0 - Snz[s]);
-      if (xnewzeros == 0) {
-        merge = true;
-      } else {
-        double xtotzeros = ((double)totzeros) + xnewzeros;
-        double xns = (double)nsc;
-        double xtotsize = (xns * (xns + 1) / 2) + xns * (lnz1 - nscol1);
-        double z = xtotzeros / xtotsize;
-        totzeros += newzeros;
-        merge = ((nsc <= nrelax1 && z < zrelax0) ||
-                 (nsc <= nrelax2 && z < zrelax1) || (z < zrelax2));
-      }
-    }
-    if (merge) {
-      Zeros[s] = totzeros;
-      Merged[s + 1] = s;
-      Snz[s] = nscol0 + Snz[s + 1];
-      Nscol[s] += Nscol[s + 1];
-    }
-  }
-
+  // rule 4: the standing supernodes in column order
   std::vector<SupernodeSpec> out;
-  std::vector<int> starts;
-  std::vector<char> flags;
-  for (int s = 0; s < nfsuper; ++s)
-    if (Merged[s] == EMPTY) {
-      starts.push_back(Super[s]);
-      flags.push_back(IsSep[s]);
-    }
-  starts.push_back(n);
-  for (size_t k = 0; k + 1 < starts.size(); ++k) {
+  for (int q = 0; q < nf; ++q) {
+    if (f[q].taken_by >= 0) continue;
     SupernodeSpec sp;
-    sp.first = starts[k];
-    sp.last = starts[k + 1] - 1;
-    sp.compress = flags[k] != 0;
+    sp.first = f[q].first;
+    sp.compress = f[q].whole;
+    if (!out.empty()) out.back().last = sp.first - 1;
     out.push_back(sp);
   }
+  if (!out.empty()) out.back().last = n - 1;
   return out;
 }
 

@@ -6,10 +6,8 @@ FactorManager.cpp:58-71), produced by the oracle's C restatement of
 MERSENNETWISTER.h (randInt / randNorm, :170-194).  Tests compare the
 device-side generator and a numpy restatement of the Box-Muller step with it.
 
-partition_poisson2d_8.json: the supernode partition of the 8x8 Poisson
-grid with maxDenseBlock 7 under the build's nested dissection, written as
-(first column, compressed) pairs; tests hold the relaxed-supernode pass
-(NestedDissection.cpp:1266-1667) to it.
+relaxed_supernodes_handworked.json is written by hand (its derivations are
+in tests/test_relaxed_handworked.py), not by this script.
 
 Run from the repository root: python tests/golden/make_goldens.py
 """
@@ -25,20 +23,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 def main():
     from oracle.oracle import gaussian, mt_output
-    import rank_structured_cholesky_amd as R
-    from helpers import symbolic
 
     raw = [int(mt_output(152, k + 1)) for k in range(64)]  # 1-based index
     g = gaussian(152, 64).tolist()
     with open(os.path.join(HERE, "gaussian_seed152.json"), "w") as f:
         json.dump({"seed": 152, "mt_outputs": raw, "normals": g}, f, indent=1)
 
-    A, xyz = R.poisson2d(8)
-    sym = symbolic(A, xyz, max_dense_block=7)
-    part = [[int(a), int(c)] for a, c in zip(sym["first"][:-1], sym["compressed"])]
-    with open(os.path.join(HERE, "partition_poisson2d_8.json"), "w") as f:
-        json.dump({"grid": 8, "max_dense_block": 7, "perm": sym["perm"].tolist(),
-                   "supernodes": part, "n": int(A._nrow)}, f)
 
 
 if __name__ == "__main__":

@@ -76,15 +76,6 @@ def test_threaded_stream_matches_oracle_across_chunks():
     assert np.array_equal(small, dev[:5])
 
 
-def test_partition_golden():
-    gold = json.load(open(os.path.join(GOLD, "partition_poisson2d_8.json")))
-    A, xyz = R.poisson2d(8)
-    sym = symbolic(A, xyz, max_dense_block=gold["max_dense_block"])
-    assert sym["perm"].tolist() == gold["perm"]
-    got = [[int(a), int(c)] for a, c in zip(sym["first"][:-1], sym["compressed"])]
-    assert got == gold["supernodes"]
-
-
 def exact_solve(A, b):
     return spla.spsolve(A.to_scipy().tocsc(), b)
 

@@ -105,86 +105,72 @@ def test_etree_and_column_counts_match_brute_force(name, build, small):
 
 
 def relaxed_supernodes_restated(parent, cc, seps, sep_parent, max_block):
-    """Independent restatement of NestedDissection::BuildRelaxedSupernodes
-    (NestedDissection.cpp:1266-1667) from its documented semantics."""
+    """The relaxed-supernode rule restated from its statement (the four
+    rules in the header of ordering.cpp's partition code), as a list of
+    column groups merged pairwise: an independent second implementation for
+    the problems below, whose hand-worked counterparts are in
+    test_relaxed_handworked.py."""
     n = len(parent)
-    nchild = np.zeros(n, int)
-    for p in parent:
-        if p >= 0:
-            nchild[p] += 1
-    comp = [False] * len(seps)
-    for s, (size, first, last, nch) in enumerate(seps):
+    kids = np.bincount([p for p in parent if p >= 0], minlength=n)
+    # rule 1: compressed components (large with children) and all ancestors
+    whole = set()
+    for c, (size, first, last, nch) in enumerate(seps):
         if size > max_block and nch > 0:
-            a = s
-            while a >= 0:
-                comp[a] = True
-                a = sep_parent[a]
-    starts, flags = [], []
-    col = 0
-    new_segment = True
-    for s, (size, first, last, nch) in enumerate(seps):
-        if comp[s]:
-            starts.append(first)
-            flags.append(True)
-            new_segment = True
+            while c >= 0 and c not in whole:
+                whole.add(c)
+                c = sep_parent[c]
+    # rule 2: groups of columns (first column, columns, whole)
+    groups = []
+    run = False
+    for c, (size, first, last, nch) in enumerate(seps):
+        if c in whole:
+            groups.append([first, 0, True])
+            run = False
             continue
-        if new_segment:
-            starts.append(first)
-            flags.append(False)
-            col = first + 1
-            new_segment = False
-        while col <= last:
-            if (parent[col - 1] != col or cc[col - 1] != cc[col] + 1 or nchild[col] > 1):
-                starts.append(col)
-                flags.append(False)
-            col += 1
-    nf = len(starts)
-    starts.append(n)
-    smap = np.empty(n, int)
-    for s in range(nf):
-        smap[starts[s]:starts[s + 1]] = s
-    spar = [smap[parent[starts[s + 1] - 1]] if parent[starts[s + 1] - 1] >= 0 else -1
-            for s in range(nf)]
-    merged = [-1] * nf
-    nscol = [starts[s + 1] - starts[s] for s in range(nf)]
-    zeros = [0] * nf
-    snz = [cc[starts[s]] for s in range(nf)]
-    for s in range(nf - 2, -1, -1):
-        if spar[s] == -1 or flags[s] or flags[s + 1]:
+        for j in range(first, last + 1):
+            chained = run and parent[j - 1] == j and cc[j - 1] == cc[j] + 1 and kids[j] <= 1
+            if not chained:
+                groups.append([j, 0, False])
+            run = True
+    bounds = [g[0] for g in groups] + [n]
+    owner = {}
+    for q, g in enumerate(groups):
+        g[1] = bounds[q + 1] - bounds[q]
+        for j in range(bounds[q], bounds[q + 1]):
+            owner[j] = q
+    up = [owner[parent[bounds[q + 1] - 1]] if bounds[q + 1] > 0 and
+          parent[bounds[q + 1] - 1] >= 0 else None for q in range(len(groups))]
+    width = [g[1] for g in groups]
+    lead = [cc[g[0]] if g[0] < n else 0 for g in groups]
+    zeros = [0] * len(groups)
+    into = {}  # group -> the group that absorbed it
+
+    def rep(q):
+        while q in into:
+            q = into[q]
+        return q
+
+    # rule 3, from the second-to-last group down
+    for s in range(len(groups) - 2, -1, -1):
+        if up[s] is None or groups[s][2] or groups[s + 1][2] or rep(up[s]) != s + 1:
             continue
-        ss = spar[s]
-        while merged[ss] != -1:
-            ss = merged[ss]
-        sp = ss
-        ss = spar[s]
-        while merged[ss] != -1:
-            nxt = merged[ss]
-            merged[ss] = sp
-            ss = nxt
-        if sp != s + 1:
-            continue
-        n0, n1 = nscol[s], nscol[s + 1]
-        ns = n0 + n1
-        tot = zeros[s + 1]
-        if ns <= 4:
-            merge = True
+        w0, w1, k0, k1 = width[s], width[s + 1], lead[s], lead[s + 1]
+        t = w0 + w1
+        new_zeros = w0 * (k1 + w0 - k0)
+        if t <= 4 or new_zeros == 0:
+            ok, z_after = True, zeros[s + 1]
         else:
-            newz = n0 * (snz[s + 1] + n0 - snz[s])
-            if newz == 0:
-                merge = True
-            else:
-                xtot = tot + newz
-                xsize = ns * (ns + 1) / 2 + ns * (snz[s + 1] - n1)
-                z = xtot / xsize
-                tot += newz
-                merge = (ns <= 16 and z < 0.8) or (ns <= 48 and z < 0.1) or z < 0.05
-        if merge:
-            zeros[s] = tot
-            merged[s + 1] = s
-            snz[s] = n0 + snz[s + 1]
-            nscol[s] += nscol[s + 1]
-    out = [(starts[s], flags[s]) for s in range(nf) if merged[s] == -1]
-    return out
+            ratio = (float(zeros[s + 1]) + float(w0) * (float(k1) + w0 - float(k0))) / \
+                (float(t) * (t + 1) / 2 + float(t) * (float(k1) - w1))
+            ok = (t <= 16 and ratio < 0.8) or (t <= 48 and ratio < 0.1) or ratio < 0.05
+            z_after = zeros[s + 1] + new_zeros
+        if ok:
+            zeros[s] = z_after
+            lead[s] = w0 + k1
+            width[s] = t
+            into[s + 1] = s
+    # rule 4
+    return [(groups[q][0], groups[q][2]) for q in range(len(groups)) if q not in into]
 
 
 @pytest.mark.parametrize("name,build,mdb", [

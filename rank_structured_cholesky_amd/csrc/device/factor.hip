@@ -199,12 +199,14 @@ class Engine final : public FactorEngine {
   // dataflow apply (one persistent launch per application)
   DBuf dapst_, dtrace_;
   DBuf dstptr_, dstnodes_, dstcols_, dpanel_;
+  DBuf dmsptr_, dmsnodes_, dmscols_;  // subtrees of the multi-right-hand-side list
   int nsub_ = 0;
-  dev::SubtreeArgs subtree_args(const double* B, int ldb) const {
+  // multi: the subtrees of the multi-right-hand-side list
+  dev::SubtreeArgs subtree_args(const double* B, int ldb, bool multi = false) const {
     dev::SubtreeArgs a{};
-    a.ptr = dstptr_.as<int>();
-    a.nodes = dstnodes_.as<int>();
-    a.cols = dstcols_.as<int>();
+    a.ptr = (multi ? dmsptr_ : dstptr_).as<int>();
+    a.nodes = (multi ? dmsnodes_ : dstnodes_).as<int>();
+    a.cols = (multi ? dmscols_ : dstcols_).as<int>();
     a.panel = dpanel_.as<int64_t>();
     a.vn = dvnodes_.as<dev::VecNode>();
     a.children = dchildren_.as<int>();
@@ -1396,13 +1398,20 @@ void Engine<T>::setup_static() {
     // and for sixteen right-hand sides at 64^3 and 128^3)
     int64_t lim = 16384;
     if (const char* e = std::getenv("RSC_SUBTREE_ELEMS")) lim = std::atoll(e);
-    build_solve_layout(P, SL, lim);
+    // multi-right-hand-side subtree tasks: the supernode limits (front and
+    // y in LDS) bound them; RSC_MS_SUBTREE_ELEMS caps their panel elements
+    int64_t mlim = 1 << 20;
+    if (const char* e = std::getenv("RSC_MS_SUBTREE_ELEMS")) mlim = std::atoll(e);
+    build_solve_layout(P, SL, lim, mlim);
   }
   phase("solve layout");
   nsub_ = (int)SL.sub.root.size();
   dstptr_.upload(SL.sub.ptr, st_);
   dstnodes_.upload(SL.sub.nodes.empty() ? std::vector<int>{0} : SL.sub.nodes, st_);
   dstcols_.upload(SL.sub.cols.empty() ? std::vector<int>{0, -1} : SL.sub.cols, st_);
+  dmsptr_.upload(SL.sub_multi.ptr, st_);
+  dmsnodes_.upload(SL.sub_multi.nodes.empty() ? std::vector<int>{0} : SL.sub_multi.nodes, st_);
+  dmscols_.upload(SL.sub_multi.cols.empty() ? std::vector<int>{0, -1} : SL.sub_multi.cols, st_);
   dpanel_.upload(SL.panel, st_);
   dchildren_.upload(SL.children, st_);
   drelmap_.upload(SL.relmap, st_);
@@ -1452,14 +1461,14 @@ void Engine<T>::setup_static() {
       std::thread second([&]() {
         try {
           df_pending2_.reset(new DataflowTasks());
-          build_dataflow_tasks(plan_, sl_, *df_pending2_, false, many);
+          build_dataflow_tasks(plan_, sl_, *df_pending2_, &sl_.sub_multi, many);
         } catch (...) {
           err2 = std::current_exception();
         }
       });
       try {
         df_pending_.reset(new DataflowTasks());
-        build_dataflow_tasks(plan_, sl_, *df_pending_, true, one);
+        build_dataflow_tasks(plan_, sl_, *df_pending_, &sl_.sub, one);
       } catch (...) {
         df_err_ = std::current_exception();
       }
@@ -1854,7 +1863,7 @@ void Engine<T>::run_apply(int k, const double* B, int ldb, double* X, int ldx,
     a.X = X;
     a.ldx = ldx;
     a.trace = dtrace_.p ? dtrace_.as<unsigned long long>() : nullptr;
-    a.sub = subtree_args(B, ldb);
+    a.sub = subtree_args(B, ldb, df_last_ == 1);
     dev::launch_df_reset(D.cnt.as<unsigned>(), D.ngroups + 1, s);
     dev::launch_apply_dataflow<T>(F(), a, vb, k, D.grid, s);
     return;
@@ -1892,8 +1901,7 @@ void Engine<T>::raise_apply_status(const int* st) {
   // clear the words so the next apply starts clean, then report
   const int z[2] = {0, 0};
   RSC_HIP(hipMemcpy(dapst_.p, z, sizeof(z), hipMemcpyHostToDevice));
-  throw DeviceError(st[0] == 2 ? "preconditioner apply: subtree task in the multi-column list"
-                               : "preconditioner apply: a dependency wait timed out");
+  throw DeviceError("preconditioner apply: a dependency wait timed out");
 }
 
 template <typename T>

@@ -1102,6 +1102,152 @@ template void launch_subtrees<double>(const double*, const SubtreeArgs&, int, bo
 template void launch_subtrees<float>(const float*, const SubtreeArgs&, int, bool, VecBufs, int,
                                      hipStream_t);
 
+// ---- multi-right-hand-side subtree tasks -------------------------------------
+// One workgroup runs a subtree of small uncompressed supernodes for sixteen
+// right-hand sides per pass.  A supernode's whole front vector (ncols + R
+// entries) and y (ncols) sit in LDS; every product runs on the FP64 matrix
+// cores (sixteen right-hand sides per MFMA 16x16x4), the matrix loaded once
+// per pass.  Per supernode (panel [Lc^{-T}; Zp]):
+//   forward:  fv_c = b[perm], fv_R = 0 + children's fv_R (written by this
+//             workgroup);  y = Lc^{-1} fv_c;  fv_R -= Zp fv_c;  z = Lc^{-T} y
+//   backward: x_R = w[rows];  x_c = z - Zp^T x_R
+// The stored Lc^{-T} block is exactly zero below its diagonal, so the
+// products run over the full ncols x ncols block.  z, w and the subtree
+// root's fv_R are written with device-scope stores (other workgroups read
+// them); the other supernodes' fv_R stay with this workgroup.
+constexpr int kMsFld = kMsMaxFront + 1;  // front row stride in LDS (odd: banks)
+constexpr int kMsYld = kMsMaxCols + 1;
+constexpr int kMsLds = kMfmaRhs * kMsFld + kMfmaRhs * kMsYld;  // doubles
+
+// out(i, r, v) for i < rows, r < kb: v = sum_j A(i, j) x(r, j), j < cols,
+// with A(i, j) = M[i + j ld] (TRANS false) or M[j + i ld] (TRANS true) and
+// x in LDS (right-hand side r at xs + r * xld).  Wave w takes the 16-row
+// blocks w, w + 4, ...; columns go in chunks of 32 whose eight loads per lane
+// are issued together, the sum over j in groups of four in increasing order
+// for every right-hand side (so a column's bits do not depend on kb).
+template <typename T, bool TRANS, typename Out>
+__device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
+                           const double* xs, int xld, int kb, Out out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, kq = lane >> 4;
+  for (int i0 = wave * 16; i0 < rows; i0 += 64) {
+    const int i = i0 + li;
+    const bool iok = i < rows;
+    d4v acc = {0, 0, 0, 0};
+    for (int j0 = 0; j0 < cols; j0 += 32) {
+      T m[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + 4 * u + kq;
+        const bool ok = iok && j < cols;
+        const int ii = ok ? i : 0, jj = ok ? j : 0;
+        const T v = TRANS ? M[jj + (int64_t)ii * ld] : M[ii + (int64_t)jj * ld];
+        m[u] = ok ? v : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (j0 + 4 * u >= cols) break;
+        const int j = j0 + 4 * u + kq;
+        const double xv = (j < cols && li < kb) ? xs[li * xld + j] : 0.0;
+        acc = mfma64((double)m[u], xv, acc);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = i0 + kq + 4 * v;
+      if (row < rows && li < kb) out(row, li, acc[v]);
+    }
+  }
+}
+
+template <typename T>
+__device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                           const VecBufs& b, int k, double* lds) {
+  const int tid = threadIdx.x;
+  double* Fr = lds;                       // [16][kMsFld] front vector
+  double* Yr = lds + kMfmaRhs * kMsFld;   // [16][kMsYld] y
+  const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
+  const int rootn = uni(a.nodes[q1 - 1]);
+  for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
+    const int kb = min(kMfmaRhs, k - c0);
+    for (int qi = q0; qi < q1; ++qi) {
+      const int j = uni(a.nodes[qi]);
+      const VecNode S = uniform_node(a.vn[j]);
+      const int64_t P = uni64(a.panel[j]);
+      const int c = S.ncols, R = S.R, ld = c + R;
+      const T* __restrict__ Lt = F + P;
+      for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
+        const int r = idx / ld, p = idx - r * ld;
+        double v = 0.0;
+        if (r < kb && p < c) v = a.B[a.perm[S.first + p] + (int64_t)(c0 + r) * a.ldb];
+        Fr[r * kMsFld + p] = v;
+      }
+      __syncthreads();
+      for (int q = 0; q < S.nchild; ++q) {
+        const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
+        const int* rm = a.relmap + C.relmap_off;
+        const double* fc = b.fv + C.fv_off + C.ncols;
+        for (int idx = tid; idx < kb * C.R; idx += 256) {
+          const int r = idx / C.R, t = idx - r * C.R;
+          Fr[r * kMsFld + rm[t]] += fc[(int64_t)(c0 + r) * b.fvs + t];
+        }
+        __syncthreads();
+      }
+      // y = Lc^{-1} fv_c (y_i = sum_j Lt[j, i] fv_c[j]) and fv_R -= Zp fv_c
+      ms_product<T, true>(Lt, ld, c, c, Fr, kMsFld, kb,
+                          [&](int i, int r, double v) { Yr[r * kMsYld + i] = v; });
+      ms_product<T, false>(Lt + c, ld, R, c, Fr, kMsFld, kb,
+                           [&](int i, int r, double v) { Fr[r * kMsFld + c + i] -= v; });
+      __syncthreads();
+      // z = Lc^{-T} y
+      double* zb = b.z + (int64_t)c0 * b.n + S.first;
+      ms_product<T, false>(Lt, ld, c, c, Yr, kMsYld, kb, [&](int i, int r, double v) {
+        vst<true>(zb + (int64_t)r * b.n + i, v);
+      });
+      double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
+      const bool root = j == rootn;
+      for (int idx = tid; idx < kb * R; idx += 256) {
+        const int r = idx / R, t = idx - r * R;
+        const double v = Fr[r * kMsFld + c + t];
+        if (root) vst<true>(fvR + (int64_t)r * b.fvs + t, v);
+        else fvR[(int64_t)r * b.fvs + t] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T>
+__device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
+                            const VecBufs& b, int k, double* lds) {
+  const int tid = threadIdx.x;
+  double* Fr = lds;  // [16][kMsFld]: x_R at positions c .. c + R
+  const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
+  for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
+    const int kb = min(kMfmaRhs, k - c0);
+    for (int qi = q1 - 1; qi >= q0; --qi) {
+      const int j = uni(a.nodes[qi]);
+      const VecNode S = uniform_node(a.vn[j]);
+      const int64_t P = uni64(a.panel[j]);
+      const int c = S.ncols, R = S.R, ld = c + R;
+      const int* rw = a.rows + S.rows_off;
+      const double* wb = b.w + (int64_t)c0 * b.n;
+      for (int idx = tid; idx < kb * R; idx += 256) {
+        const int r = idx / R, t = idx - r * R;
+        Fr[r * kMsFld + c + t] = vld<true>(wb + (int64_t)r * b.n + rw[t]);
+      }
+      __syncthreads();
+      // x_c = z_c - Zp^T x_R
+      const double* zb = b.z + (int64_t)c0 * b.n + S.first;
+      double* xb = b.w + (int64_t)c0 * b.n + S.first;
+      ms_product<T, true>(F + P + c, ld, c, R, Fr + c, kMsFld, kb, [&](int i, int r, double v) {
+        vst<true>(xb + (int64_t)r * b.n + i, vld<true>(zb + (int64_t)r * b.n + i) - v);
+      });
+      __syncthreads();
+    }
+  }
+}
+
 constexpr int DF_SCATTER_CHUNK = kScatterChunk;  // x entries per scatter task (8 per thread)
 static_assert(DF_SCATTER_CHUNK % (8 * 256) == 0, "scatter chunk: eight entries per thread per pass");
 
@@ -1140,8 +1286,9 @@ struct DfLdsScalar {
 };
 static_assert(kXlds >= VCH, "the assembly chunk fits the x staging area");
 static_assert(kXlds >= kColRowsMax, "a column task's x fits the staging area");
+constexpr int kMultiLds = kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH;
 struct DfLdsMulti {
-  double mf[kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH];
+  double mf[kMultiLds > kMsLds ? kMultiLds : kMsLds];
 };
 template <bool MULTI>
 struct DfLds {
@@ -1218,9 +1365,8 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
       else gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.xl);
     } else if (type == 4 || type == 5) {
       if constexpr (MULTI) {
-        // not in the multi-right-hand-side list: report instead of running
-        if (tid == 0)
-          __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (type == 4) ms_forward<T>(F, a.sub, ta, b, k, lds.mf);
+        else ms_backward<T>(F, a.sub, ta, b, k, lds.mf);
       } else if (type == 4) {
         subtree_forward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
       } else {

@@ -12,7 +12,8 @@
 
 namespace rsc {
 
-void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S) {
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, int max_cols,
+                    int max_rows, int max_front) {
   const int ns = (int)P.nodes.size();
   S.of.assign(ns, -1);
   S.ptr.assign(1, 0);
@@ -25,7 +26,8 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S) {
   std::vector<int64_t> cost(ns, 0);
   for (int j = 0; j < ns; ++j) {
     const Supernode& s = P.nodes[j];
-    bool e = !s.compress && s.ncols <= kSubtreeMaxCols && s.R() <= kSubtreeMaxRows;
+    bool e = !s.compress && s.ncols <= max_cols && s.R() <= max_rows &&
+             s.ncols + s.R() <= max_front;
     int64_t c = (int64_t)s.ncols * (s.ncols + s.R());
     for (int ch : s.children) {
       if (ch >= j) throw std::logic_error("supernodes not in postorder");
@@ -66,7 +68,8 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S) {
   }
 }
 
-void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems) {
+void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems,
+                        int64_t multi_subtree_elems) {
   const int ns = (int)P.nodes.size();
   L.children.clear();
   L.relmap.clear();
@@ -135,6 +138,7 @@ void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems) {
   for (int j = 0; j < ns; ++j)
     if (!P.nodes[j].compress) L.panel[j] = P.nodes[j].off_panel;
   build_subtrees(P, subtree_elems, L.sub);
+  build_subtrees(P, multi_subtree_elems, L.sub_multi, kMsMaxCols, kMsMaxFront, kMsMaxFront);
 }
 
 // ---- per-node stages ------------------------------------------------------
@@ -545,12 +549,16 @@ struct CostModel {
       case 3:
         return multi ? 12.5 * k : 10.0;
       default: {
-        double kb = 0, nodes = 0;
+        double kb = 0, nodes = 0, front = 0;
         for (int q = SB.ptr[t.a]; q < SB.ptr[t.a + 1]; ++q) {
           const Supernode& s = P.nodes[SB.nodes[q]];
           kb += (double)s.ncols * (s.ncols + s.R()) * e / 1000.0;
+          front += s.ncols + s.R();
           nodes += 1;
         }
+        if (multi)  // sixteen right-hand sides per pass, fronts in LDS
+          return t.type == 4 ? 6.0 + (4.5 * nodes + 0.01 * front + 0.3 * kb) * passes
+                             : 4.0 + (3.0 * nodes + 0.01 * front + 0.2 * kb) * passes;
         const double sp = std::ceil(k / kSubtreeRhs);
         return t.type == 4 ? 10.9 + (1.35 * kb + 10.7 * nodes) * sp
                            : 3.8 + (0.8 * kb + 4.3 * nodes) * sp;
@@ -659,7 +667,7 @@ double order_by_list_schedule(std::vector<dev::DfTask>& tasks, const std::vector
 }  // namespace
 
 void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out,
-                          bool use_subtrees, const TicketModel& tm) {
+                          const Subtrees* subtrees, const TicketModel& tm) {
   const int ns = (int)P.nodes.size();
   const int VCH = kSolveChunk, SCH = kScatterChunk;
   std::vector<dev::GvTask> gv;
@@ -726,7 +734,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   Subtrees none;
   none.of.assign(ns, -1);
   none.ptr.assign(1, 0);
-  const Subtrees& SB = use_subtrees ? L.sub : none;
+  const Subtrees& SB = subtrees ? *subtrees : none;
   const int nsub = (int)SB.root.size();
   std::vector<std::vector<int>> lnodes(P.levels.size());
   for (size_t lv = 0; lv < P.levels.size(); ++lv)

@@ -92,6 +92,11 @@ constexpr int kScatterChunk = 2048;  // vector entries per scatter task
 constexpr int kSubtreeMaxCols = 256;
 constexpr int kSubtreeMaxRows = 512;
 constexpr int kSubtreeRhs = 4;  // right-hand sides per pass over a subtree's panels
+// Subtree tasks of the multi-right-hand-side apply: a supernode's whole
+// front vector (ncols + R entries) and y (ncols entries) for sixteen
+// right-hand sides live in LDS, so these limits are per supernode.
+constexpr int kMsMaxCols = 32;
+constexpr int kMsMaxFront = 224;
 
 // Subtrees of uncompressed supernodes handled by one task each.
 struct Subtrees {
@@ -101,8 +106,12 @@ struct Subtrees {
   std::vector<int> cols;        // per subtree: first column, column count
                                 // (-1 when the columns are not contiguous)
 };
-// Maximal subtrees whose panels hold at most max_elems elements in total.
-void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S);
+// Maximal subtrees of uncompressed supernodes with at most max_cols columns
+// and max_rows off-diagonal rows (and ncols + R <= max_front) each, whose
+// panels hold at most max_elems elements in total.
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S,
+                    int max_cols = kSubtreeMaxCols, int max_rows = kSubtreeMaxRows,
+                    int max_front = kSubtreeMaxCols + kSubtreeMaxRows);
 
 struct SolvePhase {
   int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks,
@@ -124,7 +133,8 @@ struct SolveLayout {
   std::vector<std::vector<int64_t>> tv_h;
   std::vector<int64_t> panel;  // factor offset of each uncompressed panel (-1 otherwise)
   int64_t fv_stride = 1, tv_stride = 1;
-  Subtrees sub;
+  Subtrees sub;        // subtree tasks of the single-right-hand-side list
+  Subtrees sub_multi;  // and of the multi-right-hand-side list
 };
 
 struct LevelTasks {
@@ -143,8 +153,10 @@ struct DataflowTasks {
   double est_span_us = 0;          // makespan of the list schedule (0: level order)
 };
 
-// Vector layout; subtree_elems = 0 disables subtree tasks.
-void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems);
+// Vector layout; subtree_elems = 0 disables the single-right-hand-side
+// subtree tasks, multi_subtree_elems = 0 the multi-right-hand-side ones.
+void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems,
+                        int64_t multi_subtree_elems = 0);
 void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
 // Ticket order of a dataflow list.  workers = 0 keeps the level order;
 // otherwise the list is ordered by a list schedule on that many workgroups
@@ -159,10 +171,10 @@ struct TicketModel {
   int64_t wide_elems = 0;  // products this large are split finer (0: 2^20 elements,
                            // and only for nrhs > 1)
 };
-// use_subtrees = false: every supernode as ordinary tasks (the list of the
-// multi-right-hand-side apply, whose matrix-core bodies do better without
-// subtree tasks)
+// subtrees: the subtree tasks of the list (L.sub for the single-right-hand-
+// side list, L.sub_multi for the multi-right-hand-side list), or null for
+// every supernode as ordinary tasks
 void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& out,
-                          bool use_subtrees = true, const TicketModel& tm = TicketModel());
+                          const Subtrees* subtrees, const TicketModel& tm = TicketModel());
 
 }  // namespace rsc

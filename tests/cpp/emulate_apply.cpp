@@ -13,10 +13,12 @@
 // dependency or out-of-range index shows up.
 //
 //   emulate_apply <problem> <N> <max_dense_block> <max_diagonal_block> <nrhs> <threads>
-//                 [subtree_elems [workers [wide_elems]]]
+//                 [subtree_elems [workers [wide_elems [list]]]]
 //   workers: ticket order by a list schedule on that many workgroups
 //   (default 1024; 0 keeps the level order); wide_elems: products split
-//   finer from this many elements (default 2^20)
+//   finer from this many elements (default 2^20); list: "multi" runs the
+//   multi-right-hand-side list (its own subtree tasks, supernode limits of
+//   the sixteen-column LDS bodies) instead of the single one
 //   problem: p3d (7-point log-normal Poisson), p2d, el (elasticity)
 // Prints "ok ..." and exits 0 on success.
 #include <atomic>
@@ -70,6 +72,8 @@ struct State {
       part{{}, "part"}, z{{}, "z"};
   int n = 0, k = 1;
   int64_t ps = 1;
+  const Subtrees* dsub = nullptr;  // subtree tasks of the dataflow list
+  bool multi = false;
   std::vector<int> perm;
   void reset_work() {
     std::fill(w.v.begin(), w.v.end(), std::nan(""));
@@ -203,8 +207,7 @@ void scatter(State& S, int chunk) {
 
 // subtree tasks: every node of subtree sb, children first (forward) or
 // parents first (backward); panel = [Lc^{-T}; Zp]
-void subtree(State& S, int sb, bool fwd) {
-  const Subtrees& T = S.L->sub;
+void subtree(State& S, int sb, bool fwd, const Subtrees& T, bool multi) {
   const int q0 = chk(T.ptr, sb, "st ptr"), q1 = chk(T.ptr, sb + 1, "st ptr");
   for (int step = 0; step < q1 - q0; ++step) {
     const int j = chk(T.nodes, fwd ? q0 + step : q1 - 1 - step, "st nodes");
@@ -213,6 +216,8 @@ void subtree(State& S, int sb, bool fwd) {
     if (P < 0) fail("subtree node without a panel");
     const int c = N.ncols, R = N.R, ld = c + R;
     if (c > kSubtreeMaxCols || R > kSubtreeMaxRows) fail("subtree node too large");
+    if (multi && (c > kMsMaxCols || c + R > kMsMaxFront))
+      fail("multi subtree node exceeds the LDS front");
     for (int col = 0; col < S.k; ++col) {
       const int64_t fb = N.fv_off + (int64_t)col * S.L->fv_stride;
       const int64_t vb = (int64_t)col * S.n;
@@ -265,8 +270,8 @@ void run_df_task(State& S, const DataflowTasks& D, const dev::DfTask& t) {
     case 1: gather(S, t.a, t.b); break;
     case 2: gemv(S, chk(D.gv, t.a, "df gv")); break;
     case 3: scatter(S, t.a); break;
-    case 4: subtree(S, t.a, true); break;
-    case 5: subtree(S, t.a, false); break;
+    case 4: subtree(S, t.a, true, *S.dsub, S.multi); break;
+    case 5: subtree(S, t.a, false, *S.dsub, S.multi); break;
     default: fail("bad task type");
   }
 }
@@ -302,7 +307,8 @@ int main(int argc, char** argv) {
   build_plan(A, xyz.empty() ? nullptr : xyz.data(), opt, nullptr, P);
   SolveLayout L;
   const int64_t sub_elems = argc > 7 ? std::atoll(argv[7]) : std::max<int64_t>(256, P.factor_elems / 64);
-  build_solve_layout(P, L, sub_elems);
+  const bool multi = argc > 10 && std::string(argv[10]) == "multi";
+  build_solve_layout(P, L, sub_elems, multi ? (int64_t)1 << 20 : 0);
   LevelTasks LT;
   build_level_tasks(P, L, LT);
   DataflowTasks D;
@@ -311,7 +317,7 @@ int main(int argc, char** argv) {
   tm.wide_elems = argc > 9 ? std::atoll(argv[9]) : 0;
   tm.nrhs = k;
   const auto tb0 = std::chrono::steady_clock::now();
-  build_dataflow_tasks(P, L, D, true, tm);
+  build_dataflow_tasks(P, L, D, multi ? &L.sub_multi : &L.sub, tm);
   const double build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb0).count();
 
@@ -321,11 +327,25 @@ int main(int argc, char** argv) {
   S.n = P.n;
   S.k = k;
   S.perm = P.perm;
+  S.dsub = multi ? &L.sub_multi : &L.sub;
+  S.multi = multi;
   S.ps = std::max<int64_t>(std::max(LT.slots, D.slots) * 32, 1);
   std::mt19937_64 g(99);
   std::uniform_real_distribution<double> U(-1.0, 1.0);
   S.F.v.resize(std::max<int64_t>(P.factor_elems, 1));
   for (double& x : S.F.v) x = U(g) * 0.1;
+  // the stored triangular inverses (L^{-T}) are exactly zero below their
+  // diagonals, as the factor leaves them
+  auto zero_lower = [&](int64_t off, int b, int ld) {
+    for (int j = 0; j < b; ++j)
+      for (int i = j + 1; i < b; ++i) S.F.at(off + i + (int64_t)j * ld) = 0.0;
+  };
+  for (const Supernode& s : P.nodes) {
+    if (!s.compress) zero_lower(s.off_panel, s.ncols, s.ncols + s.R());
+    else
+      for (const HNode& h : s.hnodes)
+        if (h.leaf()) zero_lower(h.off_leaf, h.r1 - h.r0, h.r1 - h.r0);
+  }
   S.B.v.resize((size_t)S.n * k);
   for (double& x : S.B.v) x = U(g);
   S.X.v.resize((size_t)S.n * k);
@@ -343,7 +363,7 @@ int main(int argc, char** argv) {
   auto run_level = [&](const std::vector<SolvePhase>& ph, int lv) {
     for (const SolvePhase& p : ph) {
       if (p.kind == 4 || p.kind == 5) {
-        for (int sb = 0; sb < p.count; ++sb) subtree(S, sb, p.kind == 4);
+        for (int sb = 0; sb < p.count; ++sb) subtree(S, sb, p.kind == 4, L.sub, false);
       } else if (p.kind == 0 || p.kind == 1) {
         for (int j : LT.level_nodes[lv]) {
           const int nch = (P.nodes[j].ncols + P.nodes[j].R() + kSolveChunk - 1) / kSolveChunk;
@@ -462,7 +482,7 @@ int main(int argc, char** argv) {
   std::printf(
       "ok%s n=%d nodes=%d levels=%d subtrees=%zu level_tasks=%zu df_tasks=%zu groups=%d "
       "fused_row_tasks=%d df_build_ms=%.1f\n",
-      exact ? "" : " (split rows)", S.n, (int)P.nodes.size(), nlev, L.sub.root.size(),
+      exact ? "" : " (split rows)", S.n, (int)P.nodes.size(), nlev, S.dsub->root.size(),
       LT.gv.size(), D.tasks.size(), ng, fused, build_ms);
   return 0;
 }

@@ -40,6 +40,12 @@ def built():
     "p2d 40 10 16 1 8 1000 1",  # list schedule on one workgroup
     "p3d 32 2000 400 1 8 16384 1024 20000",  # wide products split finer
     "p3d 24 63 128 2 8 16384 4096 5000",  # the same with compressed separators
+    # the multi-right-hand-side list with its sixteen-column subtree tasks
+    "p3d 20 63 50 16 8 16384 1024 0 multi",
+    "p3d 24 63 128 17 8 16384 1024 0 multi",  # two passes of right-hand sides
+    "p2d 40 10 16 3 8 16384 64 0 multi",
+    "el 5 30 40 2 8 16384 1024 0 multi",
+    "p3d 16 63 64 1 8 16384 0 0 multi",  # level order
 ])
 def test_dataflow_schedule_matches_level_schedule(args):
     r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)

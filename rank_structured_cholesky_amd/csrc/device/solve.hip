@@ -1118,11 +1118,19 @@ template void launch_subtrees<float>(const float*, const SubtreeArgs&, int, bool
 constexpr int kMsFld = kMsMaxFront + 1;  // front row stride in LDS (odd: banks)
 constexpr int kMsYld = kMsMaxCols + 1;
 constexpr int kMsLds = kMfmaRhs * kMsFld + kMfmaRhs * kMsYld;  // doubles
+// Larger supernodes (up to kMlMaxCols columns, any R): fv_c and y in LDS
+// ([16][kMlFld] each), fv_R in device memory; the backward product stages
+// x_R through LDS in chunks of kMlChunk rows.
+constexpr int kMlFld = kMlMaxCols + 1;
+constexpr int kMlChunk = 256;
+static_assert(2 * kMfmaRhs * kMlFld <= kMsLds, "fv_c and y of a larger supernode fit");
+static_assert(kMfmaRhs * (kMlChunk + 1) <= kMsLds, "an x_R chunk fits");
+static_assert(kMlMaxCols <= 128, "at most two 16-row blocks per wave");
 
 // out(i, r, v) for i < rows, r < kb: v = sum_j A(i, j) x(r, j), j < cols,
 // with A(i, j) = M[i + j ld] (TRANS false) or M[j + i ld] (TRANS true) and
 // x in LDS (right-hand side r at xs + r * xld).  Wave w takes the 16-row
-// blocks w, w + 4, ...; columns go in chunks of 32 whose eight loads per lane
+// blocks w, w + 4, ...; columns go in chunks of 16 whose four loads per lane
 // are issued together, the sum over j in groups of four in increasing order
 // for every right-hand side (so a column's bits do not depend on kb).
 template <typename T, bool TRANS, typename Out>
@@ -1134,10 +1142,10 @@ __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
     const int i = i0 + li;
     const bool iok = i < rows;
     d4v acc = {0, 0, 0, 0};
-    for (int j0 = 0; j0 < cols; j0 += 32) {
-      T m[8];
+    for (int j0 = 0; j0 < cols; j0 += 16) {
+      T m[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int j = j0 + 4 * u + kq;
         const bool ok = iok && j < cols;
         const int ii = ok ? i : 0, jj = ok ? j : 0;
@@ -1145,7 +1153,7 @@ __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
         m[u] = ok ? v : T(0);
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 4; ++u) {
         if (j0 + 4 * u >= cols) break;
         const int j = j0 + 4 * u + kq;
         const double xv = (j < cols && li < kb) ? xs[li * xld + j] : 0.0;
@@ -1158,6 +1166,123 @@ __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
       if (row < rows && li < kb) out(row, li, acc[v]);
     }
   }
+}
+
+// Forward step of a larger supernode of a multi subtree (fv_R in device
+// memory, plain accesses except the subtree root's final values).
+template <typename T>
+__device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
+                                 const SubtreeArgs& a, const VecBufs& b, int c0, int kb,
+                                 bool root, double* lds) {
+  const int tid = threadIdx.x;
+  const int c = S.ncols, R = S.R, ld = c + R;
+  double* Fc = lds;                      // [16][kMlFld] fv_c
+  double* Yl = lds + kMfmaRhs * kMlFld;  // [16][kMlFld] y
+  double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
+  for (int idx = tid; idx < kMfmaRhs * c; idx += 256) {
+    const int r = idx / c, p = idx - r * c;
+    Fc[r * kMlFld + p] = r < kb ? a.B[a.perm[S.first + p] + (int64_t)(c0 + r) * a.ldb] : 0.0;
+  }
+  for (int idx = tid; idx < kb * R; idx += 256) {
+    const int r = idx / R, t = idx - r * R;
+    fvR[(int64_t)r * b.fvs + t] = 0.0;
+  }
+  __syncthreads();
+  for (int q = 0; q < S.nchild; ++q) {
+    const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
+    const int* rm = a.relmap + C.relmap_off;
+    const double* fc = b.fv + C.fv_off + C.ncols + (int64_t)c0 * b.fvs;
+    for (int idx = tid; idx < kb * C.R; idx += 256) {
+      const int r = idx / C.R, t = idx - r * C.R;
+      const int p = rm[t];
+      const double v = fc[(int64_t)r * b.fvs + t];
+      if (p < c) Fc[r * kMlFld + p] += v;
+      else fvR[(int64_t)r * b.fvs + p - c] += v;
+    }
+    __syncthreads();
+  }
+  ms_product<T, true>(Lt, ld, c, c, Fc, kMlFld, kb,
+                      [&](int i, int r, double v) { Yl[r * kMlFld + i] = v; });
+  ms_product<T, false>(Lt + c, ld, R, c, Fc, kMlFld, kb, [&](int i, int r, double v) {
+    double* p = fvR + (int64_t)r * b.fvs + i;
+    const double nv = *p - v;
+    if (root) vst<true>(p, nv);
+    else *p = nv;
+  });
+  __syncthreads();
+  double* zb = b.z + (int64_t)c0 * b.n + S.first;
+  ms_product<T, false>(Lt, ld, c, c, Yl, kMlFld, kb, [&](int i, int r, double v) {
+    vst<true>(zb + (int64_t)r * b.n + i, v);
+  });
+  __syncthreads();
+}
+
+// Backward step of a larger supernode: x_c = z_c - Zp^T x_R with x_R staged
+// through LDS in chunks of kMlChunk rows; each wave keeps the sums of its
+// (at most two) 16-row blocks of x_c across the chunks, the sum running over
+// the rows in increasing order.
+template <typename T>
+__device__ void ms_backward_large(const T* __restrict__ Zp, const VecNode& S,
+                                  const SubtreeArgs& a, const VecBufs& b, int c0, int kb,
+                                  double* lds) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, kq = lane >> 4;
+  const int c = S.ncols, R = S.R, ld = c + R;
+  constexpr int XL = kMlChunk + 1;
+  const int* rw = a.rows + S.rows_off;
+  const double* wb = b.w + (int64_t)c0 * b.n;
+  d4v acc[2];
+  acc[0] = d4v{0, 0, 0, 0};
+  acc[1] = d4v{0, 0, 0, 0};
+  for (int r0 = 0; r0 < R; r0 += kMlChunk) {
+    const int nr = min(kMlChunk, R - r0);
+    for (int idx = tid; idx < kb * nr; idx += 256) {
+      const int r = idx / nr, t = idx - r * nr;
+      lds[r * XL + t] = vld<true>(wb + (int64_t)r * b.n + rw[r0 + t]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+      const int i0 = wave * 16 + 64 * bi;
+      if (i0 >= c) break;
+      const int i = i0 + li;
+      const bool iok = i < c;
+      for (int j0 = 0; j0 < nr; j0 += 16) {
+        T m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + 4 * u + kq;
+          const bool ok = iok && j < nr;
+          const T v = Zp[(int64_t)(ok ? r0 + j : 0) + (int64_t)(ok ? i : 0) * ld];
+          m[u] = ok ? v : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (j0 + 4 * u >= nr) break;
+          const int j = j0 + 4 * u + kq;
+          const double xv = (j < nr && li < kb) ? lds[li * XL + j] : 0.0;
+          acc[bi] = mfma64((double)m[u], xv, acc[bi]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double* zb = b.z + (int64_t)c0 * b.n + S.first;
+  double* xb = b.w + (int64_t)c0 * b.n + S.first;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    const int i0 = wave * 16 + 64 * bi;
+    if (i0 >= c) break;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = i0 + kq + 4 * v;
+      if (row < c && li < kb) {
+        const int64_t o = (int64_t)li * b.n + row;
+        vst<true>(xb + o, vld<true>(zb + o) - acc[bi][v]);
+      }
+    }
+  }
+  __syncthreads();
 }
 
 template <typename T>
@@ -1176,6 +1301,10 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
       const int64_t P = uni64(a.panel[j]);
       const int c = S.ncols, R = S.R, ld = c + R;
       const T* __restrict__ Lt = F + P;
+      if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
+        ms_forward_large<T>(Lt, S, a, b, c0, kb, j == rootn, lds);
+        continue;
+      }
       for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
         const int r = idx / ld, p = idx - r * ld;
         double v = 0.0;
@@ -1230,6 +1359,10 @@ __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int s
       const VecNode S = uniform_node(a.vn[j]);
       const int64_t P = uni64(a.panel[j]);
       const int c = S.ncols, R = S.R, ld = c + R;
+      if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
+        ms_backward_large<T>(F + P + c, S, a, b, c0, kb, lds);
+        continue;
+      }
       const int* rw = a.rows + S.rows_off;
       const double* wb = b.w + (int64_t)c0 * b.n;
       for (int idx = tid; idx < kb * R; idx += 256) {

@@ -12,8 +12,7 @@
 
 namespace rsc {
 
-void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, int max_cols,
-                    int max_rows, int max_front) {
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi) {
   const int ns = (int)P.nodes.size();
   S.of.assign(ns, -1);
   S.ptr.assign(1, 0);
@@ -26,8 +25,8 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, int max_cols,
   std::vector<int64_t> cost(ns, 0);
   for (int j = 0; j < ns; ++j) {
     const Supernode& s = P.nodes[j];
-    bool e = !s.compress && s.ncols <= max_cols && s.R() <= max_rows &&
-             s.ncols + s.R() <= max_front;
+    bool e = !s.compress && (multi ? multi_subtree_node(s.ncols, s.R())
+                                   : s.ncols <= kSubtreeMaxCols && s.R() <= kSubtreeMaxRows);
     int64_t c = (int64_t)s.ncols * (s.ncols + s.R());
     for (int ch : s.children) {
       if (ch >= j) throw std::logic_error("supernodes not in postorder");
@@ -138,7 +137,7 @@ void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems,
   for (int j = 0; j < ns; ++j)
     if (!P.nodes[j].compress) L.panel[j] = P.nodes[j].off_panel;
   build_subtrees(P, subtree_elems, L.sub);
-  build_subtrees(P, multi_subtree_elems, L.sub_multi, kMsMaxCols, kMsMaxFront, kMsMaxFront);
+  build_subtrees(P, multi_subtree_elems, L.sub_multi, true);
 }
 
 // ---- per-node stages ------------------------------------------------------
@@ -557,8 +556,8 @@ struct CostModel {
           nodes += 1;
         }
         if (multi)  // sixteen right-hand sides per pass, fronts in LDS
-          return t.type == 4 ? 6.0 + (4.5 * nodes + 0.01 * front + 0.3 * kb) * passes
-                             : 4.0 + (3.0 * nodes + 0.01 * front + 0.2 * kb) * passes;
+          return t.type == 4 ? 6.0 + (4.5 * nodes + 0.01 * front + 0.1 * kb) * passes
+                             : 4.0 + (3.0 * nodes + 0.01 * front + 0.1 * kb) * passes;
         const double sp = std::ceil(k / kSubtreeRhs);
         return t.type == 4 ? 10.9 + (1.35 * kb + 10.7 * nodes) * sp
                            : 3.8 + (0.8 * kb + 4.3 * nodes) * sp;

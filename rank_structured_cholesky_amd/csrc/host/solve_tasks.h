@@ -97,6 +97,14 @@ constexpr int kSubtreeRhs = 4;  // right-hand sides per pass over a subtree's pa
 // right-hand sides live in LDS, so these limits are per supernode.
 constexpr int kMsMaxCols = 32;
 constexpr int kMsMaxFront = 224;
+// Larger supernodes of such subtrees keep fv_c and y in LDS and fv_R in
+// device memory (any R): at most kMlMaxCols columns.
+constexpr int kMlMaxCols = 128;
+// A supernode of a multi-right-hand-side subtree task: uncompressed, and
+// either small (front in LDS) or at most kMlMaxCols columns.
+inline bool multi_subtree_node(int ncols, int R) {
+  return (ncols <= kMsMaxCols && ncols + R <= kMsMaxFront) || ncols <= kMlMaxCols;
+}
 
 // Subtrees of uncompressed supernodes handled by one task each.
 struct Subtrees {
@@ -109,9 +117,9 @@ struct Subtrees {
 // Maximal subtrees of uncompressed supernodes with at most max_cols columns
 // and max_rows off-diagonal rows (and ncols + R <= max_front) each, whose
 // panels hold at most max_elems elements in total.
-void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S,
-                    int max_cols = kSubtreeMaxCols, int max_rows = kSubtreeMaxRows,
-                    int max_front = kSubtreeMaxCols + kSubtreeMaxRows);
+// multi: the supernode rule of the multi-right-hand-side list
+// (multi_subtree_node) instead of kSubtreeMaxCols / kSubtreeMaxRows.
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi = false);
 
 struct SolvePhase {
   int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks,

@@ -216,8 +216,7 @@ void subtree(State& S, int sb, bool fwd, const Subtrees& T, bool multi) {
     if (P < 0) fail("subtree node without a panel");
     const int c = N.ncols, R = N.R, ld = c + R;
     if (c > kSubtreeMaxCols || R > kSubtreeMaxRows) fail("subtree node too large");
-    if (multi && (c > kMsMaxCols || c + R > kMsMaxFront))
-      fail("multi subtree node exceeds the LDS front");
+    if (multi && !multi_subtree_node(c, R)) fail("multi subtree node exceeds its LDS limits");
     for (int col = 0; col < S.k; ++col) {
       const int64_t fb = N.fv_off + (int64_t)col * S.L->fv_stride;
       const int64_t vb = (int64_t)col * S.n;

@@ -287,6 +287,7 @@ class Engine final : public FactorEngine {
     step_ctx_.push_back(ctx_);
   }
   void add_checks();
+  void write_factor_trace(const char* path, std::vector<hipEvent_t>& ev);
   std::vector<dev::RowTile> etiles_;
   std::vector<dev::AsmTask> tasks_;
   int64_t inv_elems_ = 0;
@@ -1253,6 +1254,49 @@ void Engine<T>::emit_solve_helpers(int j) {
   }
 }
 
+// Timing trace of one numeric factorization: one line per program step
+// (index, operation, workgroups launched, floating-point operations of the
+// step's jobs, milliseconds between the events around it, context).
+template <typename T>
+void Engine<T>::write_factor_trace(const char* path, std::vector<hipEvent_t>& ev) {
+  static const char* names[] = {"assemble", "gemm_nn", "gemm_tn", "gemm_nt", "gemm_tt",
+                                "potrf", "trsm", "copy", "shift", "eye", "check", "conv"};
+  FILE* fp = std::fopen(path, "w");
+  for (size_t si = 0; fp && si < prog_.size(); ++si) {
+    const Step& st = prog_[si];
+    float ms = 0.f;
+    RSC_HIP(hipEventElapsedTime(&ms, ev[si], ev[si + 1]));
+    double flops = 0;
+    int grid = st.count;
+    if (st.op >= OP_GEMM_NN && st.op <= OP_GEMM_TT) {
+      for (int t = 0; t < st.count; ++t) {
+        const dev::GemmTile& tl = gtiles_[st.first + t];
+        const dev::GemmJob& J = gjobs_[tl.job];
+        const double r = std::min(dev::TILE, J.m - tl.tm * dev::TILE);
+        const double c = std::min(dev::TILE, J.n - tl.tn * dev::TILE);
+        flops += 2.0 * r * c * (tl.k1 - tl.k0);
+      }
+    } else if (st.op == OP_POTRF) {
+      for (int t = 0; t < st.count; ++t) {
+        const double n = pjobs_[st.first + t].n;
+        flops += n * n * n / 3.0;
+      }
+    } else if (st.op == OP_TRSM) {
+      for (int t = 0; t < st.count; ++t) {
+        const dev::RowTile& tl = ttiles_[st.first + t];
+        const dev::TrsmJob& J = tjobs_[tl.job];
+        const double rows = std::min(dev::TILE, J.m - tl.row0);
+        flops += rows * J.nb * J.nb;
+      }
+    }
+    std::fprintf(fp, "%zu %s%s%s %d %.0f %.4f %s\n", si, names[st.op], st.f64 ? "64" : "",
+                 st.bf16 ? "bf" : "", grid, flops, ms, step_ctx_[si].c_str());
+  }
+  if (fp) std::fclose(fp);
+  for (auto& e : ev) RSC_HIP(hipEventDestroy(e));
+  ev.clear();
+}
+
 // Debug: after every program step, scan that step's outputs for non-finite
 // values; the first failing step index is reported with its description.
 template <typename T>
@@ -1695,7 +1739,16 @@ void Engine<T>::numeric(const double* Ax) {
   int status0[4] = {0, INT_MAX, 0, INT_MAX};
   RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
                          hipMemcpyHostToDevice, st_));
+  // optional timing trace: an event after every step (RSC_FACTOR_TRACE=path)
+  const char* ftp = std::getenv("RSC_FACTOR_TRACE");
+  std::vector<hipEvent_t> fev;
+  if (ftp && ftp[0]) {
+    fev.resize(prog_.size() + 1);
+    for (auto& e : fev) RSC_HIP(hipEventCreate(&e));
+    RSC_HIP(hipEventRecord(fev[0], st_));
+  }
   for (size_t si = 0; si < prog_.size(); ++si) {
+    if (!fev.empty() && si > 0) RSC_HIP(hipEventRecord(fev[si], st_));
     const Step& s = prog_[si];
     if ((int)si == rng_step_ && !rng_on_device_) upload_rng();
     switch (s.op) {
@@ -1770,9 +1823,11 @@ void Engine<T>::numeric(const double* Ax) {
     }
   }
   RSC_HIP(hipGetLastError());
+  if (!fev.empty()) RSC_HIP(hipEventRecord(fev[prog_.size()], st_));
   int status[4];
   RSC_HIP(hipMemcpyAsync(status, dstatus_.p, sizeof(status), hipMemcpyDeviceToHost, st_));
   RSC_HIP(hipStreamSynchronize(st_));
+  if (!fev.empty()) write_factor_trace(ftp, fev);
   if (rng_on_device_ && rng_host_) {
     RSC_HIP(hipHostFree(rng_host_));
     rng_host_ = nullptr;

@@ -122,6 +122,7 @@ class Engine final : public FactorEngine {
     if (rng_thread_.joinable()) rng_thread_.join();
     if (df_thread_.joinable()) df_thread_.join();
     if (rng_host_) (void)hipHostFree(rng_host_);
+    drop_graph();
   }
 
   void numeric(const double* Ax) override;
@@ -138,6 +139,7 @@ class Engine final : public FactorEngine {
   int pivot_replacements() const override { return pivot_replacements_; }
   void release_workspace() override {
     (void)hipStreamSynchronize(st_);
+    drop_graph();
     work_.reset();
     program_ready_ = false;
   }
@@ -239,6 +241,18 @@ class Engine final : public FactorEngine {
   std::vector<int> lev_off_;
   std::vector<int> lev_front_;  // largest front (ncols + R) per level
   bool program_ready_ = false;
+  // The program captured as a graph (kernel nodes only), replayed by later
+  // numeric factorizations of the same pattern; rebuilt with the program.
+  hipGraph_t fgraph_ = nullptr;
+  hipGraphExec_t fgexec_ = nullptr;
+  bool graph_failed_ = false;
+  void drop_graph() {
+    if (fgexec_) (void)hipGraphExecDestroy(fgexec_);
+    if (fgraph_) (void)hipGraphDestroy(fgraph_);
+    fgexec_ = nullptr;
+    fgraph_ = nullptr;
+  }
+  void run_program(std::vector<hipEvent_t>* ev);
   int pivot_replacements_ = 0;
   // program
   std::vector<Step> prog_;
@@ -1731,24 +1745,11 @@ void Engine<T>::build_program() {
   program_ready_ = true;
 }
 
+// One pass of the program: the launches of every step, in order.
 template <typename T>
-void Engine<T>::numeric(const double* Ax) {
-  if (!program_ready_) build_program();
-  std::vector<double> ax(Ax, Ax + A_.nnz());
-  dAx_.upload(ax, st_);
-  int status0[4] = {0, INT_MAX, 0, INT_MAX};
-  RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
-                         hipMemcpyHostToDevice, st_));
-  // optional timing trace: an event after every step (RSC_FACTOR_TRACE=path)
-  const char* ftp = std::getenv("RSC_FACTOR_TRACE");
-  std::vector<hipEvent_t> fev;
-  if (ftp && ftp[0]) {
-    fev.resize(prog_.size() + 1);
-    for (auto& e : fev) RSC_HIP(hipEventCreate(&e));
-    RSC_HIP(hipEventRecord(fev[0], st_));
-  }
+void Engine<T>::run_program(std::vector<hipEvent_t>* ev) {
   for (size_t si = 0; si < prog_.size(); ++si) {
-    if (!fev.empty() && si > 0) RSC_HIP(hipEventRecord(fev[si], st_));
+    if (ev && si > 0) RSC_HIP(hipEventRecord((*ev)[si], st_));
     const Step& s = prog_[si];
     if ((int)si == rng_step_ && !rng_on_device_) upload_rng();
     switch (s.op) {
@@ -1822,6 +1823,53 @@ void Engine<T>::numeric(const double* Ax) {
         break;
     }
   }
+}
+
+template <typename T>
+void Engine<T>::numeric(const double* Ax) {
+  if (!program_ready_) {
+    drop_graph();
+    build_program();
+  }
+  // matrix values into a buffer that keeps its address (the graph reads it)
+  const size_t abytes = std::max<size_t>((size_t)A_.nnz() * sizeof(double), 8);
+  if (!dAx_.p || dAx_.bytes != abytes) dAx_.alloc(abytes);
+  if (A_.nnz() > 0)
+    RSC_HIP(hipMemcpyAsync(dAx_.p, Ax, (size_t)A_.nnz() * sizeof(double),
+                           hipMemcpyHostToDevice, st_));
+  int status0[4] = {0, INT_MAX, 0, INT_MAX};
+  RSC_HIP(hipMemcpyAsync(dstatus_.p, status0, sizeof(status0),
+                         hipMemcpyHostToDevice, st_));
+  // optional timing trace: an event after every step (RSC_FACTOR_TRACE=path)
+  const char* ftp = std::getenv("RSC_FACTOR_TRACE");
+  std::vector<hipEvent_t> fev;
+  if (ftp && ftp[0]) {
+    fev.resize(prog_.size() + 1);
+    for (auto& e : fev) RSC_HIP(hipEventCreate(&e));
+    RSC_HIP(hipEventRecord(fev[0], st_));
+  }
+  // Later factorizations replay the program as a graph (RSC_FACTOR_GRAPH=0
+  // keeps every launch eager).  The first one runs eagerly: it copies the
+  // sketch entries in mid-program, while the host is still drawing them.
+  const char* ge = std::getenv("RSC_FACTOR_GRAPH");
+  const bool graph = !(ge && ge[0] == '0') && fev.empty() && !debug_check_ &&
+                     !graph_failed_ && (rng_on_device_ || plan_.rng_count <= 0) &&
+                     !prog_.empty();
+  if (graph && !fgexec_) {
+    hipError_t e = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      run_program(nullptr);
+      e = hipStreamEndCapture(st_, &fgraph_);
+      if (e == hipSuccess) e = hipGraphInstantiate(&fgexec_, fgraph_, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      drop_graph();
+      graph_failed_ = true;
+    }
+  }
+  if (graph && fgexec_) RSC_HIP(hipGraphLaunch(fgexec_, st_));
+  else run_program(fev.empty() ? nullptr : &fev);
   RSC_HIP(hipGetLastError());
   if (!fev.empty()) RSC_HIP(hipEventRecord(fev[prog_.size()], st_));
   int status[4];

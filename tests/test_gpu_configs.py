@@ -12,6 +12,7 @@ subspace iteration does); FP32 factor 1e-3 and BF16 sketches 1e-2 of the
 FP64 oracle; PCG iteration counts within 10% (and at least +-1) of the
 oracle's at the same ranks, the north-star parity bar."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -288,6 +289,45 @@ def test_refactor_new_coefficients_matches_oracle(prec):
     xo, ro = O2.pcg(b, 1e-8, 500)
     assert abs(res["iterations"] - ro["iterations"]) <= 1
     assert res["true_relres"] < 2e-8
+
+
+@pytest.mark.parametrize("prec", [R.RSC_FP64, R.RSC_FP32])
+def test_refactor_graph_replays_match_eager(prec):
+    """Refactorizations after the first replay the numeric program as one
+    captured graph.  Five refactorizations in a row, alternating two
+    coefficient sets, with solves (eager work) between the replays: every
+    replayed factor applies bitwise like a fresh, eagerly launched
+    factorization of the same values, and like the eager launches of the
+    same handle (RSC_FACTOR_GRAPH=0)."""
+    A, xyz = R.poisson3d(18, 1.5, 1234)
+    A2, _ = R.poisson3d(18, 1.5, 99)
+    sopts = dict(max_dense_block=63, max_diagonal_block=50, explicit_block_threshold=120,
+                 factor_precision=prec)
+    b = R.uniform_rhs(A._nrow, 5678)
+
+    def applied(M):
+        y = np.empty_like(b)
+        M.solve(y, b)
+        return y
+
+    want = {0: applied(_factor(A, xyz, **sopts)), 1: applied(_factor(A2, xyz, **sopts))}
+    M = _factor(A, xyz, **sopts)
+    assert np.array_equal(applied(M), want[0])
+    mats = (A, A2)
+    for rep in range(5):
+        which = (rep + 1) % 2
+        _lib.check(_lib.lib().rsc_factor_refactor(M.handle, _lib.dptr(mats[which]._x)))
+        assert np.array_equal(applied(M), want[which]), rep
+    old = os.environ.get("RSC_FACTOR_GRAPH")
+    os.environ["RSC_FACTOR_GRAPH"] = "0"
+    try:
+        _lib.check(_lib.lib().rsc_factor_refactor(M.handle, _lib.dptr(A2._x)))
+        assert np.array_equal(applied(M), want[1])
+    finally:
+        if old is None:
+            del os.environ["RSC_FACTOR_GRAPH"]
+        else:
+            os.environ["RSC_FACTOR_GRAPH"] = old
 
 
 # ---- indefinite-diagonal retry -----------------------------------------------

@@ -118,6 +118,7 @@ class Engine final : public FactorEngine {
     setup_static();
   }
   ~Engine() override {
+    if (prog_thread_.joinable()) prog_thread_.join();
     (void)hipStreamSynchronize(st_);
     if (rng_thread_.joinable()) rng_thread_.join();
     if (df_thread_.joinable()) df_thread_.join();
@@ -138,6 +139,8 @@ class Engine final : public FactorEngine {
   int64_t workspace_bytes() const override { return (int64_t)work_.bytes; }
   int pivot_replacements() const override { return pivot_replacements_; }
   void release_workspace() override {
+    if (prog_thread_.joinable()) prog_thread_.join();
+    prog_err_ = nullptr;
     (void)hipStreamSynchronize(st_);
     drop_graph();
     work_.reset();
@@ -253,6 +256,18 @@ class Engine final : public FactorEngine {
     fgraph_ = nullptr;
   }
   void run_program(std::vector<hipEvent_t>* ev);
+  // the first program build runs on this thread during the engine setup
+  std::thread prog_thread_;
+  std::exception_ptr prog_err_;
+  void join_program_build() {
+    if (prog_thread_.joinable()) prog_thread_.join();
+    if (prog_err_) {
+      std::exception_ptr e = prog_err_;
+      prog_err_ = nullptr;
+      program_ready_ = false;
+      std::rethrow_exception(e);
+    }
+  }
   int pivot_replacements_ = 0;
   // program
   std::vector<Step> prog_;
@@ -1427,6 +1442,22 @@ void Engine<T>::setup_static() {
     std::printf("setup %s: %.3f s\n", what, std::chrono::duration<double>(now - tp).count());
     tp = now;
   };
+  // the factor arena and the scratch layout first: the numeric program is
+  // then built on a host thread while the rest of the setup runs here
+  {
+    const char* dbg = std::getenv("RSC_DEBUG_CHECK");
+    debug_check_ = dbg && dbg[0] == '1';
+  }
+  factor_.alloc(std::max<int64_t>(P.factor_elems, 1) * sizeof(T));
+  layout_scratch();
+  phase("factor allocation and scratch layout");
+  prog_thread_ = std::thread([this]() {
+    try {
+      build_program();
+    } catch (...) {
+      prog_err_ = std::current_exception();
+    }
+  });
   // permuted lower pattern and the destination of every entry in its panel
   LowerCsc L = permuted_lower(A_, P.pinv);
   std::vector<int64_t> a_dst(L.i.size());
@@ -1579,15 +1610,9 @@ void Engine<T>::setup_static() {
   if (levnodes.empty()) levnodes.push_back(0);
   dlevnodes_.upload(levnodes, st_);
   std::vector<int> status = {0, INT_MAX, 0, INT_MAX};
-  const char* dbg = std::getenv("RSC_DEBUG_CHECK");
-  debug_check_ = dbg && dbg[0] == '1';
   dstatus_.upload(status, st_);
-  phase("uploads");
-  factor_.alloc(std::max<int64_t>(P.factor_elems, 1) * sizeof(T));
-  phase("factor allocation");
-  layout_scratch();
   RSC_HIP(hipStreamSynchronize(st_));
-  phase("scratch layout");
+  phase("uploads");
 }
 
 template <typename T>
@@ -1649,6 +1674,8 @@ void Engine<T>::build_program() {
     }
   }
   dfronts_.upload(fr, st_);
+  double t_comp = 0;
+  auto tl0 = std::chrono::steady_clock::now();
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
     const std::vector<int>& nodes = P.levels[lv];
     ctx_ = "level " + std::to_string(lv) + " uncompressed";
@@ -1721,9 +1748,16 @@ void Engine<T>::build_program() {
     std::vector<int> comp;
     for (int j : nodes)
       if (P.nodes[j].compress) comp.push_back(j);
+    auto tc = std::chrono::steady_clock::now();
     emit_level_compressed(comp);
+    t_comp += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
 
   }
+  if (P.opt.verbose)
+    std::printf("program levels: %.3f s (compressed supernodes %.3f s), %zu steps, %zu GEMM "
+                "tiles\n",
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - tl0).count(),
+                t_comp, prog_.size(), gtiles_.size());
   if (debug_check_) add_checks();
   dgjobs_.upload(gjobs_, st_);
   dgtiles_.upload(gtiles_, st_);
@@ -1827,9 +1861,19 @@ void Engine<T>::run_program(std::vector<hipEvent_t>* ev) {
 
 template <typename T>
 void Engine<T>::numeric(const double* Ax) {
+  auto tv = std::chrono::steady_clock::now();
+  auto vphase = [&](const char* what) {
+    if (!plan_.opt.verbose) return;
+    auto now = std::chrono::steady_clock::now();
+    std::printf("numeric %s: %.3f s\n", what, std::chrono::duration<double>(now - tv).count());
+    tv = now;
+  };
+  join_program_build();
+  vphase("program build (joined)");
   if (!program_ready_) {
     drop_graph();
     build_program();
+    vphase("program build");
   }
   // matrix values into a buffer that keeps its address (the graph reads it)
   const size_t abytes = std::max<size_t>((size_t)A_.nnz() * sizeof(double), 8);
@@ -1872,15 +1916,18 @@ void Engine<T>::numeric(const double* Ax) {
   else run_program(fev.empty() ? nullptr : &fev);
   RSC_HIP(hipGetLastError());
   if (!fev.empty()) RSC_HIP(hipEventRecord(fev[prog_.size()], st_));
+  vphase("launches");
   int status[4];
   RSC_HIP(hipMemcpyAsync(status, dstatus_.p, sizeof(status), hipMemcpyDeviceToHost, st_));
   RSC_HIP(hipStreamSynchronize(st_));
+  vphase("device program");
   if (!fev.empty()) write_factor_trace(ftp, fev);
   if (rng_on_device_ && rng_host_) {
     RSC_HIP(hipHostFree(rng_host_));
     rng_host_ = nullptr;
   }
   finish_solve_setup();
+  vphase("solve setup");
   pivot_replacements_ = status[2];
   if (status[3] != INT_MAX)
     throw DeviceError("non-finite factor values first after program step " +

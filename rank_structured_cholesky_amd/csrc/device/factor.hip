@@ -1462,19 +1462,24 @@ void Engine<T>::setup_static() {
   LowerCsc L = permuted_lower(A_, P.pinv);
   std::vector<int64_t> a_dst(L.i.size());
   {
-    std::vector<int> rpos(P.n, -1);  // front position of a row, one supernode at a time
-    for (int j = 0; j < ns; ++j) {
-      const Supernode& s = P.nodes[j];
-      const int c = s.ncols, ld = c + s.R(), last = s.first + c - 1;
-      for (size_t q = 0; q < s.rows.size(); ++q) rpos[s.rows[q]] = c + (int)q;
-      for (int k = s.first; k <= last; ++k)
-        for (int e = L.p[k]; e < L.p[k + 1]; ++e) {
-          const int r = L.i[e];
-          const int pos = r <= last ? r - s.first : rpos[r];
-          a_dst[e] = (int64_t)pos + (int64_t)(k - s.first) * ld;
-        }
-      for (int r : s.rows) rpos[r] = -1;
-    }
+    // front position of a row, one supernode at a time per thread
+    std::vector<std::vector<int>> rposes(stream_threads());
+    parallel_for(ns, 256, [&](int64_t b, int64_t e, int t) {
+      std::vector<int>& rpos = rposes[t];
+      if (rpos.empty()) rpos.assign(P.n, -1);
+      for (int64_t j = b; j < e; ++j) {
+        const Supernode& s = P.nodes[j];
+        const int c = s.ncols, ld = c + s.R(), last = s.first + c - 1;
+        for (size_t q = 0; q < s.rows.size(); ++q) rpos[s.rows[q]] = c + (int)q;
+        for (int k = s.first; k <= last; ++k)
+          for (int x = L.p[k]; x < L.p[k + 1]; ++x) {
+            const int r = L.i[x];
+            const int pos = r <= last ? r - s.first : rpos[r];
+            a_dst[x] = (int64_t)pos + (int64_t)(k - s.first) * ld;
+          }
+        for (int r : s.rows) rpos[r] = -1;
+      }
+    });
   }
   dlowp_.upload(L.p, st_);
   da_dst_.upload(a_dst, st_);

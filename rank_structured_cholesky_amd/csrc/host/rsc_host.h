@@ -3,8 +3,12 @@
 // symbolic plan that the device factor and solves execute.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/rsc.h"
@@ -188,5 +192,44 @@ struct LowerCsc {
 LowerCsc permuted_lower(const Csc& A, const std::vector<int>& pinv);
 
 std::string& last_error();
+
+// Host threads for the parallel setup loops (at most 16).
+int stream_threads();
+
+// f(begin, end, thread) over [0, n) in chunks of `grain`, on up to
+// stream_threads() threads (the calling thread included).  Chunks are
+// handed out dynamically; f must write only data owned by its range.  An
+// exception thrown by f stops the loop and is rethrown here.
+template <typename F>
+void parallel_for(int64_t n, int64_t grain, F f) {
+  if (n <= 0) return;
+  grain = std::max<int64_t>(grain, 1);
+  const int64_t nchunks = (n + grain - 1) / grain;
+  const int nt = (int)std::min<int64_t>(stream_threads(), nchunks);
+  if (nt <= 1) {
+    f((int64_t)0, n, 0);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  std::vector<std::exception_ptr> err(nt);
+  auto work = [&](int t) {
+    try {
+      for (;;) {
+        const int64_t c = next.fetch_add(1);
+        if (c >= nchunks) return;
+        f(c * grain, std::min(n, (c + 1) * grain), t);
+      }
+    } catch (...) {
+      err[t] = std::current_exception();
+      next.store(nchunks);  // the other threads stop at their next chunk
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
 
 }  // namespace rsc

@@ -14,6 +14,7 @@
 // diagonal-block leaves and low-rank (V, U) pairs.
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -92,15 +93,16 @@ inline double box_muller(uint32_t a, uint32_t b) {
   return r * std::cos(phi);
 }
 
-// The raw MT19937 stream is sequential; its transform is not.  One thread
-// generates the raw pairs chunk by chunk and publishes its progress, the
-// others transform published chunks; every entry is computed by the same
-// scalar code as a single-threaded loop, so the result does not depend on
-// the thread count.
+// The raw MT19937 stream is sequential; its transform is not.  The calling
+// thread generates the raw pairs chunk by chunk into a small ring of
+// buffers and publishes its progress; the other threads transform published
+// chunks in order and hand their ring slots back.  Every entry is computed
+// by the same scalar code as a single-threaded loop, so the result does not
+// depend on the thread count.
 template <typename T>
 void gaussian_fill(uint32_t seed, int64_t count, T* out, int nthreads) {
   if (count <= 0) return;
-  constexpr int64_t kChunk = 1 << 18;
+  constexpr int64_t kChunk = 1 << 16;  // values per chunk
   const int64_t nchunks = (count + kChunk - 1) / kChunk;
   nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, nchunks));
   if (nthreads == 1) {
@@ -112,19 +114,23 @@ void gaussian_fill(uint32_t seed, int64_t count, T* out, int nthreads) {
     }
     return;
   }
-  std::vector<uint32_t> raw(2 * (size_t)count);
+  const int slots = 4 * nthreads;
+  std::unique_ptr<uint32_t[]> ring(new uint32_t[(size_t)slots * 2 * kChunk]);
+  // chunk c uses slot c % slots once chunk c - slots has been transformed
+  std::unique_ptr<std::atomic<int64_t>[]> free_for(new std::atomic<int64_t>[slots]);
+  for (int q = 0; q < slots; ++q) free_for[q].store(q);
   std::atomic<int64_t> produced{0};  // chunks whose raw draws are written
   std::atomic<int64_t> next{0};      // next chunk to transform
-  auto transform = [&](int64_t c) {
-    const int64_t b = c * kChunk, e = std::min(count, b + kChunk);
-    for (int64_t k = b; k < e; ++k) out[k] = (T)box_muller(raw[2 * k], raw[2 * k + 1]);
-  };
   auto worker = [&]() {
     for (;;) {
-      int64_t c = next.fetch_add(1);
+      const int64_t c = next.fetch_add(1);
       if (c >= nchunks) return;
       while (produced.load(std::memory_order_acquire) <= c) std::this_thread::yield();
-      transform(c);
+      const uint32_t* raw = ring.get() + (size_t)(c % slots) * 2 * kChunk;
+      const int64_t b = c * kChunk, e = std::min(count, b + kChunk);
+      for (int64_t k = b; k < e; ++k)
+        out[k] = (T)box_muller(raw[2 * (k - b)], raw[2 * (k - b) + 1]);
+      free_for[c % slots].store(c + slots, std::memory_order_release);
     }
   };
   std::vector<std::thread> pool;
@@ -132,8 +138,10 @@ void gaussian_fill(uint32_t seed, int64_t count, T* out, int nthreads) {
   {
     Mt19937 g(seed);
     for (int64_t c = 0; c < nchunks; ++c) {
+      const int q = (int)(c % slots);
+      while (free_for[q].load(std::memory_order_acquire) != c) std::this_thread::yield();
       const int64_t b = c * kChunk, e = std::min(count, b + kChunk);
-      g.fill(raw.data() + 2 * b, 2 * (e - b));
+      g.fill(ring.get() + (size_t)q * 2 * kChunk, 2 * (e - b));
       produced.store(c + 1, std::memory_order_release);
     }
   }
@@ -161,36 +169,42 @@ void gaussian_stream_threads(uint32_t seed, int64_t count, double* out, int nthr
 }
 
 LowerCsc permuted_lower(const Csc& A, const std::vector<int>& pinv) {
-  int n = A.n;
+  const int n = A.n;
   std::vector<int> perm(n);
   for (int o = 0; o < n; ++o) perm[pinv[o]] = o;
   LowerCsc L;
   L.p.assign(n + 1, 0);
-  for (int k = 0; k < n; ++k) {
-    int o = perm[k];
-    int cnt = 0;
-    for (int p = A.p[o]; p < A.p[o + 1]; ++p)
-      if (pinv[A.i[p]] >= k) ++cnt;
-    L.p[k + 1] = L.p[k] + cnt;
-  }
+  // columns are independent: count, prefix sum, then fill and sort each
+  parallel_for(n, 4096, [&](int64_t b, int64_t e, int) {
+    for (int64_t k = b; k < e; ++k) {
+      const int o = perm[k];
+      int cnt = 0;
+      for (int p = A.p[o]; p < A.p[o + 1]; ++p)
+        if (pinv[A.i[p]] >= k) ++cnt;
+      L.p[k + 1] = cnt;
+    }
+  });
+  for (int k = 0; k < n; ++k) L.p[k + 1] += L.p[k];
   L.i.resize(L.p[n]);
   L.src.resize(L.p[n]);
-  std::vector<std::pair<int, int64_t>> tmp;
-  for (int k = 0; k < n; ++k) {
-    int o = perm[k];
-    tmp.clear();
-    for (int p = A.p[o]; p < A.p[o + 1]; ++p) {
-      int r = pinv[A.i[p]];
-      if (r >= k) tmp.push_back({r, (int64_t)p});
+  parallel_for(n, 4096, [&](int64_t b, int64_t e, int) {
+    std::vector<std::pair<int, int64_t>> tmp;
+    for (int64_t k = b; k < e; ++k) {
+      const int o = perm[k];
+      tmp.clear();
+      for (int p = A.p[o]; p < A.p[o + 1]; ++p) {
+        const int r = pinv[A.i[p]];
+        if (r >= k) tmp.push_back({r, (int64_t)p});
+      }
+      std::sort(tmp.begin(), tmp.end());
+      int q = L.p[k];
+      for (auto& x : tmp) {
+        L.i[q] = x.first;
+        L.src[q] = x.second;
+        ++q;
+      }
     }
-    std::sort(tmp.begin(), tmp.end());
-    int q = L.p[k];
-    for (auto& e : tmp) {
-      L.i[q] = e.first;
-      L.src[q] = e.second;
-      ++q;
-    }
-  }
+  });
   return L;
 }
 
@@ -525,32 +539,62 @@ void build_plan(const Csc& A, const double* xyz, const rsc_options& opt,
     sn.compress = plan.specs[s].compress;
     for (int k = sn.first; k <= plan.specs[s].last; ++k) plan.col2node[k] = s;
   }
-  std::vector<int> mark(n, -1);
+  // The supernode tree from the elimination tree: the parent of s is the
+  // supernode holding the etree parent of s's last column (the smallest row
+  // of s's structure below it).  Row sets then go level by level (children
+  // first), the supernodes of one level in parallel; each is the union of
+  // its columns' entries of A and its children's rows below its last column.
+  std::vector<int> sparent(ns, -1), height(ns, 0);
   for (int s = 0; s < ns; ++s) {
-    Supernode& sn = plan.nodes[s];
-    const int last = sn.first + sn.ncols - 1;
-    std::vector<int>& rows = sn.rows;
-    for (int k = sn.first; k <= last; ++k) {
-      int o = plan.perm[k];
-      for (int p = A.p[o]; p < A.p[o + 1]; ++p) {
-        int r = plan.pinv[A.i[p]];
-        if (r > last && mark[r] != s) {
-          mark[r] = s;
-          rows.push_back(r);
+    const int lastcol = plan.nodes[s].first + plan.nodes[s].ncols - 1;
+    const int pc = parent[lastcol];
+    if (pc >= 0) {
+      const int ps = plan.col2node[pc];
+      if (ps <= s) throw std::logic_error("supernode tree is not in postorder");
+      sparent[s] = ps;
+      plan.nodes[ps].children.push_back(s);
+    }
+  }
+  int hmax = 0;
+  for (int s = 0; s < ns; ++s)
+    if (sparent[s] >= 0) {
+      height[sparent[s]] = std::max(height[sparent[s]], height[s] + 1);
+      hmax = std::max(hmax, height[sparent[s]]);
+    }
+  std::vector<std::vector<int>> by_height(hmax + 1);
+  for (int s = 0; s < ns; ++s) by_height[height[s]].push_back(s);
+  std::vector<std::vector<int>> marks(stream_threads());
+  for (const std::vector<int>& lv : by_height) {
+    parallel_for((int64_t)lv.size(), 64, [&](int64_t b, int64_t e, int t) {
+      std::vector<int>& mark = marks[t];
+      if (mark.empty()) mark.assign(n, -1);
+      for (int64_t x = b; x < e; ++x) {
+        const int s = lv[x];
+        Supernode& sn = plan.nodes[s];
+        const int last = sn.first + sn.ncols - 1;
+        std::vector<int>& rows = sn.rows;
+        for (int k = sn.first; k <= last; ++k) {
+          const int o = plan.perm[k];
+          for (int p = A.p[o]; p < A.p[o + 1]; ++p) {
+            const int r = plan.pinv[A.i[p]];
+            if (r > last && mark[r] != s) {
+              mark[r] = s;
+              rows.push_back(r);
+            }
+          }
         }
+        for (int ch : sn.children)
+          for (int r : plan.nodes[ch].rows)
+            if (r > last && mark[r] != s) {
+              mark[r] = s;
+              rows.push_back(r);
+            }
+        std::sort(rows.begin(), rows.end());
+        sn.parent = sparent[s];
+        if ((rows.empty() ? -1 : plan.col2node[rows[0]]) != sparent[s])
+          throw std::logic_error("row sets disagree with the elimination tree");
       }
-    }
-    for (int ch : sn.children)
-      for (int r : plan.nodes[ch].rows)
-        if (r > last && mark[r] != s) {
-          mark[r] = s;
-          rows.push_back(r);
-        }
-    std::sort(rows.begin(), rows.end());
-    if (!rows.empty()) {
-      sn.parent = plan.col2node[rows[0]];
-      plan.nodes[sn.parent].children.push_back(s);
-    }
+    });
   }
   phase("row sets");
   // levels (height) and relative maps into the parent's front

@@ -64,10 +64,11 @@ def test_box_muller_matches_formula_and_device_stream():
 
 
 def test_threaded_stream_matches_oracle_across_chunks():
-    # the library draws raw pairs on one thread and transforms chunks of
-    # 2^18 entries on others; several chunks and a ragged tail must give the
+    # the library draws raw pairs on one thread into a ring of 4 x threads
+    # chunks of 2^16 entries and transforms them on the others; a stream long
+    # enough to wrap the ring at 16 threads, with a ragged tail, must give the
     # oracle's single-threaded stream bit for bit
-    n = 3 * (1 << 18) + 12345
+    n = 5 * (1 << 20) + 12345
     dev = np.empty(n)
     _lib.check(_lib.lib().rsc_gaussian_stream(152, n, _lib.dptr(dev)))
     assert np.array_equal(dev, gaussian(152, n))

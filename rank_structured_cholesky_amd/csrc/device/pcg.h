@@ -8,7 +8,6 @@ namespace rsc {
 struct DevCsr {
   int n = 0;
   int64_t nnz = 0;
-  int lanes = 8;
   DBuf p, i, x;
   void upload(const Csc& A, hipStream_t s);
   void update_values(const double* vals, hipStream_t s);

@@ -348,7 +348,6 @@ void DevCsr::upload(const Csc& A, hipStream_t s) {
   p.upload(A.p, s);
   i.upload(A.i, s);
   x.upload(A.x, s);
-  lanes = (nnz > 16 * (int64_t)n) ? 32 : 8;
 }
 
 void DevCsr::update_values(const double* vals, hipStream_t s) {
@@ -358,7 +357,7 @@ void DevCsr::update_values(const double* vals, hipStream_t s) {
 void DevCsr::spmv(const double* X, int ldx, double* Y, int ldy, int k,
                   const int* active, hipStream_t s) const {
   dev::launch_spmv(n, p.as<int>(), i.as<int>(), x.as<double>(), X, ldx, Y, ldy, k,
-                   active, s, lanes);
+                   active, s);
 }
 
 PcgWork::~PcgWork() {

@@ -98,7 +98,7 @@ int dataflow_blocks_per_cu(bool multi);
 // CSR (= CSC of the symmetric matrix) y = A x for k columns.
 void launch_spmv(int n, const int* p, const int* idx, const double* val,
                  const double* x, int ldx, double* y, int ldy, int k,
-                 const int* active, hipStream_t s, int lanes_per_row);
+                 const int* active, hipStream_t s);
 
 }  // namespace dev
 }  // namespace rsc

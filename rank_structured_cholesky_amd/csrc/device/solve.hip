@@ -1592,40 +1592,49 @@ template int dataflow_blocks_per_cu<double>(bool);
 template int dataflow_blocks_per_cu<float>(bool);
 
 // ---- SpMV ------------------------------------------------------------------
-// LPR lanes cooperate on one row; the row's partial sums are combined with
-// wavefront shuffles.
-template <int LPR>
+// One thread per row for up to 16 columns per pass: the row's entries and
+// column indices are read once for all the pass's columns, and each
+// column's sum runs over the row's entries in storage order, so a column's
+// result does not depend on how many columns share the launch.  Inactive
+// columns (active[col] == 0) are neither computed nor written.
+constexpr int kSpmvCols = 16;
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ p,
                                               const int* __restrict__ idx,
                                               const double* __restrict__ val,
                                               const double* __restrict__ x, int ldx,
-                                              double* __restrict__ y, int ldy,
+                                              double* __restrict__ y, int ldy, int k,
                                               const int* __restrict__ active) {
-  const int col = blockIdx.y;
-  if (active && !active[col]) return;
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int row = (int)(t / LPR), sub = (int)(t % LPR);
-  const double* xc = x + (int64_t)col * ldx;
-  double acc = 0;
-  if (row < n)
-    for (int e = p[row] + sub; e < p[row + 1]; e += LPR) acc += val[e] * xc[idx[e]];
-  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
-  if (row < n && sub == 0) y[row + (int64_t)col * ldy] = acc;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int e0 = p[row], e1 = p[row + 1];
+  for (int c0 = 0; c0 < k; c0 += kSpmvCols) {
+    const int kc = min(kSpmvCols, k - c0);
+    bool on[kSpmvCols];
+    double acc[kSpmvCols];
+#pragma unroll
+    for (int c = 0; c < kSpmvCols; ++c) {
+      on[c] = c < kc && (!active || active[c0 + c]);
+      acc[c] = 0;
+    }
+    const double* xc = x + (int64_t)c0 * ldx;
+    for (int e = e0; e < e1; ++e) {
+      const double v = val[e];
+      const int64_t j = idx[e];
+#pragma unroll
+      for (int c = 0; c < kSpmvCols; ++c)
+        if (on[c]) acc[c] += v * xc[j + (int64_t)c * ldx];
+    }
+#pragma unroll
+    for (int c = 0; c < kSpmvCols; ++c)
+      if (on[c]) y[row + (int64_t)(c0 + c) * ldy] = acc[c];
+  }
 }
 
 void launch_spmv(int n, const int* p, const int* idx, const double* val,
                  const double* x, int ldx, double* y, int ldy, int k,
-                 const int* active, hipStream_t s, int lpr) {
-  dim3 grid;
-  grid.y = k;
-  grid.z = 1;
-  if (lpr >= 32) {
-    grid.x = (unsigned)(((int64_t)n * 32 + 255) / 256);
-    k_spmv<32><<<grid, 256, 0, s>>>(n, p, idx, val, x, ldx, y, ldy, active);
-  } else {
-    grid.x = (unsigned)(((int64_t)n * 8 + 255) / 256);
-    k_spmv<8><<<grid, 256, 0, s>>>(n, p, idx, val, x, ldx, y, ldy, active);
-  }
+                 const int* active, hipStream_t s) {
+  if (n <= 0 || k <= 0) return;
+  k_spmv<<<(n + 255) / 256, 256, 0, s>>>(n, p, idx, val, x, ldx, y, ldy, k, active);
 }
 
 }  // namespace dev

@@ -1133,9 +1133,13 @@ static_assert(kMlMaxCols <= 128, "at most two 16-row blocks per wave");
 // blocks w, w + 4, ...; columns go in chunks of 16 whose four loads per lane
 // are issued together, the sum over j in groups of four in increasing order
 // for every right-hand side (so a column's bits do not depend on kb).
-template <typename T, bool TRANS, typename Out>
+//
+// The epilogue reads the four old values a lane needs (get(row, r), e.g.
+// from device memory) before writing any result (put(row, r, old, v)), so
+// a read-modify-write epilogue costs one round trip per row block.
+template <typename T, bool TRANS, typename Get, typename Put>
 __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
-                           const double* xs, int xld, int kb, Out out) {
+                           const double* xs, int xld, int kb, Get get, Put put) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, kq = lane >> 4;
   for (int i0 = wave * 16; i0 < rows; i0 += 64) {
@@ -1160,10 +1164,47 @@ __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
         acc = mfma64((double)m[u], xv, acc);
       }
     }
+    double old[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = i0 + kq + 4 * v;
-      if (row < rows && li < kb) out(row, li, acc[v]);
+      old[v] = (row < rows && li < kb) ? get(row, li) : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = i0 + kq + 4 * v;
+      if (row < rows && li < kb) put(row, li, old[v], acc[v]);
+    }
+  }
+}
+
+// The epilogue of a product whose results are written without reading.
+__device__ inline double ms_none(int, int) { return 0.0; }
+
+// Element loop of the subtree bodies over items [0, total) on the
+// workgroup's 256 threads, NB items per thread per round: every index load
+// of a round (index(i)), then every value load (load(i, ix)), then every
+// use (use(i, ix, v)), so a round costs one memory round trip per stage
+// instead of one per item.
+template <int NB, typename Index, typename Load, typename Use>
+__device__ inline void ms_batched(int total, Index index, Load load, Use use) {
+  for (int base = threadIdx.x; base < total; base += 256 * NB) {
+    int ix[NB];
+    double v[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int i = base + 256 * u;
+      ix[u] = i < total ? index(i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int i = base + 256 * u;
+      v[u] = i < total ? load(i, ix[u]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int i = base + 256 * u;
+      if (i < total) use(i, ix[u], v[u]);
     }
   }
 }
@@ -1181,8 +1222,12 @@ __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
   double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
   for (int idx = tid; idx < kMfmaRhs * c; idx += 256) {
     const int r = idx / c, p = idx - r * c;
-    Fc[r * kMlFld + p] = r < kb ? a.B[a.perm[S.first + p] + (int64_t)(c0 + r) * a.ldb] : 0.0;
+    if (r >= kb) Fc[r * kMlFld + p] = 0.0;
   }
+  ms_batched<4>(
+      kb * c, [&](int i) { return a.perm[S.first + i % c]; },
+      [&](int i, int o) { return a.B[o + (int64_t)(c0 + i / c) * a.ldb]; },
+      [&](int i, int, double v) { Fc[(i / c) * kMlFld + i % c] = v; });
   for (int idx = tid; idx < kb * R; idx += 256) {
     const int r = idx / R, t = idx - r * R;
     fvR[(int64_t)r * b.fvs + t] = 0.0;
@@ -1192,28 +1237,50 @@ __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
     const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
     const int* rm = a.relmap + C.relmap_off;
     const double* fc = b.fv + C.fv_off + C.ncols + (int64_t)c0 * b.fvs;
-    for (int idx = tid; idx < kb * C.R; idx += 256) {
-      const int r = idx / C.R, t = idx - r * C.R;
-      const int p = rm[t];
-      const double v = fc[(int64_t)r * b.fvs + t];
-      if (p < c) Fc[r * kMlFld + p] += v;
-      else fvR[(int64_t)r * b.fvs + p - c] += v;
+    const int CR = C.R;
+    // a child's entries land on distinct positions: every load of a round
+    // (its values, and the old fv_R entries they are added to) goes first
+    for (int base = tid; base < kb * CR; base += 256 * 4) {
+      int pp[4];
+      double v[4], o[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + 256 * u;
+        pp[u] = i < kb * CR ? rm[i % CR] : 0;
+        v[u] = i < kb * CR ? fc[(int64_t)(i / CR) * b.fvs + i % CR] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + 256 * u;
+        o[u] = (i < kb * CR && pp[u] >= c) ? fvR[(int64_t)(i / CR) * b.fvs + pp[u] - c] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + 256 * u;
+        if (i >= kb * CR) continue;
+        const int r = i / CR;
+        if (pp[u] < c) Fc[r * kMlFld + pp[u]] += v[u];
+        else fvR[(int64_t)r * b.fvs + pp[u] - c] = o[u] + v[u];
+      }
     }
     __syncthreads();
   }
-  ms_product<T, true>(Lt, ld, c, c, Fc, kMlFld, kb,
-                      [&](int i, int r, double v) { Yl[r * kMlFld + i] = v; });
-  ms_product<T, false>(Lt + c, ld, R, c, Fc, kMlFld, kb, [&](int i, int r, double v) {
-    double* p = fvR + (int64_t)r * b.fvs + i;
-    const double nv = *p - v;
-    if (root) vst<true>(p, nv);
-    else *p = nv;
-  });
+  ms_product<T, true>(Lt, ld, c, c, Fc, kMlFld, kb, ms_none,
+                      [&](int i, int r, double, double v) { Yl[r * kMlFld + i] = v; });
+  ms_product<T, false>(
+      Lt + c, ld, R, c, Fc, kMlFld, kb,
+      [&](int i, int r) { return fvR[(int64_t)r * b.fvs + i]; },
+      [&](int i, int r, double old, double v) {
+        double* p = fvR + (int64_t)r * b.fvs + i;
+        if (root) vst<true>(p, old - v);
+        else *p = old - v;
+      });
   __syncthreads();
   double* zb = b.z + (int64_t)c0 * b.n + S.first;
-  ms_product<T, false>(Lt, ld, c, c, Yl, kMlFld, kb, [&](int i, int r, double v) {
-    vst<true>(zb + (int64_t)r * b.n + i, v);
-  });
+  ms_product<T, false>(Lt, ld, c, c, Yl, kMlFld, kb, ms_none,
+                       [&](int i, int r, double, double v) {
+                         vst<true>(zb + (int64_t)r * b.n + i, v);
+                       });
   __syncthreads();
 }
 
@@ -1236,10 +1303,10 @@ __device__ void ms_backward_large(const T* __restrict__ Zp, const VecNode& S,
   acc[1] = d4v{0, 0, 0, 0};
   for (int r0 = 0; r0 < R; r0 += kMlChunk) {
     const int nr = min(kMlChunk, R - r0);
-    for (int idx = tid; idx < kb * nr; idx += 256) {
-      const int r = idx / nr, t = idx - r * nr;
-      lds[r * XL + t] = vld<true>(wb + (int64_t)r * b.n + rw[r0 + t]);
-    }
+    ms_batched<8>(
+        kb * nr, [&](int i) { return rw[r0 + i % nr]; },
+        [&](int i, int row) { return vld<true>(wb + (int64_t)(i / nr) * b.n + row); },
+        [&](int i, int, double v) { lds[(i / nr) * XL + i % nr] = v; });
     __syncthreads();
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
@@ -1273,13 +1340,16 @@ __device__ void ms_backward_large(const T* __restrict__ Zp, const VecNode& S,
   for (int bi = 0; bi < 2; ++bi) {
     const int i0 = wave * 16 + 64 * bi;
     if (i0 >= c) break;
+    double zo[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = i0 + kq + 4 * v;
-      if (row < c && li < kb) {
-        const int64_t o = (int64_t)li * b.n + row;
-        vst<true>(xb + o, vld<true>(zb + o) - acc[bi][v]);
-      }
+      zo[v] = (row < c && li < kb) ? vld<true>(zb + (int64_t)li * b.n + row) : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = i0 + kq + 4 * v;
+      if (row < c && li < kb) vst<true>(xb + (int64_t)li * b.n + row, zo[v] - acc[bi][v]);
     }
   }
   __syncthreads();
@@ -1307,32 +1377,36 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
       }
       for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
         const int r = idx / ld, p = idx - r * ld;
-        double v = 0.0;
-        if (r < kb && p < c) v = a.B[a.perm[S.first + p] + (int64_t)(c0 + r) * a.ldb];
-        Fr[r * kMsFld + p] = v;
+        if (r >= kb || p >= c) Fr[r * kMsFld + p] = 0.0;
       }
+      ms_batched<2>(
+          kb * c, [&](int i) { return a.perm[S.first + i % c]; },
+          [&](int i, int o) { return a.B[o + (int64_t)(c0 + i / c) * a.ldb]; },
+          [&](int i, int, double v) { Fr[(i / c) * kMsFld + i % c] = v; });
       __syncthreads();
       for (int q = 0; q < S.nchild; ++q) {
         const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
         const int* rm = a.relmap + C.relmap_off;
         const double* fc = b.fv + C.fv_off + C.ncols;
-        for (int idx = tid; idx < kb * C.R; idx += 256) {
-          const int r = idx / C.R, t = idx - r * C.R;
-          Fr[r * kMsFld + rm[t]] += fc[(int64_t)(c0 + r) * b.fvs + t];
-        }
+        const int CR = C.R;
+        ms_batched<4>(
+            kb * CR, [&](int i) { return rm[i % CR]; },
+            [&](int i, int) { return fc[(int64_t)(c0 + i / CR) * b.fvs + i % CR]; },
+            [&](int i, int p, double v) { Fr[(i / CR) * kMsFld + p] += v; });
         __syncthreads();
       }
       // y = Lc^{-1} fv_c (y_i = sum_j Lt[j, i] fv_c[j]) and fv_R -= Zp fv_c
-      ms_product<T, true>(Lt, ld, c, c, Fr, kMsFld, kb,
-                          [&](int i, int r, double v) { Yr[r * kMsYld + i] = v; });
-      ms_product<T, false>(Lt + c, ld, R, c, Fr, kMsFld, kb,
-                           [&](int i, int r, double v) { Fr[r * kMsFld + c + i] -= v; });
+      ms_product<T, true>(Lt, ld, c, c, Fr, kMsFld, kb, ms_none,
+                          [&](int i, int r, double, double v) { Yr[r * kMsYld + i] = v; });
+      ms_product<T, false>(Lt + c, ld, R, c, Fr, kMsFld, kb, ms_none,
+                           [&](int i, int r, double, double v) { Fr[r * kMsFld + c + i] -= v; });
       __syncthreads();
       // z = Lc^{-T} y
       double* zb = b.z + (int64_t)c0 * b.n + S.first;
-      ms_product<T, false>(Lt, ld, c, c, Yr, kMsYld, kb, [&](int i, int r, double v) {
-        vst<true>(zb + (int64_t)r * b.n + i, v);
-      });
+      ms_product<T, false>(Lt, ld, c, c, Yr, kMsYld, kb, ms_none,
+                           [&](int i, int r, double, double v) {
+                             vst<true>(zb + (int64_t)r * b.n + i, v);
+                           });
       double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
       const bool root = j == rootn;
       for (int idx = tid; idx < kb * R; idx += 256) {
@@ -1365,17 +1439,18 @@ __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int s
       }
       const int* rw = a.rows + S.rows_off;
       const double* wb = b.w + (int64_t)c0 * b.n;
-      for (int idx = tid; idx < kb * R; idx += 256) {
-        const int r = idx / R, t = idx - r * R;
-        Fr[r * kMsFld + c + t] = vld<true>(wb + (int64_t)r * b.n + rw[t]);
-      }
+      ms_batched<4>(
+          kb * R, [&](int i) { return rw[i % R]; },
+          [&](int i, int row) { return vld<true>(wb + (int64_t)(i / R) * b.n + row); },
+          [&](int i, int, double v) { Fr[(i / R) * kMsFld + c + i % R] = v; });
       __syncthreads();
       // x_c = z_c - Zp^T x_R
       const double* zb = b.z + (int64_t)c0 * b.n + S.first;
       double* xb = b.w + (int64_t)c0 * b.n + S.first;
-      ms_product<T, true>(F + P + c, ld, c, R, Fr + c, kMsFld, kb, [&](int i, int r, double v) {
-        vst<true>(xb + (int64_t)r * b.n + i, vld<true>(zb + (int64_t)r * b.n + i) - v);
-      });
+      ms_product<T, true>(
+          F + P + c, ld, c, R, Fr + c, kMsFld, kb,
+          [&](int i, int r) { return vld<true>(zb + (int64_t)r * b.n + i); },
+          [&](int i, int r, double z, double v) { vst<true>(xb + (int64_t)r * b.n + i, z - v); });
       __syncthreads();
     }
   }

@@ -64,12 +64,14 @@ void launch_scatter(const double* w, double* x, int ldx, const int* perm, int n,
 
 // ---- vector gathers of a level -------------------------------------------
 
-// Front vectors are processed in chunks of 1024 entries (blockIdx.y), so a
-// large supernode spreads over many workgroups.  In the forward assembly a
+// Front vectors are processed in chunks of kSolveChunk entries (blockIdx.y),
+// so a large supernode spreads over many workgroups.  In the forward assembly a
 // chunk adds, child by child, the child entries whose parent position falls
 // in the chunk: deterministic, no atomics.
-constexpr int VCH = 1024;
-constexpr int kAsmRhs = 4;  // right-hand sides per LDS pass of a front-vector assembly
+constexpr int VCH = kSolveChunk;  // front-vector entries per assembly / gather task
+// right-hand sides per LDS pass of a front-vector assembly: all sixteen of
+// a matrix-core pass, so a task reads each child entry in one round trip
+constexpr int kAsmRhs = 16;
 
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -83,7 +83,7 @@ struct DfWait {
 
 }  // namespace dev
 
-constexpr int kSolveChunk = 1024;    // front-vector entries per assembly/gather task
+constexpr int kSolveChunk = 256;     // front-vector entries per assembly/gather task
 constexpr int kRowSplitCols = 1024;  // dataflow ROW tasks: columns per partial task
 constexpr int kColRowsMax = 2048;    // dataflow COL tasks: most rows (x staged in LDS)
 constexpr int kScatterChunk = 2048;  // vector entries per scatter task

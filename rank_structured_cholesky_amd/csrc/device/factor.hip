@@ -1546,6 +1546,8 @@ void Engine<T>::setup_static() {
     one.elem_bytes = many.elem_bytes = (int)sizeof(T);
     one.nrhs = 1;
     many.nrhs = 16;
+    one.chunk_group = kSingleChunkGroup;
+    many.chunk_group = 1;
     if (const char* e = std::getenv("RSC_DF_COL_ROWS")) one.col_rows = many.col_rows = std::atoi(e);
     if (const char* e = std::getenv("RSC_DF_ROW_COLS")) one.row_cols = many.row_cols = std::atoi(e);
     // the dataflow task lists are built on host threads while the device

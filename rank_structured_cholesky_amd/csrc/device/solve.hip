@@ -173,31 +173,38 @@ struct AsmChild {
   int a0, cr;      // its rows that land in this chunk: [a0, a0 + cr)
 };
 
-template <bool COH>
+// SPAN consecutive chunks (span index `chunk`) form one task: a child's
+// rows landing in them are contiguous, from the first chunk's range start
+// to the last chunk's range end.
+template <bool COH, int SPAN = 1>
 __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __restrict__ nodes,
                                const int* __restrict__ children,
                                const int* __restrict__ relmap,
                                const int* __restrict__ ranges, const VecBufs& b, int k,
                                const double* __restrict__ src, int64_t lds,
                                const int* __restrict__ perm, double* __restrict__ sm) {
-  // The chunk of kAsmRhs right-hand sides at a time is summed in LDS (sm:
-  // kAsmRhs * VCH doubles): its own entries, then child by child (a child's
-  // rows map to distinct positions), then written once.  Each entry's sum
-  // order is the same for any k.  The children's descriptors and the own
-  // entries' source indices are read once for all passes.
+  // The span of kAsmRhs right-hand sides at a time is summed in LDS (sm:
+  // kAsmRhs * SPAN * VCH doubles, at most what the caller's area holds):
+  // its own entries, then child by child (a child's rows map to distinct
+  // positions), then written once.  Each entry's sum order is the same for
+  // any k.  The children's descriptors and the own entries' source indices
+  // are read once for all passes.
+  constexpr int WS = SPAN * VCH;  // entries of a span
   __shared__ AsmChild s_ch[kAsmChildren];
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
+  const int p0 = chunk * WS, p1 = min(p0 + WS, c + R);
   if (p0 >= c + R) return;
   const int len = p1 - p0;
   const int nch = S.nchild;
+  const int q0 = chunk * SPAN, q1 = (p1 - 1) / VCH;  // first and last chunk of the span
   const bool cached = nch <= kAsmChildren;
   if (cached && tid < nch) {
     const VecNode C = nodes[children[S.child_off + tid]];
-    const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * nch + tid);
-    s_ch[tid] = AsmChild{C.relmap_off, C.fv_off + C.ncols, rg[0], rg[1] - rg[0]};
+    const int* rg0 = ranges + 2 * ((int64_t)S.range_off + (int64_t)q0 * nch + tid);
+    const int* rg1 = ranges + 2 * ((int64_t)S.range_off + (int64_t)q1 * nch + tid);
+    s_ch[tid] = AsmChild{C.relmap_off, C.fv_off + C.ncols, rg0[0], rg1[1] - rg0[0]};
   }
-  constexpr int PER = VCH / 256;
+  constexpr int PER = WS / 256;
   int own[PER];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
@@ -211,7 +218,7 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
         const int i = tid + 256 * u;
-        if (i < len) sm[cc * VCH + i] = own[u] >= 0 ? sc[own[u]] : 0.0;
+        if (i < len) sm[cc * WS + i] = own[u] >= 0 ? sc[own[u]] : 0.0;
       }
     }
     __syncthreads();
@@ -225,11 +232,12 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
         cr = uni(s_ch[q].cr);
       } else {
         const VecNode C = uniform_node(nodes[uni(children[S.child_off + q])]);
-        const int* rg = ranges + 2 * ((int64_t)S.range_off + (int64_t)chunk * nch + q);
+        const int* rg0 = ranges + 2 * ((int64_t)S.range_off + (int64_t)q0 * nch + q);
+        const int* rg1 = ranges + 2 * ((int64_t)S.range_off + (int64_t)q1 * nch + q);
         rm_off = C.relmap_off;
         fc_off = C.fv_off + C.ncols;
-        a0 = uni(rg[0]);
-        cr = uni(rg[1]) - a0;
+        a0 = uni(rg0[0]);
+        cr = uni(rg1[1]) - a0;
       }
       const int* rm = relmap + rm_off;
       for (int t = tid; t < cr; t += 256) {
@@ -241,13 +249,13 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
           v[cc] = cc < g ? vld<COH>(b.fv + fc_off + (int64_t)(c0 + cc) * b.fvs + a) : 0.0;
 #pragma unroll
         for (int cc = 0; cc < kAsmRhs; ++cc)
-          if (cc < g) sm[cc * VCH + p] += v[cc];
+          if (cc < g) sm[cc * WS + p] += v[cc];
       }
       __syncthreads();
     }
     for (int idx = tid; idx < g * len; idx += 256) {
       const int cc = idx / len, i = idx - cc * len;
-      vst<COH>(b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + p0 + i, sm[cc * VCH + i]);
+      vst<COH>(b.fv + S.fv_off + (int64_t)(c0 + cc) * b.fvs + p0 + i, sm[cc * WS + i]);
     }
     __syncthreads();
   }
@@ -256,11 +264,11 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
 // Backward gather of one chunk: fv_R[a] = w[rows[a]] (the solution at the
 // node's off-diagonal rows) and w[cols] = z[cols] (start of x_c); FLAT runs
 // all right-hand sides in one pass.
-template <bool COH, bool FLAT = false>
+template <bool COH, bool FLAT = false, int SPAN = 1>
 __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
                              const VecBufs& b, int k) {
   const int c = S.ncols, R = S.R, tid = threadIdx.x;
-  const int p0 = chunk * VCH, p1 = min(p0 + VCH, c + R);
+  const int p0 = chunk * SPAN * VCH, p1 = min(p0 + SPAN * VCH, c + R);
   if (p0 >= c + R) return;
   const int* rw = rows + S.rows_off;
   if (!FLAT) {
@@ -1494,7 +1502,7 @@ struct DfLdsScalar {
     double xl[kXlds];  // also the one-right-hand-side assembly chunk (VCH)
   };
 };
-static_assert(kXlds >= VCH, "the assembly chunk fits the x staging area");
+static_assert(kXlds >= kSingleChunkGroup * VCH, "the assembly span fits the x staging area");
 static_assert(kXlds >= kColRowsMax, "a column task's x fits the staging area");
 constexpr int kMultiLds = kMfmaLds > kAsmRhs * VCH ? kMfmaLds : kAsmRhs * VCH;
 struct DfLdsMulti {
@@ -1560,16 +1568,25 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     }
     if (a.trace && tid == 0) s_tready = wall_clock64();
     if (type == 0) {
-      // the chunk is summed in LDS (one right-hand side: the first VCH
-      // entries of the x staging area), then stored once
-      if constexpr (MULTI)
+      // the chunk is summed in LDS (one right-hand side: the first
+      // kSingleChunkGroup * VCH entries of the x staging area), then stored
+      // once; a task of the single-right-hand-side list spans
+      // kSingleChunkGroup chunks
+      if constexpr (MULTI) {
         assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
                              a.ranges, b, k, a.B, a.ldb, a.perm, lds.mf);
-      else
-        assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
-                             a.ranges, b, 1, a.B, a.ldb, a.perm, lds.xl);
+      } else {
+        assemble_chunk<true, kSingleChunkGroup>(uniform_node(a.nodes[ta]), tb, a.nodes,
+                                                a.children, a.relmap, a.ranges, b, 1, a.B,
+                                                a.ldb, a.perm, lds.xl);
+      }
     } else if (type == 1) {
-      gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
+      if constexpr (MULTI) {
+        gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
+      } else {
+        gather_chunk<true, false, kSingleChunkGroup>(uniform_node(a.nodes[ta]), tb, a.rows, b,
+                                                     k);
+      }
     } else if (type == 2) {
       if constexpr (MULTI) gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.mf);
       else gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.xl);

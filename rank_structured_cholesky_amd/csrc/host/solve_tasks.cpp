@@ -521,19 +521,31 @@ struct CostModel {
     switch (t.type) {
       case 0: {
         const dev::VecNode& v = L.vn[t.a];
-        double ent = std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)t.b * kSolveChunk);
-        for (int q = 0; q < v.nchild; ++q) {
-          const int64_t r = 2 * ((int64_t)v.range_off + (int64_t)t.b * v.nchild + q);
-          ent += L.asm_ranges[r + 1] - L.asm_ranges[r];
+        const int G = std::max(1, m.chunk_group);
+        const int nch = (v.ncols + v.R + kSolveChunk - 1) / kSolveChunk;
+        double us = 0;
+        for (int ch = t.b * G; ch < std::min(nch, (t.b + 1) * G); ++ch) {
+          double ent = std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)ch * kSolveChunk);
+          for (int q = 0; q < v.nchild; ++q) {
+            const int64_t r = 2 * ((int64_t)v.range_off + (int64_t)ch * v.nchild + q);
+            ent += L.asm_ranges[r + 1] - L.asm_ranges[r];
+          }
+          us += multi ? 5.5 + 2.7e-3 * ent * k + 8.2 * v.nchild
+                      : 0.8 + 4.1e-3 * ent * k + 1.5 * v.nchild;
         }
-        return multi ? 5.5 + 2.7e-3 * ent * k + 8.2 * v.nchild
-                     : 0.8 + 4.1e-3 * ent * k + 1.5 * v.nchild;
+        return us;
       }
       case 1: {
         const dev::VecNode& v = L.vn[t.a];
-        const double ent =
-            std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)t.b * kSolveChunk);
-        return multi ? 5.4 + 4.5e-3 * ent * k : 2.7 + 2.5e-3 * ent * k;
+        const int G = std::max(1, m.chunk_group);
+        const int nch = (v.ncols + v.R + kSolveChunk - 1) / kSolveChunk;
+        double us = 0;
+        for (int ch = t.b * G; ch < std::min(nch, (t.b + 1) * G); ++ch) {
+          const double ent =
+              std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)ch * kSolveChunk);
+          us += multi ? 5.4 + 4.5e-3 * ent * k : 2.7 + 2.5e-3 * ent * k;
+        }
+        return us;
       }
       case 2: {
         const dev::GvTask& g = gv[t.a];
@@ -727,7 +739,12 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     }
     return last;
   };
-  auto chunks = [&](int j) { return (P.nodes[j].ncols + P.nodes[j].R() + VCH - 1) / VCH; };
+  // assembly / gather tasks of a supernode: groups of chunk_group chunks
+  const int CG = std::max(1, tm.chunk_group);
+  auto chunks = [&](int j) {
+    const int nch = (P.nodes[j].ncols + P.nodes[j].R() + VCH - 1) / VCH;
+    return (nch + CG - 1) / CG;
+  };
   std::vector<NodeStages> st(ns);
   for (int j = 0; j < ns; ++j) st[j] = node_stages(P, L, j);
   Subtrees none;

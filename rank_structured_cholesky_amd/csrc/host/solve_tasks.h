@@ -170,9 +170,14 @@ void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
 // otherwise the list is ordered by a list schedule on that many workgroups
 // (resident workgroups of the launch), with task costs estimated for nrhs
 // right-hand sides and a factor of elem_bytes per element.
+// Consecutive kSolveChunk chunks per assembly / gather task of the
+// single-right-hand-side list (its tasks are short: fewer, longer ones pay
+// the per-task overhead less often); the multi-right-hand-side list has one.
+constexpr int kSingleChunkGroup = 4;
 struct TicketModel {
   int workers = 0;
   int nrhs = 1;
+  int chunk_group = 1;  // chunks per assembly / gather task (b = group index)
   int elem_bytes = 8;
   int col_rows = 0;  // rows per COL task before partial sums (0: 512)
   int row_cols = 0;  // columns per ROW task before partial sums (0: kRowSplitCols)

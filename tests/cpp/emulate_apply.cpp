@@ -264,9 +264,21 @@ void subtree(State& S, int sb, bool fwd, const Subtrees& T, bool multi) {
 }
 
 void run_df_task(State& S, const DataflowTasks& D, const dev::DfTask& t) {
+  // an assembly / gather task (a = supernode) covers chunk_group
+  // consecutive chunks
+  const int G = S.multi ? 1 : kSingleChunkGroup;
+  auto nchunks = [&]() {
+    const Supernode& sn = S.P->nodes.at(t.a);
+    return (sn.ncols + sn.R() + kSolveChunk - 1) / kSolveChunk;
+  };
   switch (t.type) {
-    case 0: assemble(S, t.a, t.b, false); break;
-    case 1: gather(S, t.a, t.b); break;
+    case 0:
+      for (int ch = t.b * G; ch < std::min(nchunks(), (t.b + 1) * G); ++ch)
+        assemble(S, t.a, ch, false);
+      break;
+    case 1:
+      for (int ch = t.b * G; ch < std::min(nchunks(), (t.b + 1) * G); ++ch) gather(S, t.a, ch);
+      break;
     case 2: gemv(S, chk(D.gv, t.a, "df gv")); break;
     case 3: scatter(S, t.a); break;
     case 4: subtree(S, t.a, true, *S.dsub, S.multi); break;
@@ -315,6 +327,7 @@ int main(int argc, char** argv) {
   tm.workers = argc > 8 ? std::atoi(argv[8]) : 1024;
   tm.wide_elems = argc > 9 ? std::atoll(argv[9]) : 0;
   tm.nrhs = k;
+  tm.chunk_group = multi ? 1 : kSingleChunkGroup;
   const auto tb0 = std::chrono::steady_clock::now();
   build_dataflow_tasks(P, L, D, multi ? &L.sub_multi : &L.sub, tm);
   const double build_ms =

@@ -221,9 +221,21 @@ typedef struct rsc_server rsc_server;
 typedef struct {
   double eps;          /* relative residual target of every column */
   int max_iters;       /* per column */
-  int slots;           /* column slots per factor (1..64) */
+  int slots;           /* most column slots of one factor (1..64) */
+  /* Device memory for the column slots of all factors together (each
+   * factor's slot vectors and the apply buffers of its columns).  The
+   * budget is divided among the factors by water-filling over their bytes
+   * per column, up to `slots` each and at least one; 0 = half of the device
+   * memory free when the server is created. */
+  int64_t memory_bytes;
+  /* 1: release every factor's numeric workspace at creation (as
+   * rsc_factor_release_workspace), since served factors are not
+   * refactorized while the server exists */
+  int release_workspace;
+  int reserved;
 } rsc_server_options;
-rsc_status rsc_server_options_default(rsc_server_options* o);  /* 1e-8, 10000, 32 */
+/* 1e-8, 10000, 64 slots, memory_bytes 0, release_workspace 1 */
+rsc_status rsc_server_options_default(rsc_server_options* o);
 rsc_status rsc_server_create(rsc_factor* const* factors, int nfactors,
                              const rsc_server_options* opt, rsc_server** out);
 /* B (n x nrhs, ld ldb) and X (n x nrhs, ld ldx) are DEVICE pointers that
@@ -236,12 +248,23 @@ rsc_status rsc_server_submit(rsc_server* s, int factor, int nrhs, const double* 
 rsc_status rsc_server_wait(rsc_server* s, int64_t ticket, rsc_pcg_result* res);
 /* Steps run and the largest number of busy slots of any step, so far. */
 rsc_status rsc_server_stats(const rsc_server* s, int64_t* steps, int* max_busy);
+/* Column slots the memory budget gave factor `factor` (its index at
+ * creation). */
+rsc_status rsc_server_slots(const rsc_server* s, int factor, int* slots);
 void rsc_server_destroy(rsc_server* s);
 
 /* y = A x on the device (the PCG operator; DEVICE pointers). */
 rsc_status rsc_spmv(rsc_factor* f, const double* x, double* y, void* stream);
 
 rsc_status rsc_factor_get_info(const rsc_factor* f, rsc_factor_info* info);
+
+/* Frees the numeric factorization's workspace (update matrices, work
+ * panels, sketch and QR scratch: rsc_factor_info.workspace_bytes, then 0).
+ * The factor, its apply and PCG keep working; the next rsc_factor_refactor
+ * allocates the workspace and builds the numeric program again.  No
+ * counterpart in the reference (its factor frees its transient storage as
+ * it goes); for callers that keep many factors resident. */
+rsc_status rsc_factor_release_workspace(rsc_factor* f);
 
 /* Permutation used by the factor: perm[new] = old (n entries, host). */
 rsc_status rsc_factor_get_permutation(const rsc_factor* f, int* perm);

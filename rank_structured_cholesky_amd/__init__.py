@@ -256,7 +256,12 @@ class FactorPreconditioner:
         self._options = None
 
     def __del__(self):
-        self.clearSystem()
+        # at interpreter shutdown the module's globals may already be gone;
+        # the process then releases the device memory itself
+        try:
+            self.clearSystem()
+        except (AttributeError, TypeError, ImportError):
+            pass
 
     def clearSystem(self):
         if self._h:
@@ -339,6 +344,11 @@ class FactorPreconditioner:
         x[...] = X.reshape(b.shape)
         return x
 
+    def release_workspace(self):
+        """Free the numeric factorization's workspace; solves keep working,
+        and a later refactorization allocates it again."""
+        check(lib().rsc_factor_release_workspace(self._h))
+
     def info(self):
         inf = _lib.FactorInfo()
         check(lib().rsc_factor_get_info(self._h, C.byref(inf)))
@@ -418,10 +428,17 @@ class RequestServer:
     alive until wait() returns for its ticket.  The factors must not be used
     elsewhere while the server exists."""
 
-    def __init__(self, factors, eps=1e-8, max_iters=10000, slots=32):
+    def __init__(self, factors, eps=1e-8, max_iters=10000, slots=64, memory_bytes=0,
+                 release_workspace=True):
+        """slots: most column slots of one factor; memory_bytes: device
+        memory for the slots of all factors (0: half of the free memory),
+        divided among the factors by their bytes per column;
+        release_workspace: free the factors' numeric workspaces (they are
+        not refactorized while the server exists)."""
         self._factors = list(factors)  # keep the handles alive
         hs = (C.c_void_p * len(self._factors))(*[M.handle for M in self._factors])
-        o = _lib.ServerOptions(float(eps), int(max_iters), int(slots))
+        o = _lib.ServerOptions(float(eps), int(max_iters), int(slots), int(memory_bytes),
+                               int(bool(release_workspace)), 0)
         h = C.c_void_p()
         check(lib().rsc_server_create(hs, len(self._factors), C.byref(o), C.byref(h)))
         self._h = h
@@ -444,6 +461,12 @@ class RequestServer:
         steps, busy = C.c_int64(), C.c_int()
         check(lib().rsc_server_stats(self._h, C.byref(steps), C.byref(busy)))
         return dict(steps=steps.value, max_busy=busy.value)
+
+    def slots(self, factor):
+        """Column slots the memory budget gave factor `factor`."""
+        k = C.c_int()
+        check(lib().rsc_server_slots(self._h, int(factor), C.byref(k)))
+        return k.value
 
     def close(self):
         if self._h:

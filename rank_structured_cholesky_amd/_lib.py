@@ -33,7 +33,9 @@ class PcgResult(C.Structure):
 
 
 class ServerOptions(C.Structure):
-    _fields_ = [("eps", C.c_double), ("max_iters", C.c_int), ("slots", C.c_int)]
+    _fields_ = [("eps", C.c_double), ("max_iters", C.c_int), ("slots", C.c_int),
+                ("memory_bytes", C.c_int64), ("release_workspace", C.c_int),
+                ("reserved", C.c_int)]
 
 
 class FactorInfo(C.Structure):
@@ -102,6 +104,8 @@ SIGNATURES = {
     "rsc_selftest_gemm": (_I, [_I, _I, _I, _I, _I, _I, _D, _DP, _I, _DP, _I, _D, _DP, _I]),
     "pcg_pysolve": (None, [_I, _IP, _IP, _DP, _DP, _DP]),
     "rsc_server_options_default": (_I, [C.POINTER(ServerOptions)]),
+    "rsc_server_slots": (_I, [_P, _I, C.POINTER(_I)]),
+    "rsc_factor_release_workspace": (_I, [_P]),
     "rsc_server_create": (_I, [C.POINTER(_P), _I, C.POINTER(ServerOptions), C.POINTER(_P)]),
     "rsc_server_submit": (_I, [_P, _I, _I, _P, _I, _P, _I, _I64P]),
     "rsc_server_wait": (_I, [_P, _I64, C.POINTER(PcgResult)]),

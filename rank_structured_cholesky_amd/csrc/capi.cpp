@@ -370,6 +370,15 @@ rsc_status rsc_factor_get_info(const rsc_factor* f, rsc_factor_info* info) {
   return RSC_OK;
 }
 
+rsc_status rsc_factor_release_workspace(rsc_factor* f) {
+  if (!f) return fail(RSC_BAD_INPUT, "null factor");
+  return guard([&]() -> rsc_status {
+    f->eng->release_workspace();
+    f->info.workspace_bytes = f->eng->workspace_bytes();
+    return RSC_OK;
+  });
+}
+
 rsc_status rsc_factor_get_permutation(const rsc_factor* f, int* perm) {
   if (!f || !perm) return fail(RSC_BAD_INPUT, "null argument");
   std::copy(f->plan.perm.begin(), f->plan.perm.end(), perm);

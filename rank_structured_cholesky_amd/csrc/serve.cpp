@@ -9,6 +9,7 @@
 // already queued), retires converged columns (x copied out, true residual)
 // and admits queued columns into the freed slots between steps, so requests
 // that arrive while others run join at the next step.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -268,13 +269,76 @@ void rsc_server::run() {
   cv_done.notify_all();
 }
 
+namespace {
+
+// Device bytes one column slot of factor f takes: the slot's PCG vectors
+// (b, x, r, d, q, s) and the apply buffers of one more column.
+int64_t column_bytes(const rsc_factor* f) {
+  const int64_t n = f->plan.n;
+  return 6 * n * (int64_t)sizeof(double) +
+         f->eng->apply_scratch_doubles(1) * (int64_t)sizeof(double) + 4096;
+}
+
+// Slots per factor from one memory budget: water-filling over the bytes per
+// column (every factor an equal share; a factor that reaches the slot
+// limit gives the rest of its share back to the others), at least one slot
+// each.  The budget is o.memory_bytes, or half of the free device memory.
+std::vector<int> slot_counts(rsc_factor* const* fs, int nf, const rsc_server_options& o) {
+  int64_t budget = o.memory_bytes;
+  if (budget <= 0) {
+    size_t freeb = 0, total = 0;
+    RSC_HIP(hipMemGetInfo(&freeb, &total));
+    budget = (int64_t)(freeb / 2);
+  }
+  std::vector<int64_t> cb(nf);
+  for (int i = 0; i < nf; ++i) cb[i] = column_bytes(fs[i]);
+  std::vector<int> k(nf, 0);
+  std::vector<bool> open(nf, true);
+  int64_t left = budget;
+  for (int round = 0; round < nf; ++round) {
+    int nopen = 0;
+    for (int i = 0; i < nf; ++i) nopen += open[i] ? 1 : 0;
+    if (nopen == 0 || left <= 0) break;
+    const int64_t share = left / nopen;
+    bool capped = false;
+    for (int i = 0; i < nf; ++i) {
+      if (!open[i]) continue;
+      const int64_t want = (int64_t)o.slots - k[i];
+      const int64_t can = std::min<int64_t>(want, share / cb[i]);
+      if (can >= want) {
+        k[i] = o.slots;
+        left -= want * cb[i];
+        open[i] = false;
+        capped = true;
+      }
+    }
+    if (!capped) {
+      // no factor reaches the limit: every open one takes its share
+      for (int i = 0; i < nf; ++i)
+        if (open[i]) {
+          const int64_t can = std::min<int64_t>((int64_t)o.slots - k[i], share / cb[i]);
+          k[i] += (int)can;
+          left -= can * cb[i];
+          open[i] = false;
+        }
+    }
+  }
+  for (int i = 0; i < nf; ++i) k[i] = std::max(1, k[i]);
+  return k;
+}
+
+}  // namespace
+
 extern "C" {
 
 rsc_status rsc_server_options_default(rsc_server_options* o) {
   if (!o) return fail(RSC_BAD_INPUT, "null options");
   o->eps = 1e-8;
   o->max_iters = 10000;
-  o->slots = 32;
+  o->slots = rsc::kSlotList;
+  o->memory_bytes = 0;
+  o->release_workspace = 1;
+  o->reserved = 0;
   return RSC_OK;
 }
 
@@ -292,27 +356,35 @@ rsc_status rsc_server_create(rsc_factor* const* factors, int nfactors,
   rsc_server_options o;
   if (opt) o = *opt;
   else rsc_server_options_default(&o);
-  if (o.slots < 1 || o.slots > rsc::kSlotList || o.max_iters < 0 || !(o.eps >= 0))
-    return fail(RSC_BAD_INPUT, "bad server options (slots 1..64)");
+  if (o.slots < 1 || o.slots > rsc::kSlotList || o.max_iters < 0 || !(o.eps >= 0) ||
+      o.memory_bytes < 0)
+    return fail(RSC_BAD_INPUT, "bad server options (slots 1..64, memory_bytes >= 0)");
   try {
     std::unique_ptr<rsc_server> S(new rsc_server());
     S->opt = o;
+    if (o.release_workspace)
+      for (int i = 0; i < nfactors; ++i) {
+        factors[i]->eng->release_workspace();
+        factors[i]->info.workspace_bytes = factors[i]->eng->workspace_bytes();
+      }
+    const std::vector<int> caps = slot_counts(factors, nfactors, o);
     for (int i = 0; i < nfactors; ++i) {
       std::unique_ptr<Lane> L(new Lane());
       L->f = factors[i];
-      L->kcap = o.slots;
+      const int kc = caps[i];
+      L->kcap = kc;
       const int n = L->f->plan.n;
       RSC_HIP(hipStreamCreate(&L->st));
-      L->sb.alloc(n, o.slots);
-      L->slot_req.assign(o.slots, nullptr);
-      L->slot_col.assign(o.slots, 0);
-      L->slot_since.assign(o.slots, 0);
-      RSC_HIP(hipHostMalloc((void**)&L->snap, 2 * (size_t)o.slots * sizeof(int),
+      L->sb.alloc(n, kc);
+      L->slot_req.assign(kc, nullptr);
+      L->slot_col.assign(kc, 0);
+      L->slot_since.assign(kc, 0);
+      RSC_HIP(hipHostMalloc((void**)&L->snap, 2 * (size_t)kc * sizeof(int),
                             hipHostMallocDefault));
       for (hipEvent_t& e : L->sev) RSC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       // the apply's per-column buffers, grown once here (never mid-stream)
       RSC_HIP(hipMemsetAsync(L->sb.r.p, 0, L->sb.r.bytes, L->st));
-      L->f->eng->apply(o.slots, L->sb.r.as<double>(), n, L->sb.s.as<double>(), n, L->st);
+      L->f->eng->apply(kc, L->sb.r.as<double>(), n, L->sb.s.as<double>(), n, L->st);
       RSC_HIP(hipStreamSynchronize(L->st));
       L->f->eng->check_apply(L->st);
       S->lanes.push_back(std::move(L));
@@ -398,6 +470,13 @@ rsc_status rsc_server_stats(const rsc_server* s, int64_t* steps, int* max_busy) 
   std::lock_guard<std::mutex> lk(const_cast<rsc_server*>(s)->mu);
   if (steps) *steps = s->total_steps;
   if (max_busy) *max_busy = s->max_busy;
+  return RSC_OK;
+}
+
+rsc_status rsc_server_slots(const rsc_server* s, int factor, int* slots) {
+  if (!s || !slots || factor < 0 || factor >= (int)s->lanes.size())
+    return fail(RSC_BAD_INPUT, "bad argument");
+  *slots = s->lanes[factor]->kcap;
   return RSC_OK;
 }
 

@@ -117,7 +117,8 @@ def test_server_options_and_bad_input():
     L = _lib.lib()
     o = _lib.ServerOptions()
     _lib.check(L.rsc_server_options_default(C.byref(o)))
-    assert (o.eps, o.max_iters, o.slots) == (1e-8, 10000, 32)
+    assert (o.eps, o.max_iters, o.slots, o.memory_bytes, o.release_workspace) == (
+        1e-8, 10000, 64, 0, 1)
     h = C.c_void_p()
     assert L.rsc_server_create(None, 0, C.byref(o), C.byref(h)) == _lib.RSC_BAD_INPUT
     none = (C.c_void_p * 1)(None)
@@ -132,4 +133,11 @@ def test_server_options_and_bad_input():
     assert L.rsc_server_submit(None, 0, 1, None, 1, None, 1, C.byref(t)) == _lib.RSC_BAD_INPUT
     assert L.rsc_server_wait(None, 1, None) == _lib.RSC_BAD_INPUT
     assert L.rsc_apply_status(None, None) == _lib.RSC_BAD_INPUT
+    k = C.c_int()
+    assert L.rsc_server_slots(None, 0, C.byref(k)) == _lib.RSC_BAD_INPUT
+    assert L.rsc_factor_release_workspace(None) == _lib.RSC_BAD_INPUT
+    bad = _lib.ServerOptions()
+    _lib.check(L.rsc_server_options_default(C.byref(bad)))
+    bad.memory_bytes = -1
+    assert L.rsc_server_create(twice, 1, C.byref(bad), C.byref(h)) == _lib.RSC_BAD_INPUT
     L.rsc_server_destroy(None)

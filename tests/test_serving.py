@@ -111,3 +111,54 @@ def test_served_results_equal_solo_solves():
             assert np.array_equal(x0, xs), i
         else:
             assert np.linalg.norm(x0 - xs) <= 1e-6 * np.linalg.norm(xs), i
+
+
+def test_server_memory_budget_and_workspace_release():
+    """Slots come from one device-memory budget, divided by each factor's
+    bytes per column: a 1-byte budget leaves one slot per factor (a request
+    wider than that is admitted column by column and still equals its solo
+    solve bitwise), an unlimited one the 64-slot limit, a moderate one fewer
+    slots for the largest system than for the smallest.  The server releases
+    the factors' numeric workspaces; a later refactorization allocates them
+    again and gives the same preconditioner bitwise."""
+    torch = pytest.importorskip("torch")
+    fac = _factors()
+    mats = [A for A, _ in fac]
+    Ms = [M for _, M in fac]
+    assert all(M.info()["workspace_bytes"] > 0 for M in Ms)
+    b = R.uniform_rhs(mats[0]._nrow, 77)
+    y0 = np.empty_like(b)
+    Ms[0].solve(y0, b)
+    B = R.uniform_rhs(mats[0]._nrow * 3, 78).reshape(3, -1).T.copy()
+    xs, rs = _solo(torch, Ms[0], B)
+
+    srv = R.RequestServer(Ms, eps=EPS, max_iters=500, slots=64, memory_bytes=1)
+    assert [srv.slots(f) for f in range(4)] == [1, 1, 1, 1]
+    assert all(M.info()["workspace_bytes"] == 0 for M in Ms)
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")
+    dX = torch.full_like(dB, float("nan"))
+    torch.cuda.synchronize()
+    n = dB.shape[1]
+    res = srv.wait(srv.submit(0, 3, dB.data_ptr(), n, dX.data_ptr(), n), 3)
+    torch.cuda.synchronize()
+    assert srv.stats()["max_busy"] == 1
+    srv.close()
+    assert np.array_equal(dX.cpu().numpy().T, xs)
+    for r, (it, rec, tru) in zip(res, rs):
+        assert (r["iterations"], r["recurrence_relres"], r["true_relres"]) == (it, rec, tru)
+
+    big = R.RequestServer(Ms, eps=EPS, max_iters=500, slots=64, memory_bytes=1 << 40)
+    assert [big.slots(f) for f in range(4)] == [64, 64, 64, 64]
+    big.close()
+    # 40 MB: the 20^3 system (largest columns) cannot reach the limit, the
+    # 12^3 one (smallest) gets at least as many slots
+    mid = R.RequestServer(Ms, eps=EPS, max_iters=500, slots=64, memory_bytes=40 << 20)
+    k = [mid.slots(f) for f in range(4)]
+    mid.close()
+    assert k[1] < 64 and k[2] >= k[1] and min(k) >= 1, k
+
+    _lib.check(_lib.lib().rsc_factor_refactor(Ms[0].handle, _lib.dptr(mats[0]._x)))
+    assert Ms[0].info()["workspace_bytes"] > 0
+    y1 = np.empty_like(b)
+    Ms[0].solve(y1, b)
+    assert np.array_equal(y0, y1)

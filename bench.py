@@ -49,7 +49,7 @@ CONFIGS = {
     # request serving over eight resident factors (continuous batching)
     "serving": dict(grids=(32, 48, 64, 80, 96, 96, 112, 128), sigma=1.0,
                     max_dense_block=255, max_rank=64, oversampling=5, requests=24,
-                    max_rhs=256, rate=20.0, slots=32),
+                    max_rhs=256, rate=20.0, slots=64),
 }
 
 HBM_PEAK_GBS = 8000.0
@@ -127,6 +127,7 @@ def serving_leg(cfg, eps):
 
     # (a) continuous batching
     srv = R.RequestServer(facs, eps=eps, max_iters=10000, slots=cfg["slots"])
+    slots_given = [srv.slots(f) for f in range(len(facs))]
     tickets = [None] * nreq
     t0 = time.perf_counter()
 
@@ -180,7 +181,9 @@ def serving_leg(cfg, eps):
     return dict(
         rhs_total=total, requests=nreq, wall_s=wall_a, solves_per_s=total / wall_a,
         p50_ms=1e3 * pct(lat_a, 0.5), p99_ms=1e3 * pct(lat_a, 0.99),
-        steps=st["steps"], max_busy_slots=st["max_busy"], worst_true_relres=worst,
+        steps=st["steps"], max_busy_slots=st["max_busy"], slots_given=slots_given,
+        workspace_gb_after_release=round(sum(M.info()["workspace_bytes"] for M in facs) / 1e9, 3),
+        worst_true_relres=worst,
         iterations_mean=float(np.mean(iters_a)),
         one_at_a_time=dict(wall_s=wall_b, solves_per_s=total / wall_b,
                            p50_ms=1e3 * pct(lat_b, 0.5), p99_ms=1e3 * pct(lat_b, 0.99)),

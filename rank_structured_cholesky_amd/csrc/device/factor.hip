@@ -106,7 +106,10 @@ inline bool inside_subtree(const Supernode& s, int q, int sub) {
 struct DfList {
   DBuf tasks, gv, gw, cnt;
   int ntasks = 0, ngroups = 0, grid = 1;
-  std::vector<dev::DfTask> host;  // kept for the optional timing trace
+  // kept for the optional timing trace
+  std::vector<dev::DfTask> host;
+  std::vector<dev::DfWait> host_wait;
+  std::vector<dev::GvTask> host_gv;
 };
 
 template <typename T>
@@ -1418,7 +1421,11 @@ void Engine<T>::finish_solve_setup() {
     D.gw.upload(waits, st_);
     D.cnt.alloc((size_t)(D.ngroups + 1) * sizeof(unsigned));
     D.grid = std::max(1, std::min(D.ntasks, df_resident_[q]));
-    if (!df_trace_path_.empty()) D.host = DT.tasks;
+    if (!df_trace_path_.empty()) {
+      D.host = DT.tasks;
+      D.host_wait = waits;
+      D.host_gv = DT.gv;
+    }
     slots = std::max(slots, DT.slots);
     maxtasks = std::max<int64_t>(maxtasks, D.ntasks);
   }
@@ -2072,8 +2079,10 @@ void Engine<T>::check_apply(hipStream_t s) {
   RSC_HIP(hipStreamSynchronize(s));
   raise_apply_status(st);
   if (dtrace_.p) {
-    // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each
-    const std::vector<dev::DfTask>& df_host_ = df_[df_last_].host;
+    // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each,
+    // then the wait-list length, the wait list and the matrix-vector entries
+    const DfList& DL = df_[df_last_];
+    const std::vector<dev::DfTask>& df_host_ = DL.host;
     std::vector<unsigned long long> tr(4 * df_host_.size());
     RSC_HIP(hipMemcpy(tr.data(), dtrace_.p, tr.size() * sizeof(unsigned long long),
                       hipMemcpyDeviceToHost));
@@ -2082,6 +2091,10 @@ void Engine<T>::check_apply(hipStream_t s) {
       std::fwrite(&nt, sizeof(nt), 1, fp);
       std::fwrite(df_host_.data(), sizeof(dev::DfTask), df_host_.size(), fp);
       std::fwrite(tr.data(), sizeof(unsigned long long), tr.size(), fp);
+      const int64_t nw = (int64_t)DL.host_wait.size();
+      std::fwrite(&nw, sizeof(nw), 1, fp);
+      std::fwrite(DL.host_wait.data(), sizeof(dev::DfWait), DL.host_wait.size(), fp);
+      std::fwrite(DL.host_gv.data(), sizeof(dev::GvTask), DL.host_gv.size(), fp);
       std::fclose(fp);
     }
   }

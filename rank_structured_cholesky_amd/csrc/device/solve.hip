@@ -101,6 +101,16 @@ __device__ inline int64_t uni64(int64_t v) {
   const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((uint64_t)v >> 32));
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
+// The thread index as a value the compiler cannot hoist: the task bodies of
+// the persistent dataflow kernel run inside its task loop, and values
+// derived from a plain threadIdx.x (LDS and vector addresses of every body)
+// would be computed once before the loop and kept live across all bodies,
+// beyond the register budget of four resident workgroups per compute unit.
+__device__ inline int body_tid() {
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
 __device__ inline GvTask uniform_task(const GvTask& t) {
   GvTask u;
   u.mat = uni64(t.mat);
@@ -191,7 +201,7 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
   // are read once for all passes.
   constexpr int WS = SPAN * VCH;  // entries of a span
   __shared__ AsmChild s_ch[kAsmChildren];
-  const int c = S.ncols, R = S.R, tid = threadIdx.x;
+  const int c = S.ncols, R = S.R, tid = body_tid();
   const int p0 = chunk * WS, p1 = min(p0 + WS, c + R);
   if (p0 >= c + R) return;
   const int len = p1 - p0;
@@ -213,12 +223,28 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
   }
   for (int c0 = 0; c0 < k; c0 += kAsmRhs) {
     const int g = min(kAsmRhs, k - c0);
-    for (int cc = 0; cc < g; ++cc) {
-      const double* sc = src + (int64_t)(c0 + cc) * lds;
+    if constexpr (PER == 1) {
+      // the thread's entry for every right-hand side of the pass, loaded
+      // together (clamped in-range addresses, masked values)
+      double v[kAsmRhs];
+      const int o = own[0];
 #pragma unroll
-      for (int u = 0; u < PER; ++u) {
-        const int i = tid + 256 * u;
-        if (i < len) sm[cc * WS + i] = own[u] >= 0 ? sc[own[u]] : 0.0;
+      for (int cc = 0; cc < kAsmRhs; ++cc) {
+        const bool ok = cc < g && o >= 0;
+        const double x = src[ok ? (int64_t)(c0 + cc) * lds + o : 0];
+        v[cc] = ok ? x : 0.0;
+      }
+      if (tid < len)
+#pragma unroll
+        for (int cc = 0; cc < kAsmRhs; ++cc) sm[cc * WS + tid] = v[cc];
+    } else {
+      for (int cc = 0; cc < g; ++cc) {
+        const double* sc = src + (int64_t)(c0 + cc) * lds;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int i = tid + 256 * u;
+          if (i < len) sm[cc * WS + i] = own[u] >= 0 ? sc[own[u]] : 0.0;
+        }
       }
     }
     __syncthreads();
@@ -267,7 +293,7 @@ __device__ void assemble_chunk(const VecNode& S, int chunk, const VecNode* __res
 template <bool COH, bool FLAT = false, int SPAN = 1>
 __device__ void gather_chunk(const VecNode& S, int chunk, const int* __restrict__ rows,
                              const VecBufs& b, int k) {
-  const int c = S.ncols, R = S.R, tid = threadIdx.x;
+  const int c = S.ncols, R = S.R, tid = body_tid();
   const int p0 = chunk * SPAN * VCH, p1 = min(p0 + SPAN * VCH, c + R);
   if (p0 >= c + R) return;
   const int* rw = rows + S.rows_off;
@@ -352,7 +378,7 @@ __global__ __launch_bounds__(256) void k_gemv_small(const T* __restrict__ F,
   if (b.live && *b.live == 0) return;
   const GvTask t = tasks[blockIdx.x];
   const T* __restrict__ M = F + t.mat;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
   for (int col = 0; col < k; ++col) {
     double* y = vsel(b, t.ysel, col) + t.yoff;
     if (t.kind == 2) {
@@ -403,7 +429,7 @@ template <typename T>
 __device__ void gemv_task(const T* __restrict__ F, const GvTask& t, const VecBufs& b, int k,
                           double (*red)[64]) {
   const T* __restrict__ M = F + t.mat;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
   for (int col = 0; col < k; ++col) {
     double* y = vsel(b, t.ysel, col) + t.yoff;
     if (t.kind == 2) {
@@ -532,7 +558,7 @@ template <typename T, bool COH, int KRHS>
 __device__ void gemv_multi(const T* __restrict__ F, const GvTask& t, const VecBufs& b, int k,
                            double (*red)[64], double* xl) {
   const T* __restrict__ M = F + t.mat;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
   constexpr int NB = kXlds / KRHS;
   for (int c0 = 0; c0 < k; c0 += KRHS) {
     const int kb = min(KRHS, k - c0);
@@ -681,14 +707,80 @@ __device__ inline d4v mfma64(double a, double b, d4v c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Stride between the right-hand-side columns of vector sel (see vsel).
+__device__ inline int64_t vstride(const VecBufs& b, int sel) {
+  const int64_t n = b.n, fvs = b.fvs, tvs = b.tvs;
+  return sel == 1 ? fvs : (sel == 2 ? tvs : n);
+}
+
+// The 16 x kRowChunk window of a ROW task's x (right-hand sides c0 .. c0 +
+// kb, entries jb .. jb + nb), eight entries per thread.  Every load of the
+// window is issued before the first is used (one memory round trip; with a
+// fused reduction, one per partial slot), outside the window a clamped
+// in-range address whose value is masked.  Thread tid holds right-hand side
+// 2u + (tid >> 7), entry tid & 127 in xv[u].
+template <bool COH>
+__device__ inline void load_row_x(const VecBufs& b, const GvTask& t, int c0, int kb, int jb,
+                                  int nb, double (&xv)[8]) {
+  const int tid = body_tid();
+  const int jj = tid & (kRowChunk - 1), ch = tid >> 7;
+  if (t.xpart < 0) {
+    // 32-bit entry offsets from one uniform base (a vector column of the
+    // apply holds fewer than 2^31 / 16 entries)
+    const double* x0 = vsel(b, t.xsel, c0) + t.xoff + jb;
+    const int st = (int)vstride(b, t.xsel);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cc = 2 * u + ch;
+      const bool ok = cc < kb && jj < nb;
+      const double v = vld<COH>(x0 + (ok ? cc * st + jj : 0));
+      xv[u] = ok ? v : 0.0;
+    }
+    return;
+  }
+  // x[j] = sum over s < xnsub of the producing column task's partial slots,
+  // in slot order (the reduce task's order)
+  const double* p0 = b.part + (int64_t)c0 * b.ps + (int64_t)t.xpart * 32;
+  const int j = jb + jj;
+  int64_t off[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int cc = 2 * u + ch;
+    const bool ok = cc < kb && jj < nb;
+    off[u] = ok ? (int64_t)cc * b.ps + (int64_t)(j >> 5) * t.xnsub * 32 + (j & 31) : 0;
+    xv[u] = 0.0;
+  }
+  for (int s2 = 0; s2 < t.xnsub; ++s2) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = vld<COH>(p0 + off[u] + (int64_t)s2 * 32);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] += v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int cc = 2 * u + ch;
+    if (!(cc < kb && jj < nb)) xv[u] = 0.0;
+  }
+}
+
 template <typename T, bool COH>
 __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBufs& b, int k,
                           double* sm) {
   const T* __restrict__ M = F + t.mat;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int li = lane & 15, kq = lane >> 4;
+  // subtracting tasks read y's old values with their first loads (y is
+  // written by no other task while this one runs), so the epilogue only
+  // stores
+  const bool rmw = t.op && t.part < 0;
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
+    // lane quantities per pass (not hoisted out of the pass loop)
+    const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, kq = lane >> 4;
     const int kb = min(kMfmaRhs, k - c0);
+    // this lane's result column (clamped in range; masked at the stores)
+    double* const ycol =
+        t.part < 0 ? vsel(b, t.ysel, c0 + min(li, kb - 1)) + t.yoff : nullptr;
+    double yold[4] = {0, 0, 0, 0};
     if (t.kind == 0) {
       constexpr int LD = kRowChunk + 1;
       const int r = wave * 16 + li;
@@ -713,14 +805,21 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       for (int jb = 0; jb < t.cols; jb += kRowChunk) {
         const int nb = min(kRowChunk, t.cols - jb);
         // the chunk's matrix block is loaded a quarter ahead: the first
-        // quarter while x is staged, each next one while the previous is
-        // multiplied
+        // quarter with the x window (and, at the first chunk, the old y),
+        // each next one while the previous is multiplied
         T ma[8], mb[8];
         if (wact) load_q(jb, nb, ma);
-        for (int idx = tid; idx < kMfmaRhs * kRowChunk; idx += 256) {
-          const int cc = idx / kRowChunk, jj = idx - cc * kRowChunk;
-          sm[cc * LD + jj] = (cc < kb && jj < nb) ? x_entry<COH>(b, t, c0 + cc, jb + jj) : 0.0;
+        double xv[8];
+        load_row_x<COH>(b, t, c0, kb, jb, nb, xv);
+        if (jb == 0 && rmw && wact) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = wave * 16 + kq + 4 * v;
+            yold[v] = vld<COH>(ycol + (row < t.rows ? row : 0));
+          }
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sm[(2 * u + (tid >> 7)) * LD + (tid & 127)] = xv[u];
         __syncthreads();
         if (wact) {
           const double* xr = sm + li * LD + kq;
@@ -744,8 +843,7 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
               vst<COH>(b.part + (int64_t)(c0 + li) * b.ps +
                            (int64_t)(t.part + (row >> 5)) * 32 + (row & 31), val);
             } else {
-              double* y = vsel(b, t.ysel, c0 + li) + t.yoff + row;
-              vst<COH>(y, t.op ? vld<COH>(y) - val : val);
+              vst<COH>(ycol + row, rmw ? yold[v] - val : val);
             }
           }
         }
@@ -753,22 +851,55 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       double* Ms = sm;             // [64][33]: row-major transposed tile
       double* xs = sm + 64 * 33;   // [16][65]
       const int g = wave & 1, h = wave >> 1;
+      const double* x0 = vsel(b, t.xsel, c0) + t.xoff;
+      const int64_t xst = vstride(b, t.xsel);
       d4v acc = {0, 0, 0, 0};
-      for (int rb = 0; rb < t.rows; rb += 64) {
+      // a 64-row block: eight matrix entries and four x entries per thread,
+      // loaded one block ahead (in flight while the previous block is
+      // multiplied), stored to LDS after the barrier that frees it
+      T mv[8];
+      double xv[4];
+      auto load_blk = [&](int rb) {
         const int nr = min(64, t.rows - rb);
-        for (int idx = tid; idx < 64 * 32; idx += 256) {
-          const int rr = idx & 63, cc = idx >> 6;
-          Ms[rr * 33 + cc] = (rr < nr && cc < t.cols)
-                                 ? (double)M[(int64_t)(rb + rr) + (int64_t)cc * t.ld]
-                                 : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+          const bool ok = rr < nr && cc < t.cols;
+          mv[u] = M[ok ? (int64_t)(rb + rr) + (int64_t)cc * t.ld : 0];
         }
-        for (int idx = tid; idx < kMfmaRhs * 64; idx += 256) {
-          const int rr = idx & 63, cc = idx >> 6;
-          xs[cc * 65 + rr] = (rr < nr && cc < kb)
-                                 ? vld<COH>(vsel(b, t.xsel, c0 + cc) + t.xoff + rb + rr)
-                                 : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+          const bool ok = rr < nr && cc < kb;
+          const double v = vld<COH>(ok ? x0 + cc * xst + rb + rr : x0);
+          xv[u] = ok ? v : 0.0;
         }
+      };
+      auto store_blk = [&](int rb) {
+        const int nr = min(64, t.rows - rb);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+          Ms[rr * 33 + cc] = (rr < nr && cc < t.cols) ? (double)mv[u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+          xs[cc * 65 + rr] = xv[u];
+        }
+      };
+      load_blk(0);
+      if (rmw && h == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int col = 16 * g + kq + 4 * v;
+          yold[v] = vld<COH>(ycol + (col < t.cols ? col : 0));
+        }
+      }
+      for (int rb = 0; rb < t.rows; rb += 64) {
+        store_blk(rb);
         __syncthreads();
+        if (rb + 64 < t.rows) load_blk(rb + 64);
 #pragma unroll
         for (int q = 0; q < 32; q += 4) {
           const int rr = 32 * h + q + kq;
@@ -789,10 +920,8 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
           if (col < t.cols && li < kb) {
             if (t.part >= 0)
               vst<COH>(b.part + (int64_t)(c0 + li) * b.ps + (int64_t)t.part * 32 + col, val);
-            else {
-              double* y = vsel(b, t.ysel, c0 + li) + t.yoff + col;
-              vst<COH>(y, t.op ? vld<COH>(y) - val : val);
-            }
+            else
+              vst<COH>(ycol + col, rmw ? yold[v] - val : val);
           }
         }
       __syncthreads();
@@ -815,7 +944,7 @@ __device__ void gemv_lite(const T* __restrict__ F, const GvTask& t0, const VecBu
     else gemv_multi<T, COH, 4>(F, t, b, k, red, xl);
     return;
   }
-  const int tid = threadIdx.x;
+  const int tid = body_tid();
   // reduce: y[j] (op)= sum_s part[s][j] for every (right-hand side, j)
   // pair at once, eight partial loads in flight per item; the sum runs
   // over s in order, so the result does not depend on k
@@ -869,7 +998,7 @@ using SubtreeLds = SubtreeLdsK<kSubtreeRhs>;
 template <typename T, bool COH, int KS>
 __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                                   const VecBufs& b, int k, SubtreeLdsK<KS>& L) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   const int rootn = uni(a.nodes[q1 - 1]);
   // one right-hand side: every own entry b[perm[col]] of the subtree at once
@@ -1010,7 +1139,7 @@ __device__ void subtree_forward_g(const T* __restrict__ F, const SubtreeArgs& a,
 template <typename T, bool COH, int KS>
 __device__ void subtree_backward_g(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                                    const VecBufs& b, int k, SubtreeLdsK<KS>& L) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   // one right-hand side: z of every subtree column at once
   const int col_lo = uni(a.cols[2 * sb]), col_n = uni(a.cols[2 * sb + 1]);
@@ -1140,9 +1269,10 @@ static_assert(kMlMaxCols <= 128, "at most two 16-row blocks per wave");
 // out(i, r, v) for i < rows, r < kb: v = sum_j A(i, j) x(r, j), j < cols,
 // with A(i, j) = M[i + j ld] (TRANS false) or M[j + i ld] (TRANS true) and
 // x in LDS (right-hand side r at xs + r * xld).  Wave w takes the 16-row
-// blocks w, w + 4, ...; columns go in chunks of 16 whose four loads per lane
-// are issued together, the sum over j in groups of four in increasing order
-// for every right-hand side (so a column's bits do not depend on kb).
+// blocks w, w + 4, ..., two at a time; columns go in chunks of 32 whose
+// sixteen loads per lane are issued together, the sum over j in groups of
+// four in increasing order for every right-hand side (so a column's bits do
+// not depend on kb).
 //
 // The epilogue reads the four old values a lane needs (get(row, r), e.g.
 // from device memory) before writing any result (put(row, r, old, v)), so
@@ -1150,41 +1280,55 @@ static_assert(kMlMaxCols <= 128, "at most two 16-row blocks per wave");
 template <typename T, bool TRANS, typename Get, typename Put>
 __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
                            const double* xs, int xld, int kb, Get get, Put put) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid0 = body_tid(), lane = tid0 & 63, wave = tid0 >> 6;
   const int li = lane & 15, kq = lane >> 4;
-  for (int i0 = wave * 16; i0 < rows; i0 += 64) {
-    const int i = i0 + li;
-    const bool iok = i < rows;
-    d4v acc = {0, 0, 0, 0};
-    for (int j0 = 0; j0 < cols; j0 += 16) {
-      T m[4];
+  // two 16-row blocks per wave and pass (i0 and i0 + 64), columns in chunks
+  // of 32: the sixteen matrix loads of a chunk go out together
+  for (int i0 = wave * 16; i0 < rows; i0 += 128) {
+    const bool two = i0 + 64 < rows;
+    d4v acc[2];
+    acc[0] = d4v{0, 0, 0, 0};
+    acc[1] = d4v{0, 0, 0, 0};
+    for (int j0 = 0; j0 < cols; j0 += 32) {
+      T m[2][8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = j0 + 4 * u + kq;
-        const bool ok = iok && j < cols;
-        const int ii = ok ? i : 0, jj = ok ? j : 0;
-        const T v = TRANS ? M[jj + (int64_t)ii * ld] : M[ii + (int64_t)jj * ld];
-        m[u] = ok ? v : T(0);
+      for (int h = 0; h < 2; ++h) {
+        const int i = i0 + 64 * h + li;
+        const bool iok = i < rows;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int j = j0 + 4 * u + kq;
+          const bool ok = iok && j < cols;
+          const int ii = ok ? i : 0, jj = ok ? j : 0;
+          const T v = TRANS ? M[jj + (int64_t)ii * ld] : M[ii + (int64_t)jj * ld];
+          m[h][u] = ok ? v : T(0);
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         if (j0 + 4 * u >= cols) break;
         const int j = j0 + 4 * u + kq;
         const double xv = (j < cols && li < kb) ? xs[li * xld + j] : 0.0;
-        acc = mfma64((double)m[u], xv, acc);
+        acc[0] = mfma64((double)m[0][u], xv, acc[0]);
+        if (two) acc[1] = mfma64((double)m[1][u], xv, acc[1]);
       }
     }
-    double old[4];
+    // every old value of both blocks is read before any result is written
+    double old[2][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = i0 + kq + 4 * v;
-      old[v] = (row < rows && li < kb) ? get(row, li) : 0.0;
-    }
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = i0 + kq + 4 * v;
-      if (row < rows && li < kb) put(row, li, old[v], acc[v]);
-    }
+      for (int v = 0; v < 4; ++v) {
+        const int row = i0 + 64 * h + kq + 4 * v;
+        old[h][v] = (row < rows && li < kb) ? get(row, li) : 0.0;
+      }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = i0 + 64 * h + kq + 4 * v;
+        if (row < rows && li < kb) put(row, li, old[h][v], acc[h][v]);
+      }
   }
 }
 
@@ -1198,7 +1342,7 @@ __device__ inline double ms_none(int, int) { return 0.0; }
 // instead of one per item.
 template <int NB, typename Index, typename Load, typename Use>
 __device__ inline void ms_batched(int total, Index index, Load load, Use use) {
-  for (int base = threadIdx.x; base < total; base += 256 * NB) {
+  for (int base = body_tid(); base < total; base += 256 * NB) {
     int ix[NB];
     double v[NB];
 #pragma unroll
@@ -1225,7 +1369,7 @@ template <typename T>
 __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
                                  const SubtreeArgs& a, const VecBufs& b, int c0, int kb,
                                  bool root, double* lds) {
-  const int tid = threadIdx.x;
+  const int tid = body_tid();
   const int c = S.ncols, R = S.R, ld = c + R;
   double* Fc = lds;                      // [16][kMlFld] fv_c
   double* Yl = lds + kMfmaRhs * kMlFld;  // [16][kMlFld] y
@@ -1302,7 +1446,7 @@ template <typename T>
 __device__ void ms_backward_large(const T* __restrict__ Zp, const VecNode& S,
                                   const SubtreeArgs& a, const VecBufs& b, int c0, int kb,
                                   double* lds) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = body_tid(), lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, kq = lane >> 4;
   const int c = S.ncols, R = S.R, ld = c + R;
   constexpr int XL = kMlChunk + 1;
@@ -1368,7 +1512,6 @@ __device__ void ms_backward_large(const T* __restrict__ Zp, const VecNode& S,
 template <typename T>
 __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                            const VecBufs& b, int k, double* lds) {
-  const int tid = threadIdx.x;
   double* Fr = lds;                       // [16][kMsFld] front vector
   double* Yr = lds + kMfmaRhs * kMsFld;   // [16][kMsYld] y
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
@@ -1376,6 +1519,7 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
     for (int qi = q0; qi < q1; ++qi) {
+      const int tid = body_tid();  // per supernode (not hoisted out of the loop)
       const int j = uni(a.nodes[qi]);
       const VecNode S = uniform_node(a.vn[j]);
       const int64_t P = uni64(a.panel[j]);
@@ -1433,12 +1577,12 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
 template <typename T>
 __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                             const VecBufs& b, int k, double* lds) {
-  const int tid = threadIdx.x;
   double* Fr = lds;  // [16][kMsFld]: x_R at positions c .. c + R
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
     for (int qi = q1 - 1; qi >= q0; --qi) {
+      const int tid = body_tid();  // per supernode (not hoisted out of the loop)
       const int j = uni(a.nodes[qi]);
       const VecNode S = uniform_node(a.vn[j]);
       const int64_t P = uni64(a.panel[j]);
@@ -1535,7 +1679,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
   __shared__ int s_ticket;
   __shared__ int s_ok;
   __shared__ unsigned long long s_tgrab, s_tready;
-  const int tid = threadIdx.x;
+  const int tid = body_tid();
   if (b.live && uni(*b.live) == 0) return;
   if (tid == 0) {
     s_ok = 1;

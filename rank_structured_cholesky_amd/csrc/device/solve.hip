@@ -723,44 +723,46 @@ template <bool COH>
 __device__ inline void load_row_x(const VecBufs& b, const GvTask& t, int c0, int kb, int jb,
                                   int nb, double (&xv)[8]) {
   const int tid = body_tid();
-  const int jj = tid & (kRowChunk - 1), ch = tid >> 7;
+  const int jj = tid & (kRowChunk - 1);
+  const int ch = uni(tid >> 7);  // the same for every lane of a wave
+  const bool jok = jj < nb;
   if (t.xpart < 0) {
-    // 32-bit entry offsets from one uniform base (a vector column of the
-    // apply holds fewer than 2^31 / 16 entries)
+    // per load a wave-uniform column start (scalar registers) and the
+    // lane's entry as a small offset
     const double* x0 = vsel(b, t.xsel, c0) + t.xoff + jb;
-    const int st = (int)vstride(b, t.xsel);
+    const int64_t st = vstride(b, t.xsel);
+    const unsigned jo = jok ? (unsigned)jj : 0u;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int cc = 2 * u + ch;
-      const bool ok = cc < kb && jj < nb;
-      const double v = vld<COH>(x0 + (ok ? cc * st + jj : 0));
-      xv[u] = ok ? v : 0.0;
+      const double* xc = cc < kb ? x0 + cc * st : x0;
+      const double v = vld<COH>(xc + jo);
+      xv[u] = (cc < kb && jok) ? v : 0.0;
     }
     return;
   }
   // x[j] = sum over s < xnsub of the producing column task's partial slots,
   // in slot order (the reduce task's order)
   const double* p0 = b.part + (int64_t)c0 * b.ps + (int64_t)t.xpart * 32;
-  const int j = jb + jj;
-  int64_t off[8];
+  const int j = jb + (jok ? jj : 0);
+  const unsigned lo = (unsigned)((j >> 5) * t.xnsub * 32 + (j & 31));
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int cc = 2 * u + ch;
-    const bool ok = cc < kb && jj < nb;
-    off[u] = ok ? (int64_t)cc * b.ps + (int64_t)(j >> 5) * t.xnsub * 32 + (j & 31) : 0;
-    xv[u] = 0.0;
-  }
+  for (int u = 0; u < 8; ++u) xv[u] = 0.0;
   for (int s2 = 0; s2 < t.xnsub; ++s2) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = vld<COH>(p0 + off[u] + (int64_t)s2 * 32);
+    for (int u = 0; u < 8; ++u) {
+      const int cc = 2 * u + ch;
+      const double* pc = p0 + (int64_t)(cc < kb ? cc : 0) * b.ps;
+      v[u] = vld<COH>(pc + (lo + (unsigned)s2 * 32u));
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) xv[u] += v[u];
   }
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int cc = 2 * u + ch;
-    if (!(cc < kb && jj < nb)) xv[u] = 0.0;
+    if (!(cc < kb && jok)) xv[u] = 0.0;
   }
 }
 
@@ -786,16 +788,19 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       const int r = wave * 16 + li;
       const bool rok = r < t.rows;
       const bool wact = wave * 16 < t.rows;
-      const T* __restrict__ mrow = M + (rok ? r : 0);
       d4v acc = {0, 0, 0, 0};
-      // columns jb + 4u + kq of this lane's row, u < 8 (a quarter chunk);
-      // the lane's part of the address is fixed, the column offset uniform
-      const T* __restrict__ mlane = mrow + (int64_t)kq * t.ld;
+      // columns jb + 4u + kq of this lane's row, u < 8 (a quarter chunk)
+      // byte offsets from the block's (uniform) start as 32-bit values, one
+      // register per load: a task's block spans less than 4 GiB
+      const unsigned rlane = ((unsigned)(rok ? r : 0) + (unsigned)kq * (unsigned)t.ld) *
+                             (unsigned)sizeof(T);
+      const unsigned cstep = (unsigned)t.ld * (unsigned)sizeof(T);
+      const char* const Mb = reinterpret_cast<const char*>(M);
       auto load_q = [&](int jb, int nb, T (&m)[8]) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          m[u] = mlane[(int64_t)(jb + min(4 * u, nb - 1 - kq)) * t.ld];  // past the
-                                     // end: an in-range column times x = 0
+        for (int u = 0; u < 8; ++u)  // past the end: an in-range column times x = 0
+          m[u] = *reinterpret_cast<const T*>(
+              Mb + (rlane + (unsigned)(jb + min(4 * u, nb - 1 - kq)) * cstep));
       };
       // one accumulator, columns in increasing groups of four
       auto mul_q = [&](const T (&m)[8], const double* xr) {
@@ -804,11 +809,18 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       };
       for (int jb = 0; jb < t.cols; jb += kRowChunk) {
         const int nb = min(kRowChunk, t.cols - jb);
-        // the chunk's matrix block is loaded a quarter ahead: the first
-        // quarter with the x window (and, at the first chunk, the old y),
-        // each next one while the previous is multiplied
-        T ma[8], mb[8];
-        if (wact) load_q(jb, nb, ma);
+        // the chunk's matrix block goes out with the x window (and, at the
+        // first chunk, the old y): all four quarters for an FP32 factor,
+        // the first two for FP64 (the next two while those are multiplied);
+        // quarters past the chunk's end are skipped (their x is zero)
+        constexpr bool kAll = sizeof(T) == 4;
+        T m0[8], m1[8], m2[8], m3[8];
+        if (wact) {
+          load_q(jb, nb, m0);
+          if (nb > 32) load_q(jb + 32, nb - 32, m1);
+          if (kAll && nb > 64) load_q(jb + 64, nb - 64, m2);
+          if (kAll && nb > 96) load_q(jb + 96, nb - 96, m3);
+        }
         double xv[8];
         load_row_x<COH>(b, t, c0, kb, jb, nb, xv);
         if (jb == 0 && rmw && wact) {
@@ -823,13 +835,19 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
         __syncthreads();
         if (wact) {
           const double* xr = sm + li * LD + kq;
-          load_q(jb + 32, nb - 32, mb);
-          mul_q(ma, xr);
-          load_q(jb + 64, nb - 64, ma);
-          mul_q(mb, xr + 32);
-          load_q(jb + 96, nb - 96, mb);
-          mul_q(ma, xr + 64);
-          mul_q(mb, xr + 96);
+          if constexpr (kAll) {
+            mul_q(m0, xr);
+            if (nb > 32) mul_q(m1, xr + 32);
+            if (nb > 64) mul_q(m2, xr + 64);
+            if (nb > 96) mul_q(m3, xr + 96);
+          } else {
+            mul_q(m0, xr);
+            if (nb > 64) load_q(jb + 64, nb - 64, m0);
+            if (nb > 32) mul_q(m1, xr + 32);
+            if (nb > 96) load_q(jb + 96, nb - 96, m1);
+            if (nb > 64) mul_q(m0, xr + 64);
+            if (nb > 96) mul_q(m1, xr + 96);
+          }
         }
         __syncthreads();
       }
@@ -859,20 +877,24 @@ __device__ void gemv_mfma(const T* __restrict__ F, const GvTask& t, const VecBuf
       // multiplied), stored to LDS after the barrier that frees it
       T mv[8];
       double xv[4];
+      // entry idx = tid + 256 u of a block: row lane, column wave + 4 u (the
+      // same for every lane of a wave: a scalar column start per load)
+      const int wv = uni(wave);
       auto load_blk = [&](int rb) {
         const int nr = min(64, t.rows - rb);
+        const unsigned ro = (unsigned)(lane < nr ? rb + lane : rb);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
-          const bool ok = rr < nr && cc < t.cols;
-          mv[u] = M[ok ? (int64_t)(rb + rr) + (int64_t)cc * t.ld : 0];
+          const int cc = wv + 4 * u;
+          const T* mc = M + (int64_t)(cc < t.cols ? cc : 0) * t.ld;
+          mv[u] = mc[ro];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
-          const bool ok = rr < nr && cc < kb;
-          const double v = vld<COH>(ok ? x0 + cc * xst + rb + rr : x0);
-          xv[u] = ok ? v : 0.0;
+          const int cc = wv + 4 * u;
+          const double* xc = x0 + (cc < kb ? cc : 0) * xst;
+          const double v = vld<COH>(xc + ro);
+          xv[u] = (lane < nr && cc < kb) ? v : 0.0;
         }
       };
       auto store_blk = [&](int rb) {

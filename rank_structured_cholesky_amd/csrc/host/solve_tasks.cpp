@@ -302,6 +302,13 @@ struct TaskEmitter {
     CHUNK = chunk0;
     row_split = split0;
   }
+  // The matrix-core ROW body addresses its block with 32-bit byte offsets
+  // from the block's start (8-byte elements at most).
+  static void check_row_span(const dev::GvTask& t) {
+    const int64_t span = ((int64_t)(t.cols - 1) * t.ld + t.rows) * 8;
+    if (span >= (int64_t(1) << 32))
+      throw std::runtime_error("solve task: a row block spans 4 GiB or more");
+  }
   void emit_split(const Mv& m, std::vector<dev::GvTask>& main, std::vector<dev::GvTask>& red) {
     if (m.col) {
       const bool fuse = fuse_tv && m.ys == TV && m.op == 0 && !m.upper;
@@ -379,6 +386,7 @@ struct TaskEmitter {
         t.xoff = m.xo + (m.upper ? i0 : 0);
         t.yoff = m.yo + i0;
         if (row_split <= 0 || t.cols <= row_split) {
+          check_row_span(t);
           main.push_back(t);
           continue;
         }
@@ -394,6 +402,7 @@ struct TaskEmitter {
           u.xoff = t.xoff + c0;
           if (xpart >= 0) u.xpart = xpart + (c0 / 32) * xnsub;
           u.part = (int)(slot + 2 * s2);
+          check_row_span(u);
           main.push_back(u);
         }
         for (int h = 0; h < 2; ++h) {

@@ -105,11 +105,16 @@ inline bool inside_subtree(const Supernode& s, int q, int sub) {
 // One dataflow task list on the device.
 struct DfList {
   DBuf tasks, gv, gw, cnt;
+  // ready-queue dispatch: queue slots, pop/push counters, pending waits per
+  // group and their initial values, tasks per group, successor groups
+  DBuf q_queue, q_ctr, q_pend, q_pend0, q_need, q_gptr, q_gtasks, q_sptr, q_succ, q_init;
+  int q_n0 = 0;
   int ntasks = 0, ngroups = 0, grid = 1;
   // kept for the optional timing trace
   std::vector<dev::DfTask> host;
   std::vector<dev::DfWait> host_wait;
   std::vector<dev::GvTask> host_gv;
+  std::vector<float> host_feat;
 };
 
 template <typename T>
@@ -241,6 +246,7 @@ class Engine final : public FactorEngine {
 
   unsigned long long df_timeout_ = 0;
   bool use_dataflow_ = false;
+  bool df_queue_ = true;  // ready-queue dispatch (RSC_DF_QUEUE=0: tickets)
   int64_t fv_stride_ = 0;
   int fv_k_ = 0;
   std::vector<std::vector<int>> levnode_off_;  // per level: offset in dlevnodes
@@ -1420,11 +1426,25 @@ void Engine<T>::finish_solve_setup() {
     for (size_t q = 0; q < DT.gw.size(); ++q) waits[q] = {DT.gw[q], DT.need[DT.gw[q]]};
     D.gw.upload(waits, st_);
     D.cnt.alloc((size_t)(D.ngroups + 1) * sizeof(unsigned));
+    if (df_queue_) {
+      D.q_queue.alloc((size_t)std::max(1, D.ntasks) * sizeof(int));
+      D.q_ctr.alloc(2 * sizeof(unsigned));
+      D.q_pend.alloc((size_t)std::max(1, D.ngroups) * sizeof(int));
+      D.q_pend0.upload(DT.q_pend0, st_);
+      D.q_need.upload(DT.need, st_);
+      D.q_gptr.upload(DT.q_gptr, st_);
+      D.q_gtasks.upload(DT.q_gtasks, st_);
+      D.q_sptr.upload(DT.q_sptr, st_);
+      D.q_succ.upload(DT.q_succ.empty() ? std::vector<int>{0} : DT.q_succ, st_);
+      D.q_init.upload(DT.q_init.empty() ? std::vector<int>{0} : DT.q_init, st_);
+      D.q_n0 = (int)DT.q_init.size();
+    }
     D.grid = std::max(1, std::min(D.ntasks, df_resident_[q]));
     if (!df_trace_path_.empty()) {
       D.host = DT.tasks;
       D.host_wait = waits;
       D.host_gv = DT.gv;
+      D.host_feat = DT.trace_feat;
     }
     slots = std::max(slots, DT.slots);
     maxtasks = std::max<int64_t>(maxtasks, D.ntasks);
@@ -1531,6 +1551,10 @@ void Engine<T>::setup_static() {
     // level-scheduled launches instead
     const char* lv = std::getenv("RSC_APPLY_LEVELS");
     use_dataflow_ = !(lv && lv[0] == '1');
+    // workgroups take ready tasks from a queue; RSC_DF_QUEUE=0 hands out
+    // tickets in list-schedule order instead
+    const char* qv = std::getenv("RSC_DF_QUEUE");
+    df_queue_ = !(qv && qv[0] == '0');
   }
   if (use_dataflow_) {
     int dev = 0, cus = 1;
@@ -2028,7 +2052,21 @@ void Engine<T>::run_apply(int k, const double* B, int ldb, double* X, int ldx,
     a.ldx = ldx;
     a.trace = dtrace_.p ? dtrace_.as<unsigned long long>() : nullptr;
     a.sub = subtree_args(B, ldb, df_last_ == 1);
-    dev::launch_df_reset(D.cnt.as<unsigned>(), D.ngroups + 1, s);
+    if (df_queue_) {
+      a.q_queue = D.q_queue.as<int>();
+      a.q_ctr = D.q_ctr.as<unsigned>();
+      a.q_pend = D.q_pend.as<int>();
+      a.q_need = D.q_need.as<unsigned>();
+      a.q_gptr = D.q_gptr.as<int>();
+      a.q_gtasks = D.q_gtasks.as<int>();
+      a.q_sptr = D.q_sptr.as<int>();
+      a.q_succ = D.q_succ.as<int>();
+      dev::launch_df_reset_queue(D.cnt.as<unsigned>(), D.q_pend.as<int>(), D.q_pend0.as<int>(),
+                                 D.ngroups, D.q_queue.as<int>(), D.q_init.as<int>(), D.q_n0,
+                                 D.ntasks, D.q_ctr.as<unsigned>(), s);
+    } else {
+      dev::launch_df_reset(D.cnt.as<unsigned>(), D.ngroups + 1, s);
+    }
     dev::launch_apply_dataflow<T>(F(), a, vb, k, D.grid, s);
     return;
   }
@@ -2080,7 +2118,8 @@ void Engine<T>::check_apply(hipStream_t s) {
   raise_apply_status(st);
   if (dtrace_.p) {
     // timing trace of the last dataflow apply: ntasks, tasks, 4 clocks each,
-    // then the wait-list length, the wait list and the matrix-vector entries
+    // then the wait-list length, the wait list, the matrix-vector entries and
+    // four cost-model values per task
     const DfList& DL = df_[df_last_];
     const std::vector<dev::DfTask>& df_host_ = DL.host;
     std::vector<unsigned long long> tr(4 * df_host_.size());
@@ -2095,6 +2134,7 @@ void Engine<T>::check_apply(hipStream_t s) {
       std::fwrite(&nw, sizeof(nw), 1, fp);
       std::fwrite(DL.host_wait.data(), sizeof(dev::DfWait), DL.host_wait.size(), fp);
       std::fwrite(DL.host_gv.data(), sizeof(dev::GvTask), DL.host_gv.size(), fp);
+      std::fwrite(DL.host_feat.data(), sizeof(float), DL.host_feat.size(), fp);
       std::fclose(fp);
     }
   }

@@ -84,10 +84,26 @@ struct DfArgs {
   int64_t ldx;
   unsigned long long* trace;  // optional: per ticket grab / ready / done clock, workgroup
   SubtreeArgs sub;
+  // Ready-queue dispatch (when q_queue is set) instead of tickets: a
+  // workgroup pops the next queue slot and runs the task written there; the
+  // task completing a group's last wait pushes that group's tasks.
+  int* q_queue;          // ntasks slots, -1 until written
+  unsigned* q_ctr;       // [0] next slot to pop, [1] next slot to push
+  int* q_pend;           // per group: waits not yet complete
+  const unsigned* q_need;  // per group: tasks signalling it
+  const int* q_gptr;     // tasks of group g: q_gtasks[q_gptr[g] .. q_gptr[g + 1])
+  const int* q_gtasks;
+  const int* q_sptr;     // groups waiting on g: q_succ[q_sptr[g] .. q_sptr[g + 1])
+  const int* q_succ;
 };
 
 // Zero the counters (ngroups + 1 entries) before a launch.
 void launch_df_reset(unsigned* cnt, int count, hipStream_t s);
+// Ready-queue state before a launch: counters zero, pend = pend0, the first
+// n0 slots the initial tasks, the others empty (-1), pop 0 / push n0.
+void launch_df_reset_queue(unsigned* cnt, int* pend, const int* pend0, int ngroups,
+                           int* queue, const int* init, int n0, int ntasks, unsigned* ctr,
+                           hipStream_t s);
 template <typename T>
 void launch_apply_dataflow(const T* factor, const DfArgs& a, VecBufs b, int k, int grid,
                            hipStream_t s);

@@ -1683,6 +1683,224 @@ struct DfLds<true> {
   using type = DfLdsMulti;
 };
 
+// The body of task `ticket` (every thread of the workgroup).
+template <typename T, bool MULTI>
+__device__ inline void df_run_task(const T* __restrict__ F, const DfArgs& a, const VecBufs& b,
+                                   int k, int ticket, double (*red)[64],
+                                   typename DfLds<MULTI>::type& lds) {
+  const DfTask& tk = a.tasks[ticket];
+  const GvTask gt = uniform_task(a.gv[ticket]);
+  const int type = uni(tk.type), ta = uni(tk.a), tb = uni(tk.b);
+  const int tid = body_tid();
+  if (type == 0) {
+    // the chunk is summed in LDS (one right-hand side: the first
+    // kSingleChunkGroup * VCH entries of the x staging area), then stored
+    // once; a task of the single-right-hand-side list spans
+    // kSingleChunkGroup chunks
+    if constexpr (MULTI) {
+      assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
+                           a.ranges, b, k, a.B, a.ldb, a.perm, lds.mf);
+    } else {
+      assemble_chunk<true, kSingleChunkGroup>(uniform_node(a.nodes[ta]), tb, a.nodes,
+                                              a.children, a.relmap, a.ranges, b, 1, a.B,
+                                              a.ldb, a.perm, lds.xl);
+    }
+  } else if (type == 1) {
+    if constexpr (MULTI) {
+      gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
+    } else {
+      gather_chunk<true, false, kSingleChunkGroup>(uniform_node(a.nodes[ta]), tb, a.rows, b,
+                                                   k);
+    }
+  } else if (type == 2) {
+    if constexpr (MULTI) gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.mf);
+    else gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.xl);
+  } else if (type == 4 || type == 5) {
+    if constexpr (MULTI) {
+      if (type == 4) ms_forward<T>(F, a.sub, ta, b, k, lds.mf);
+      else ms_backward<T>(F, a.sub, ta, b, k, lds.mf);
+    } else if (type == 4) {
+      subtree_forward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
+    } else {
+      subtree_backward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
+    }
+  } else {
+    // x = P^T w over a chunk of x in its own order, x[j] = w[pinv[j]]: the
+    // stores are coalesced (a scattered 8-byte store costs a memory line);
+    // eight source indices of a thread, then per right-hand side their
+    // eight entries, are loaded together
+    const int n = (int)b.n;
+    for (int h = 0; h < DF_SCATTER_CHUNK; h += 8 * 256) {
+      const int j0 = ta * DF_SCATTER_CHUNK + h + tid;
+      int src[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) src[u] = j0 + 256 * u < n ? a.pinv[j0 + 256 * u] : 0;
+      for (int col = 0; col < k; ++col) {
+        const double* wc = b.w + (int64_t)col * b.n;
+        double* xc = a.X + (int64_t)col * a.ldx + j0;
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = vld<true>(wc + src[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (j0 + 256 * u < n) xc[256 * u] = v[u];
+      }
+    }
+  }
+}
+
+// Queue slot values besides task indices: not yet written, and a slot whose
+// task the workgroup that made it ready runs itself.
+constexpr int kQEmpty = -1;
+constexpr int kQSkip = -3;
+constexpr int kQAbort = -4;  // returned when the launch is aborted
+
+// Thread 0 spins until queue slot idx is written (or the launch is
+// aborted).  Same bound and back-off as a dependency wait.
+__device__ int df_pop_wait(const DfArgs& a, int idx) {
+  int t = __hip_atomic_load(a.q_queue + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t != kQEmpty) return t;
+  const unsigned long long t0 = wall_clock64();
+  int backoff = 1;
+  for (;;) {
+    t = __hip_atomic_load(a.q_queue + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t != kQEmpty) return t;
+    if (__hip_atomic_load(a.status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return kQAbort;
+    if (wall_clock64() - t0 > a.timeout_ticks) {
+      __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return kQAbort;
+    }
+    for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(4);
+    if (backoff < a.backoff_max) backoff *= 2;
+  }
+}
+
+// Groups made ready by one task's completion whose tasks the whole
+// workgroup pushes (more are pushed by thread 0 alone).
+constexpr int kQPush = 16;
+// Ready-queue loop: pop a slot, wait until it is written, run the task,
+// signal its groups.  A group whose last wait completes has its tasks
+// pushed (slots reserved at the push counter, written by the workgroup);
+// the first such group (successors go most critical first) gives its first
+// task to this workgroup as its next task, and its slot holds kQSkip.
+// Every slot is written exactly once and a task is popped or continued only
+// when all its waits are complete, so no workgroup holds a task that cannot
+// run.
+template <typename T, bool MULTI>
+__device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const VecBufs& b,
+                              int k, double (*red)[64], typename DfLds<MULTI>::type& lds) {
+  __shared__ int s_task, s_next, s_np;
+  __shared__ int s_pg[kQPush], s_pb[kQPush], s_pf[kQPush];
+  __shared__ unsigned long long s_tgrab, s_tready;
+  const int tid = body_tid();
+  if (tid == 0) s_next = -1;
+  for (;;) {
+    if (tid == 0) {
+      if (a.trace) s_tgrab = wall_clock64();
+      int t = s_next;
+      s_next = -1;
+      while (t < 0) {
+        const int idx = (int)__hip_atomic_fetch_add(a.q_ctr, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (idx >= a.ntasks) {
+          t = -2;  // every slot popped: done
+          break;
+        }
+        t = df_pop_wait(a, idx);
+        if (t == kQAbort) break;
+        if (t == kQSkip) t = -1;  // pop again
+      }
+      s_task = t;
+      if (a.trace) s_tready = wall_clock64();
+    }
+    __syncthreads();
+    const int ticket = uni(s_task);
+    if (ticket < 0) break;
+    df_run_task<T, MULTI>(F, a, b, k, ticket, red, lds);
+    // every thread: its device-scope stores have completed before the count
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (tid == 0) {
+      // both completion counts go out together; only the task completing a
+      // group does more (its successors' pending counts, then push slots
+      // for those that became ready)
+      const DfTask& tk = a.tasks[ticket];
+      const int g1 = tk.group, g2 = tk.group2;
+      const unsigned n1 = a.q_need[g1], n2 = g2 >= 0 ? a.q_need[g2] : 0u;
+      const unsigned o1 =
+          __hip_atomic_fetch_add(a.cnt + g1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned o2 =
+          g2 >= 0 ? __hip_atomic_fetch_add(a.cnt + g2, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)
+                  : 0u;
+      int np = 0, next = -1;
+      const int gs[2] = {o1 + 1u == n1 ? g1 : -1, (g2 >= 0 && o2 + 1u == n2) ? g2 : -1};
+      for (int e = 0; e < 2; ++e) {
+        const int g = gs[e];
+        if (g < 0) continue;
+        const int q0 = a.q_sptr[g], q1 = a.q_sptr[g + 1];
+        for (int qb = q0; qb < q1; qb += 8) {
+          // up to eight successors' pending counts at once
+          int hs[8], ps[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) hs[u] = qb + u < q1 ? a.q_succ[qb + u] : -1;
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            ps[u] = hs[u] >= 0 ? __hip_atomic_fetch_add(a.q_pend + hs[u], -1, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                               : 0;
+          for (int u = 0; u < 8; ++u) {
+            if (hs[u] < 0 || ps[u] != 1) continue;
+            const int h = hs[u];
+            const int c0 = a.q_gptr[h], cn = a.q_gptr[h + 1] - c0;
+            if (cn <= 0) continue;  // a group signalled by other groups' tasks
+            const int base = (int)__hip_atomic_fetch_add(a.q_ctr + 1, (unsigned)cn,
+                                                         __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            int first = 0;
+            if (next < 0) {
+              next = a.q_gtasks[c0];
+              first = 1;
+            }
+            if (np < kQPush) {
+              s_pg[np] = h;
+              s_pb[np] = base;
+              s_pf[np] = first;
+              ++np;
+            } else {
+              for (int i = 0; i < cn; ++i)
+                __hip_atomic_store(a.q_queue + base + i,
+                                   i < first ? kQSkip : a.q_gtasks[c0 + i], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+          }
+        }
+      }
+      s_np = np;
+      s_next = next;
+      if (a.trace) {
+        unsigned long long* tr = a.trace + 4 * (int64_t)ticket;
+        tr[0] = s_tgrab;
+        tr[1] = s_tready;
+        tr[2] = wall_clock64();
+        tr[3] = blockIdx.x;
+      }
+    }
+    __syncthreads();
+    const int np = uni(s_np);
+    for (int r = 0; r < np; ++r) {
+      const int h = uni(s_pg[r]), base = uni(s_pb[r]), first = uni(s_pf[r]);
+      const int c0 = uni(a.q_gptr[h]), cn = uni(a.q_gptr[h + 1]) - c0;
+      for (int i = tid; i < cn; i += 256)
+        __hip_atomic_store(a.q_queue + base + i, i < first ? kQSkip : a.q_gtasks[c0 + i],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // One tid == 0 region per loop iteration, with a barrier on both sides: the
 // completion signal and the next ticket are taken together, so the compiler
 // cannot merge a tid == 0 region of one iteration with one of the next (a
@@ -1701,8 +1919,12 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
   __shared__ int s_ticket;
   __shared__ int s_ok;
   __shared__ unsigned long long s_tgrab, s_tready;
-  const int tid = body_tid();
+  const int tid = threadIdx.x;
   if (b.live && uni(*b.live) == 0) return;
+  if (a.q_queue) {
+    df_queue_loop<T, MULTI>(F, a, b, k, red, lds);
+    return;
+  }
   if (tid == 0) {
     s_ok = 1;
     s_ticket = (int)__hip_atomic_fetch_add(a.cnt + a.ngroups, 1u, __ATOMIC_RELAXED,
@@ -1714,8 +1936,6 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     const int ticket = uni(s_ticket);
     if (ticket >= a.ntasks) break;
     const DfTask& tk = a.tasks[ticket];
-    const GvTask gt = uniform_task(a.gv[ticket]);
-    const int type = uni(tk.type), ta = uni(tk.a), tb = uni(tk.b);
     const int group = uni(tk.group), group2 = uni(tk.group2);
     const int w0 = uni(tk.wbeg), w1 = uni(tk.wend);
     if (w1 > w0) {
@@ -1733,61 +1953,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
       __syncthreads();
     }
     if (a.trace && tid == 0) s_tready = wall_clock64();
-    if (type == 0) {
-      // the chunk is summed in LDS (one right-hand side: the first
-      // kSingleChunkGroup * VCH entries of the x staging area), then stored
-      // once; a task of the single-right-hand-side list spans
-      // kSingleChunkGroup chunks
-      if constexpr (MULTI) {
-        assemble_chunk<true>(uniform_node(a.nodes[ta]), tb, a.nodes, a.children, a.relmap,
-                             a.ranges, b, k, a.B, a.ldb, a.perm, lds.mf);
-      } else {
-        assemble_chunk<true, kSingleChunkGroup>(uniform_node(a.nodes[ta]), tb, a.nodes,
-                                                a.children, a.relmap, a.ranges, b, 1, a.B,
-                                                a.ldb, a.perm, lds.xl);
-      }
-    } else if (type == 1) {
-      if constexpr (MULTI) {
-        gather_chunk<true, MULTI>(uniform_node(a.nodes[ta]), tb, a.rows, b, k);
-      } else {
-        gather_chunk<true, false, kSingleChunkGroup>(uniform_node(a.nodes[ta]), tb, a.rows, b,
-                                                     k);
-      }
-    } else if (type == 2) {
-      if constexpr (MULTI) gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.mf);
-      else gemv_lite<T, true, MULTI>(F, gt, b, k, red, lds.xl);
-    } else if (type == 4 || type == 5) {
-      if constexpr (MULTI) {
-        if (type == 4) ms_forward<T>(F, a.sub, ta, b, k, lds.mf);
-        else ms_backward<T>(F, a.sub, ta, b, k, lds.mf);
-      } else if (type == 4) {
-        subtree_forward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
-      } else {
-        subtree_backward_g<T, true, 1>(F, a.sub, ta, b, 1, lds.sl);
-      }
-    } else {
-      // x = P^T w over a chunk of x in its own order, x[j] = w[pinv[j]]: the
-      // stores are coalesced (a scattered 8-byte store costs a memory line);
-      // eight source indices of a thread, then per right-hand side their
-      // eight entries, are loaded together
-      const int n = (int)b.n;
-      for (int h = 0; h < DF_SCATTER_CHUNK; h += 8 * 256) {
-        const int j0 = ta * DF_SCATTER_CHUNK + h + tid;
-        int src[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) src[u] = j0 + 256 * u < n ? a.pinv[j0 + 256 * u] : 0;
-        for (int col = 0; col < k; ++col) {
-          const double* wc = b.w + (int64_t)col * b.n;
-          double* xc = a.X + (int64_t)col * a.ldx + j0;
-          double v[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) v[u] = vld<true>(wc + src[u]);
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (j0 + 256 * u < n) xc[256 * u] = v[u];
-        }
-      }
-    }
+    df_run_task<T, MULTI>(F, a, b, k, ticket, red, lds);
     // every thread: its device-scope stores have completed before the count
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1809,6 +1975,32 @@ __global__ __launch_bounds__(256, 4) void k_apply_df(const T* __restrict__ F, Df
     }
     __syncthreads();
   }
+}
+
+__global__ void k_df_reset_queue(unsigned* cnt, int* pend, const int* pend0, int ngroups,
+                                 int* queue, const int* init, int n0, int ntasks,
+                                 unsigned* ctr) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ngroups || i < ntasks; i += stride) {
+    if (i < ngroups) {
+      cnt[i] = 0u;
+      pend[i] = pend0[i];
+    }
+    if (i < ntasks) queue[i] = i < n0 ? init[i] : -1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr[0] = 0u;
+    ctr[1] = (unsigned)n0;
+  }
+}
+
+void launch_df_reset_queue(unsigned* cnt, int* pend, const int* pend0, int ngroups,
+                           int* queue, const int* init, int n0, int ntasks, unsigned* ctr,
+                           hipStream_t s) {
+  const int m = std::max(ngroups, ntasks);
+  const int blocks = std::max(1, std::min(1024, (m + 255) / 256));
+  k_df_reset_queue<<<blocks, 256, 0, s>>>(cnt, pend, pend0, ngroups, queue, init, n0, ntasks,
+                                          ctr);
 }
 
 __global__ void k_df_reset(unsigned* cnt, int count) {

@@ -539,8 +539,8 @@ struct CostModel {
             const int64_t r = 2 * ((int64_t)v.range_off + (int64_t)ch * v.nchild + q);
             ent += L.asm_ranges[r + 1] - L.asm_ranges[r];
           }
-          us += multi ? 5.5 + 2.7e-3 * ent * k + 8.2 * v.nchild
-                      : 0.8 + 4.1e-3 * ent * k + 1.5 * v.nchild;
+          us += multi ? 2.4 + 1.2e-3 * ent * k + 3.6 * v.nchild
+                      : 0.4 + 2.1e-3 * ent * k + 0.8 * v.nchild;
         }
         return us;
       }
@@ -552,22 +552,22 @@ struct CostModel {
         for (int ch = t.b * G; ch < std::min(nch, (t.b + 1) * G); ++ch) {
           const double ent =
               std::min<int64_t>(kSolveChunk, v.ncols + v.R - (int64_t)ch * kSolveChunk);
-          us += multi ? 5.4 + 4.5e-3 * ent * k : 2.7 + 2.5e-3 * ent * k;
+          us += multi ? 1.9 + 1.6e-3 * ent * k : 1.3 + 1.2e-3 * ent * k;
         }
         return us;
       }
       case 2: {
         const dev::GvTask& g = gv[t.a];
-        if (g.kind == 2) return multi ? 9.0 : 3.4;
+        if (g.kind == 2) return multi ? 8.7 : 2.6;
         const double kb = (double)g.rows * g.cols * e / 1000.0;
         if (g.kind == 0)
-          return multi ? 11.2 + 0.141 * kb * passes + 2.0 * std::ceil(g.cols / 128.0) * passes
-                       : 6.6 + 0.07 * kb + 1.0 * std::ceil(g.cols / 128.0);
-        return multi ? 10.5 + 0.198 * kb * passes + 3.1 * std::ceil(g.rows / 64.0) * passes
-                     : 5.4 + 0.074 * kb + 0.45 * std::ceil(g.rows / 64.0);
+          return multi ? 5.3 + 0.079 * kb * passes + 3.8 * std::ceil(g.cols / 128.0) * passes
+                       : 5.5 + 0.044 * kb;
+        return multi ? 6.5 + 0.036 * kb * passes + 1.9 * std::ceil(g.rows / 64.0) * passes
+                     : 6.1 + 0.10 * kb;
       }
       case 3:
-        return multi ? 12.5 * k : 10.0;
+        return multi ? 10.0 * k : 6.0;
       default: {
         double kb = 0, nodes = 0, front = 0;
         for (int q = SB.ptr[t.a]; q < SB.ptr[t.a + 1]; ++q) {
@@ -577,11 +577,11 @@ struct CostModel {
           nodes += 1;
         }
         if (multi)  // sixteen right-hand sides per pass, fronts in LDS
-          return t.type == 4 ? 6.0 + (4.5 * nodes + 0.01 * front + 0.1 * kb) * passes
-                             : 4.0 + (3.0 * nodes + 0.01 * front + 0.1 * kb) * passes;
+          return t.type == 4 ? (11.8 * nodes + 0.080 * front) * passes
+                             : (2.07 * nodes + 0.112 * front) * passes;
         const double sp = std::ceil(k / kSubtreeRhs);
-        return t.type == 4 ? 10.9 + (1.35 * kb + 10.7 * nodes) * sp
-                           : 3.8 + (0.8 * kb + 4.3 * nodes) * sp;
+        return t.type == 4 ? 10.2 + (4.68 * nodes + 0.107 * kb) * sp
+                           : 15.9 + (3.75 * nodes + 0.22 * kb) * sp;
       }
     }
   }
@@ -860,6 +860,25 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     t.wn0 = t.wg0 >= 0 ? need[t.wg0] : 0u;
   }
   {
+    // per task for timing traces: the cost model's estimate and, for
+    // subtree tasks, the model's inputs (supernodes, front entries, kB)
+    const CostModel cm{P, L, SB, gv, tm};
+    out.trace_feat.assign(4 * tasks.size(), 0.f);
+    for (size_t i = 0; i < tasks.size(); ++i) {
+      const dev::DfTask& t = tasks[i];
+      float* f = out.trace_feat.data() + 4 * i;
+      f[0] = (float)cm.us(t);
+      if (t.type == 4 || t.type == 5) {
+        for (int q = SB.ptr[t.a]; q < SB.ptr[t.a + 1]; ++q) {
+          const Supernode& s = P.nodes[SB.nodes[q]];
+          f[1] += 1.f;
+          f[2] += (float)(s.ncols + s.R());
+          f[3] += (float)((double)s.ncols * (s.ncols + s.R()) * tm.elem_bytes / 1000.0);
+        }
+      }
+    }
+  }
+  {
     // matrix-vector descriptors parallel to the task list
     dev::GvTask none{};
     none.xpart = -1;
@@ -883,6 +902,55 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
       if (g < 0 || g >= (int)need.size() || last_task[g] >= (int)i || need[g] == 0u)
         throw std::logic_error("dataflow apply: task waits on a later or empty group");
     }
+  {
+    // ready-queue form: tasks per group and successor groups, both in list
+    // order
+    const int ng = (int)need.size();
+    out.q_gptr.assign(ng + 1, 0);
+    for (const auto& t : tasks) ++out.q_gptr[t.group + 1];
+    for (int g = 0; g < ng; ++g) out.q_gptr[g + 1] += out.q_gptr[g];
+    out.q_gtasks.assign(tasks.size(), 0);
+    std::vector<int> fill(out.q_gptr.begin(), out.q_gptr.end() - 1);
+    for (size_t i = 0; i < tasks.size(); ++i) out.q_gtasks[fill[tasks[i].group]++] = (int)i;
+    out.q_sptr.assign(ng + 1, 0);
+    out.q_pend0.assign(ng, 0);
+    for (int h = 0; h < ng; ++h) {
+      out.q_pend0[h] = gwoff[h + 1] - gwoff[h];
+      for (int q = gwoff[h]; q < gwoff[h + 1]; ++q) ++out.q_sptr[gw[q] + 1];
+    }
+    for (int g = 0; g < ng; ++g) out.q_sptr[g + 1] += out.q_sptr[g];
+    out.q_succ.assign(gw.size(), 0);
+    std::vector<int> sf(out.q_sptr.begin(), out.q_sptr.end() - 1);
+    for (int h = 0; h < ng; ++h)
+      for (int q = gwoff[h]; q < gwoff[h + 1]; ++q) out.q_succ[sf[gw[q]]++] = h;
+    // bottom level of every group (estimated time from its start to the end
+    // of the apply; groups only wait on groups created before them): the
+    // successors of a group and the initial tasks go most critical first (a
+    // workgroup runs the first ready successor's first task itself)
+    std::vector<double> bl(ng, 0.0), gcost(ng, 0.0);
+    for (size_t i = 0; i < tasks.size(); ++i)
+      gcost[tasks[i].group] = std::max(gcost[tasks[i].group], (double)out.trace_feat[4 * i]);
+    std::vector<int> g2(ng, -1);
+    for (const auto& t : tasks)
+      if (t.group2 >= 0) g2[t.group] = t.group2;
+    for (int g = ng - 1; g >= 0; --g) {
+      double below = g2[g] >= 0 ? bl[g2[g]] : 0.0;
+      for (int q = out.q_sptr[g]; q < out.q_sptr[g + 1]; ++q)
+        below = std::max(below, bl[out.q_succ[q]]);
+      bl[g] = gcost[g] + below;
+    }
+    for (int g = 0; g < ng; ++g)
+      std::stable_sort(out.q_succ.begin() + out.q_sptr[g], out.q_succ.begin() + out.q_sptr[g + 1],
+                       [&](int x, int y) { return bl[x] > bl[y]; });
+    out.q_init.clear();
+    for (size_t i = 0; i < tasks.size(); ++i)
+      if (out.q_pend0[tasks[i].group] == 0) out.q_init.push_back((int)i);
+    std::stable_sort(out.q_init.begin(), out.q_init.end(), [&](int x, int y) {
+      return bl[tasks[x].group] > bl[tasks[y].group];
+    });
+    if (out.q_init.empty() && !tasks.empty())
+      throw std::logic_error("dataflow apply: no task is ready at the start");
+  }
   out.tasks.swap(tasks);
   out.gv.swap(gv);
   out.need.swap(need);

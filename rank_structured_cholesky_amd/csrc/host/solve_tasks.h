@@ -159,6 +159,14 @@ struct DataflowTasks {
   std::vector<int> gwoff, gw;      // groups waited on, per group
   int64_t slots = 0;
   double est_span_us = 0;          // makespan of the list schedule (0: level order)
+  std::vector<float> trace_feat;   // per task: estimated us, and for subtree
+                                   // tasks supernodes, front entries, kB
+  // Ready-queue form of the same graph: a group's tasks are pushed to the
+  // queue once every group it waits on has completed.
+  std::vector<int> q_gptr, q_gtasks;  // tasks of group g: q_gtasks[q_gptr[g] ..)
+  std::vector<int> q_sptr, q_succ;    // groups waiting on group g: q_succ[q_sptr[g] ..)
+  std::vector<int> q_pend0;           // groups each group waits on
+  std::vector<int> q_init;            // tasks ready at the start, in list order
 };
 
 // Vector layout; subtree_elems = 0 disables the single-right-hand-side

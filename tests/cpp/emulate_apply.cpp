@@ -8,7 +8,10 @@
 //       group a task waits on is complete when the task starts,
 //   (3) the dataflow list on several threads that take tickets and wait on
 //       completion counters, the way the persistent kernel does,
-// with every buffer access bounds-checked.  All three must give bitwise the
+//   (4) the ready-queue form on several threads (pop a slot, run, push the
+//       tasks of every group whose last wait completed, run the first of
+//       them directly), as the kernel's queue dispatch does,
+// with every buffer access bounds-checked.  All four must give bitwise the
 // same result: each task's result depends only on its inputs, so any missing
 // dependency or out-of-range index shows up.
 //
@@ -488,6 +491,88 @@ int main(int argc, char** argv) {
     if (stuck) fail("threaded dataflow run stalled");
   }
   if (!agrees(S.X.v)) fail("threaded dataflow result differs from the level path");
+
+  // (4) the ready-queue form on threads, as the device runs it: pop a slot,
+  // wait until it is written, run the task (every wait must be complete),
+  // signal its groups; the task completing a group's last wait pushes the
+  // tasks of that group
+  S.reset_work();
+  {
+    const int nt = (int)D.tasks.size();
+    std::vector<std::atomic<unsigned>> cnt(ng);
+    std::vector<std::atomic<int>> pend(ng), slot(nt);
+    std::vector<std::atomic<int>> runs(nt);
+    for (int g = 0; g < ng; ++g) {
+      cnt[g].store(0u);
+      pend[g].store(D.q_pend0[g]);
+    }
+    for (int i = 0; i < nt; ++i) {
+      slot[i].store(i < (int)D.q_init.size() ? D.q_init[i] : -1);
+      runs[i].store(0);
+    }
+    std::atomic<int> head{0}, tail{(int)D.q_init.size()};
+    std::atomic<bool> stuck{false};
+    if ((int)D.q_gptr.size() != ng + 1 || (int)D.q_sptr.size() != ng + 1 ||
+        (int)D.q_gtasks.size() != nt)
+      fail("ready-queue arrays have the wrong sizes");
+    constexpr int kSkip = -3;
+    auto worker = [&]() {
+      int next = -1;  // the task this worker made ready and runs itself
+      for (;;) {
+        int i = next;
+        next = -1;
+        while (i < 0) {
+          const int idx = head.fetch_add(1);
+          if (idx >= nt) return;
+          const auto t0 = std::chrono::steady_clock::now();
+          while ((i = slot[idx].load(std::memory_order_acquire)) == -1) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+              stuck = true;
+              return;
+            }
+            std::this_thread::yield();
+          }
+          if (i == kSkip) i = -1;
+        }
+        if (i >= nt) fail("queue slot holds an invalid task");
+        if (runs[i].fetch_add(1) != 0) fail("task run twice");
+        const dev::DfTask& t = D.tasks[i];
+        for (int q = D.gwoff[t.group]; q < D.gwoff[t.group + 1]; ++q)
+          if (cnt[D.gw[q]].load(std::memory_order_acquire) != D.need[D.gw[q]])
+            fail("queued task run before its waits completed");
+        run_df_task(S, D, t);
+        const int gs[2] = {t.group, t.group2};
+        for (int g : gs) {
+          if (g < 0) continue;
+          if (cnt[g].fetch_add(1u, std::memory_order_acq_rel) + 1u != D.need[g]) continue;
+          for (int q = D.q_sptr[g]; q < D.q_sptr[g + 1]; ++q) {
+            const int h = D.q_succ[q];
+            if (pend[h].fetch_sub(1, std::memory_order_acq_rel) != 1) continue;
+            const int c0 = D.q_gptr[h], cn = D.q_gptr[h + 1] - c0;
+            if (cn <= 0) continue;
+            const int base = tail.fetch_add(cn);
+            if (base + cn > nt) fail("ready queue overflows");
+            int first = 0;
+            if (next < 0) {
+              next = D.q_gtasks[c0];
+              first = 1;
+            }
+            for (int u = 0; u < cn; ++u)
+              slot[base + u].store(u < first ? kSkip : D.q_gtasks[c0 + u],
+                                   std::memory_order_release);
+          }
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (int q = 0; q < nthreads; ++q) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    if (stuck) fail("ready-queue run stalled");
+    for (int i = 0; i < nt; ++i)
+      if (runs[i].load() != 1) fail("a task did not run exactly once in the ready-queue run");
+    if (tail.load() != nt) fail("ready queue did not receive every task");
+  }
+  if (!agrees(S.X.v)) fail("ready-queue result differs from the level path");
   int fused = 0;
   for (size_t i = 0; i < D.tasks.size(); ++i)
     if (D.tasks[i].type == 2 && D.gv[i].kind == 0 && D.gv[i].xpart >= 0) ++fused;

@@ -1,6 +1,6 @@
 """The apply's task lists on the CPU (tests/cpp/emulate_apply.cpp): the
-dataflow list run in ticket order and on threads gives the result of the
-level-scheduled list (bitwise, or to rounding when the dataflow list splits
+dataflow list run in ticket order, on threads that take tickets, and on
+threads that pop a ready queue gives the result of the level-scheduled list (bitwise, or to rounding when the dataflow list splits
 long rows into partial sums), every wait is satisfied when its task starts,
 and every buffer access is in range; with and without subtree tasks, in
 level order and in list-schedule ticket order."""

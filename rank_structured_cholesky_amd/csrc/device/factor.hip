@@ -1523,7 +1523,20 @@ void Engine<T>::setup_static() {
     // y in LDS) bound them; RSC_MS_SUBTREE_ELEMS caps their panel elements
     int64_t mlim = 1 << 20;
     if (const char* e = std::getenv("RSC_MS_SUBTREE_ELEMS")) mlim = std::atoll(e);
-    build_solve_layout(P, SL, lim, mlim);
+    // at most as many multi-right-hand-side subtree tasks as the dataflow
+    // launch has resident workgroups (the smallest are merged), so all of
+    // them start at once; RSC_MS_SUBTREE_TASKS overrides (0: no merging)
+    int mtasks = 0;
+    {
+      int dev = 0, cus = 1;
+      RSC_HIP(hipGetDevice(&dev));
+      RSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      int cap = 4;
+      if (const char* e = std::getenv("RSC_DF_BLOCKS_PER_CU")) cap = std::max(1, std::atoi(e));
+      mtasks = cus * std::min(cap, dev::dataflow_blocks_per_cu<T>(true));
+    }
+    if (const char* e = std::getenv("RSC_MS_SUBTREE_TASKS")) mtasks = std::atoi(e);
+    build_solve_layout(P, SL, lim, mlim, mtasks);
   }
   phase("solve layout");
   nsub_ = (int)SL.sub.root.size();
@@ -1531,7 +1544,16 @@ void Engine<T>::setup_static() {
   dstnodes_.upload(SL.sub.nodes.empty() ? std::vector<int>{0} : SL.sub.nodes, st_);
   dstcols_.upload(SL.sub.cols.empty() ? std::vector<int>{0, -1} : SL.sub.cols, st_);
   dmsptr_.upload(SL.sub_multi.ptr, st_);
-  dmsnodes_.upload(SL.sub_multi.nodes.empty() ? std::vector<int>{0} : SL.sub_multi.nodes, st_);
+  {
+    // the multi-right-hand-side subtree tasks read a task root as ~j (its
+    // update vector is stored for other workgroups)
+    std::vector<int> mn(SL.sub_multi.nodes);
+    for (int& j : mn) {
+      const int par = P.nodes[j].parent;
+      if (par < 0 || SL.sub_multi.of[par] != SL.sub_multi.of[j]) j = ~j;
+    }
+    dmsnodes_.upload(mn.empty() ? std::vector<int>{0} : mn, st_);
+  }
   dmscols_.upload(SL.sub_multi.cols.empty() ? std::vector<int>{0, -1} : SL.sub_multi.cols, st_);
   dpanel_.upload(SL.panel, st_);
   dchildren_.upload(SL.children, st_);

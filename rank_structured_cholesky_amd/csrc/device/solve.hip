@@ -1537,18 +1537,20 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
   double* Fr = lds;                       // [16][kMsFld] front vector
   double* Yr = lds + kMfmaRhs * kMsFld;   // [16][kMsYld] y
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
-  const int rootn = uni(a.nodes[q1 - 1]);
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
     for (int qi = q0; qi < q1; ++qi) {
       const int tid = body_tid();  // per supernode (not hoisted out of the loop)
-      const int j = uni(a.nodes[qi]);
+      // a root of the task (~j): its update vector goes to other workgroups
+      const int jr = uni(a.nodes[qi]);
+      const bool root = jr < 0;
+      const int j = root ? ~jr : jr;
       const VecNode S = uniform_node(a.vn[j]);
       const int64_t P = uni64(a.panel[j]);
       const int c = S.ncols, R = S.R, ld = c + R;
       const T* __restrict__ Lt = F + P;
       if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
-        ms_forward_large<T>(Lt, S, a, b, c0, kb, j == rootn, lds);
+        ms_forward_large<T>(Lt, S, a, b, c0, kb, root, lds);
         continue;
       }
       for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
@@ -1584,7 +1586,6 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
                              vst<true>(zb + (int64_t)r * b.n + i, v);
                            });
       double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
-      const bool root = j == rootn;
       for (int idx = tid; idx < kb * R; idx += 256) {
         const int r = idx / R, t = idx - r * R;
         const double v = Fr[r * kMsFld + c + t];
@@ -1605,7 +1606,8 @@ __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int s
     const int kb = min(kMfmaRhs, k - c0);
     for (int qi = q1 - 1; qi >= q0; --qi) {
       const int tid = body_tid();  // per supernode (not hoisted out of the loop)
-      const int j = uni(a.nodes[qi]);
+      const int jr = uni(a.nodes[qi]);
+      const int j = jr < 0 ? ~jr : jr;  // task roots are listed as ~j
       const VecNode S = uniform_node(a.vn[j]);
       const int64_t P = uni64(a.panel[j]);
       const int c = S.ncols, R = S.R, ld = c + R;

@@ -12,12 +12,15 @@
 
 namespace rsc {
 
-void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi) {
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi,
+                    int max_tasks) {
   const int ns = (int)P.nodes.size();
   S.of.assign(ns, -1);
   S.ptr.assign(1, 0);
   S.nodes.clear();
   S.root.clear();
+  S.rptr.assign(1, 0);
+  S.roots.clear();
   S.cols.clear();
   if (max_elems <= 0) return;
   // children are numbered before their parents (postordered supernodes)
@@ -36,28 +39,71 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi) {
     ok[j] = e && c <= max_elems;
     cost[j] = c;
   }
+  // one tree per maximal subtree: its nodes in increasing (post) order
+  struct Tree {
+    std::vector<int> nodes, roots;
+    double work = 0;  // supernodes plus front entries / 150 (relative cost)
+  };
+  std::vector<Tree> trees;
   std::vector<int> stack;
   for (int j = 0; j < ns; ++j) {
     const int par = P.nodes[j].parent;
     if (!ok[j] || (par >= 0 && ok[par])) continue;
-    const int id = (int)S.root.size();
-    S.root.push_back(j);
-    std::vector<int> mem;
+    Tree t;
+    t.roots.push_back(j);
     stack.assign(1, j);
     while (!stack.empty()) {
       const int q = stack.back();
       stack.pop_back();
-      mem.push_back(q);
+      t.nodes.push_back(q);
+      t.work += 1.0 + (P.nodes[q].ncols + P.nodes[q].R()) / 150.0;
       for (int ch : P.nodes[q].children) stack.push_back(ch);
     }
-    std::sort(mem.begin(), mem.end());
-    for (int q : mem) S.of[q] = id;
-    S.nodes.insert(S.nodes.end(), mem.begin(), mem.end());
+    std::sort(t.nodes.begin(), t.nodes.end());
+    trees.push_back(std::move(t));
+  }
+  if (max_tasks > 0 && (int)trees.size() > max_tasks) {
+    // merge the two smallest tasks until max_tasks remain (a task runs its
+    // trees one after the other)
+    using Item = std::pair<double, int>;
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+    for (int i = 0; i < (int)trees.size(); ++i) heap.push({trees[i].work, i});
+    std::vector<char> live(trees.size(), 1);
+    int count = (int)trees.size();
+    while (count > max_tasks && heap.size() >= 2) {
+      const int x = heap.top().second;
+      heap.pop();
+      const int y = heap.top().second;
+      heap.pop();
+      Tree& tx = trees[x];
+      Tree& ty = trees[y];
+      tx.nodes.insert(tx.nodes.end(), ty.nodes.begin(), ty.nodes.end());
+      tx.roots.insert(tx.roots.end(), ty.roots.begin(), ty.roots.end());
+      tx.work += ty.work;
+      live[y] = 0;
+      heap.push({tx.work, x});
+      --count;
+    }
+    std::vector<Tree> kept;
+    for (size_t i = 0; i < trees.size(); ++i)
+      if (live[i]) kept.push_back(std::move(trees[i]));
+    trees.swap(kept);
+    // tasks in the order of their first root
+    std::sort(trees.begin(), trees.end(),
+              [](const Tree& u, const Tree& v) { return u.roots[0] < v.roots[0]; });
+  }
+  for (const Tree& t : trees) {
+    const int id = (int)S.root.size();
+    S.root.push_back(t.roots[0]);
+    S.roots.insert(S.roots.end(), t.roots.begin(), t.roots.end());
+    S.rptr.push_back((int)S.roots.size());
+    for (int q : t.nodes) S.of[q] = id;
+    S.nodes.insert(S.nodes.end(), t.nodes.begin(), t.nodes.end());
     S.ptr.push_back((int)S.nodes.size());
-    // column range of the subtree when its supernodes' columns are contiguous
+    // column range of the task when its supernodes' columns are contiguous
     int lo = INT_MAX, hi = 0;
     int64_t tot = 0;
-    for (int q : mem) {
+    for (int q : t.nodes) {
       lo = std::min(lo, P.nodes[q].first);
       hi = std::max(hi, P.nodes[q].first + P.nodes[q].ncols);
       tot += P.nodes[q].ncols;
@@ -68,7 +114,7 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi) {
 }
 
 void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems,
-                        int64_t multi_subtree_elems) {
+                        int64_t multi_subtree_elems, int multi_max_tasks) {
   const int ns = (int)P.nodes.size();
   L.children.clear();
   L.relmap.clear();
@@ -137,7 +183,7 @@ void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems,
   for (int j = 0; j < ns; ++j)
     if (!P.nodes[j].compress) L.panel[j] = P.nodes[j].off_panel;
   build_subtrees(P, subtree_elems, L.sub);
-  build_subtrees(P, multi_subtree_elems, L.sub_multi, true);
+  build_subtrees(P, multi_subtree_elems, L.sub_multi, true, multi_max_tasks);
 }
 
 // ---- per-node stages ------------------------------------------------------
@@ -759,6 +805,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
   Subtrees none;
   none.of.assign(ns, -1);
   none.ptr.assign(1, 0);
+  none.rptr.assign(1, 0);
   const Subtrees& SB = subtrees ? *subtrees : none;
   const int nsub = (int)SB.root.size();
   std::vector<std::vector<int>> lnodes(P.levels.size());
@@ -770,7 +817,7 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     cur_stage = 3;
     const int g = new_group({});
     add(4, sb, 0, g);
-    lastA[SB.root[sb]] = lastB[SB.root[sb]] = g;
+    for (int q = SB.rptr[sb]; q < SB.rptr[sb + 1]; ++q) lastA[SB.roots[q]] = lastB[SB.roots[q]] = g;
   }
   for (size_t lv = 0; lv < P.levels.size(); ++lv) {
     const std::vector<int>& nodes = lnodes[lv];
@@ -827,9 +874,13 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
         }
   }
   for (int sb = 0; sb < nsub; ++sb) {
-    const int r = SB.root[sb], par = P.nodes[r].parent;
-    std::vector<int> waits = {lastB[r]};
-    if (par >= 0) waits.push_back(lastC[par]);
+    const int r = SB.root[sb];
+    std::vector<int> waits;
+    for (int q = SB.rptr[sb]; q < SB.rptr[sb + 1]; ++q) {
+      const int rq = SB.roots[q], par = P.nodes[rq].parent;
+      waits.push_back(lastB[rq]);
+      if (par >= 0) waits.push_back(lastC[par]);
+    }
     cur_node = r;
     cur_stage = 4;
     const int g = new_group(waits);

@@ -108,18 +108,24 @@ inline bool multi_subtree_node(int ncols, int R) {
 
 // Subtrees of uncompressed supernodes handled by one task each.
 struct Subtrees {
-  std::vector<int> of;          // per supernode: subtree id, -1 if none
-  std::vector<int> ptr, nodes;  // nodes of subtree s (children before parents)
-  std::vector<int> root;        // root supernode of subtree s
-  std::vector<int> cols;        // per subtree: first column, column count
+  std::vector<int> of;          // per supernode: subtree task id, -1 if none
+  std::vector<int> ptr, nodes;  // nodes of task s (each tree's children before
+                                // its parents)
+  std::vector<int> root;        // first root supernode of task s
+  std::vector<int> rptr, roots;  // every root of task s: roots[rptr[s] .. rptr[s + 1])
+  std::vector<int> cols;        // per task: first column, column count
                                 // (-1 when the columns are not contiguous)
 };
 // Maximal subtrees of uncompressed supernodes with at most max_cols columns
 // and max_rows off-diagonal rows (and ncols + R <= max_front) each, whose
-// panels hold at most max_elems elements in total.
+// panels hold at most max_elems elements in total; one task per subtree.
 // multi: the supernode rule of the multi-right-hand-side list
 // (multi_subtree_node) instead of kSubtreeMaxCols / kSubtreeMaxRows.
-void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi = false);
+// max_tasks > 0: while there are more subtrees than that, the two smallest
+// tasks are merged into one (a task then runs several trees in turn), so
+// every subtree task of the list starts at once on max_tasks workgroups.
+void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi = false,
+                    int max_tasks = 0);
 
 struct SolvePhase {
   int kind;  // 0 forward assemble, 1 backward gather, 2 small tasks, 3 large tasks,
@@ -171,8 +177,10 @@ struct DataflowTasks {
 
 // Vector layout; subtree_elems = 0 disables the single-right-hand-side
 // subtree tasks, multi_subtree_elems = 0 the multi-right-hand-side ones.
+// multi_max_tasks: subtree tasks of the multi-right-hand-side list at most
+// (0: no merging).
 void build_solve_layout(const Plan& P, SolveLayout& L, int64_t subtree_elems,
-                        int64_t multi_subtree_elems = 0);
+                        int64_t multi_subtree_elems = 0, int multi_max_tasks = 0);
 void build_level_tasks(const Plan& P, const SolveLayout& L, LevelTasks& out);
 // Ticket order of a dataflow list.  workers = 0 keeps the level order;
 // otherwise the list is ordered by a list schedule on that many workgroups

@@ -16,12 +16,13 @@
 // dependency or out-of-range index shows up.
 //
 //   emulate_apply <problem> <N> <max_dense_block> <max_diagonal_block> <nrhs> <threads>
-//                 [subtree_elems [workers [wide_elems [list]]]]
+//                 [subtree_elems [workers [wide_elems [list [max_tasks]]]]]
 //   workers: ticket order by a list schedule on that many workgroups
 //   (default 1024; 0 keeps the level order); wide_elems: products split
 //   finer from this many elements (default 2^20); list: "multi" runs the
 //   multi-right-hand-side list (its own subtree tasks, supernode limits of
-//   the sixteen-column LDS bodies) instead of the single one
+//   the sixteen-column LDS bodies) instead of the single one; max_tasks (after
+//   list): at most that many multi subtree tasks (the smallest merged)
 //   problem: p3d (7-point log-normal Poisson), p2d, el (elasticity)
 // Prints "ok ..." and exits 0 on success.
 #include <atomic>
@@ -322,7 +323,10 @@ int main(int argc, char** argv) {
   SolveLayout L;
   const int64_t sub_elems = argc > 7 ? std::atoll(argv[7]) : std::max<int64_t>(256, P.factor_elems / 64);
   const bool multi = argc > 10 && std::string(argv[10]) == "multi";
-  build_solve_layout(P, L, sub_elems, multi ? (int64_t)1 << 20 : 0);
+  const int max_tasks = argc > 11 ? std::atoi(argv[11]) : 0;
+  build_solve_layout(P, L, sub_elems, multi ? (int64_t)1 << 20 : 0, max_tasks);
+  if (multi && max_tasks > 0 && (int)L.sub_multi.root.size() > max_tasks)
+    fail("more subtree tasks than the limit");
   LevelTasks LT;
   build_level_tasks(P, L, LT);
   DataflowTasks D;

@@ -46,6 +46,8 @@ def built():
     "p2d 40 10 16 3 8 16384 64 0 multi",
     "el 5 30 40 2 8 16384 1024 0 multi",
     "p3d 16 63 64 1 8 16384 0 0 multi",  # level order
+    "p3d 24 63 64 16 8 16384 1024 0 multi 7",  # subtrees merged into seven tasks
+    "el 6 30 40 2 8 16384 1024 0 multi 3",  # three tasks of several trees each
 ])
 def test_dataflow_schedule_matches_level_schedule(args):
     r = subprocess.run([BIN] + args.split(), capture_output=True, text=True, timeout=240)

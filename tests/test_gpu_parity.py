@@ -151,11 +151,14 @@ def test_block_pcg_is_batch_invariant():
 
 
 @pytest.mark.parametrize("env", [dict(RSC_APPLY_LEVELS="1"), dict(RSC_SUBTREE_ELEMS="1"),
-                                 dict(RSC_SUBTREE_ELEMS="1000000")])
+                                 dict(RSC_SUBTREE_ELEMS="1000000"), dict(RSC_DF_QUEUE="0"),
+                                 dict(RSC_MS_SUBTREE_TASKS="3")])
 def test_apply_schedules_agree(monkeypatch, env):
-    """The level-scheduled launches and the dataflow launch with no or with
-    large subtree tasks compute the same apply (one and three right-hand
-    sides) as the default dataflow schedule, to rounding."""
+    """The level-scheduled launches, the dataflow launch with no or with
+    large subtree tasks, ticket dispatch instead of the ready queue, and the
+    multi-right-hand-side subtrees merged into three tasks compute the same
+    apply (one and three right-hand sides) as the default dataflow schedule,
+    to rounding."""
     build, sopts, okw, tol = CASES["p3d20_hodlr"]
     A, xyz = build()
     n = A._nrow

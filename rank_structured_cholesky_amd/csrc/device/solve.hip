@@ -2050,8 +2050,13 @@ template int dataflow_blocks_per_cu<float>(bool);
 // column indices are read once for all the pass's columns, and each
 // column's sum runs over the row's entries in storage order, so a column's
 // result does not depend on how many columns share the launch.  Inactive
-// columns (active[col] == 0) are neither computed nor written.
+// columns (active[col] == 0) are neither computed nor written.  Entries go
+// in groups of eight: their values and indices load together, then per
+// column the eight x entries (clamped to the row's first entry past the
+// end, masked), so a row costs a few memory round trips instead of two per
+// entry.
 constexpr int kSpmvCols = 16;
+constexpr int kSpmvEnt = 8;
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ p,
                                               const int* __restrict__ idx,
                                               const double* __restrict__ val,
@@ -2071,12 +2076,26 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ p,
       acc[c] = 0;
     }
     const double* xc = x + (int64_t)c0 * ldx;
-    for (int e = e0; e < e1; ++e) {
-      const double v = val[e];
-      const int64_t j = idx[e];
+    for (int eb = e0; eb < e1; eb += kSpmvEnt) {
+      double v[kSpmvEnt];
+      int64_t j[kSpmvEnt];
 #pragma unroll
-      for (int c = 0; c < kSpmvCols; ++c)
-        if (on[c]) acc[c] += v * xc[j + (int64_t)c * ldx];
+      for (int u = 0; u < kSpmvEnt; ++u) {
+        const int e = min(eb + u, e1 - 1);
+        v[u] = val[e];
+        j[u] = idx[e];
+      }
+#pragma unroll
+      for (int c = 0; c < kSpmvCols; ++c) {
+        if (!on[c]) continue;
+        const double* xcc = xc + (int64_t)c * ldx;
+        double xv[kSpmvEnt];
+#pragma unroll
+        for (int u = 0; u < kSpmvEnt; ++u) xv[u] = xcc[j[u]];
+#pragma unroll
+        for (int u = 0; u < kSpmvEnt; ++u)
+          if (eb + u < e1) acc[c] += v[u] * xv[u];
+      }
     }
 #pragma unroll
     for (int c = 0; c < kSpmvCols; ++c)

@@ -100,19 +100,38 @@ __global__ void k_residual(int n, int k, const double* __restrict__ b, int ldb,
   r[t] = b[i + (int64_t)j * ldb] - r[t];
 }
 
-// x += alpha d ; r -= alpha q
-__global__ void k_update_xr(int n, int k, const PcgCol* __restrict__ st,
-                            double* __restrict__ x, int ldx, double* __restrict__ r,
-                            const double* __restrict__ d, const double* __restrict__ q) {
+// x += alpha d ; r -= alpha q (active columns) with the partial sums of
+// r.r: the grid and row order of k_dot (nb x k blocks of RB threads, rows
+// strided by nb RB), so the partials are those k_dot(r, r) gives after the
+// update.  Columns with mask[col] == 0 (when mask is set) write no partials.
+__global__ __launch_bounds__(RB) void k_update_xr_rr(int n, const PcgCol* __restrict__ st,
+                                                     double* __restrict__ x, int ldx,
+                                                     double* __restrict__ r,
+                                                     const double* __restrict__ d,
+                                                     const double* __restrict__ q,
+                                                     double* __restrict__ partial,
+                                                     const int* __restrict__ mask) {
+  __shared__ double sh[RB];
   const int col = blockIdx.y;
-  if (!st[col].active) return;
+  const bool act = st[col].active != 0;
+  const bool dot = !mask || mask[col];
+  if (!act && !dot) return;
   const double al = st[col].alpha;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t o = i + (int64_t)col * n;
-    x[i + (int64_t)col * ldx] += al * d[o];
-    r[o] -= al * q[o];
+  double acc = 0;
+  for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RB) {
+    const int64_t o = i + (int64_t)col * n;
+    double rv = r[o];
+    if (act) {
+      x[i + (int64_t)col * ldx] += al * d[o];
+      rv -= al * q[o];
+      r[o] = rv;
+    }
+    acc += rv * rv;
   }
+  if (!dot) return;
+  const double sres = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[col * RMAX + blockIdx.x] = sres;
 }
 
 // d = beta d + s
@@ -423,10 +442,10 @@ void pcg_solve_device(FactorEngine& M, const DevCsr& A, PcgWork& w, int k, const
     A.spmv(D, n, Q, n, k, w.flags.as<int>(), s);
     k_dot<<<dgrid, RB, 0, s>>>(n, D, n, Q, n, w.part1.as<double>());
     k_scalars<<<k, RB, 0, s>>>(2, w.part1.as<double>(), nullptr, nb, st, eps, max_iters);
-    k_update_xr<<<ugrid, 256, 0, s>>>(n, k, st, X, ldx, R, D, Q);
+    // the update with the partials of r.r (r does not change in the apply)
+    k_update_xr_rr<<<dgrid, RB, 0, s>>>(n, st, X, ldx, R, D, Q, w.part2.as<double>(), nullptr);
     M.apply(k, R, n, S, n, s, any);
     k_dot<<<dgrid, RB, 0, s>>>(n, R, n, S, n, w.part1.as<double>());
-    k_dot<<<dgrid, RB, 0, s>>>(n, R, n, R, n, w.part2.as<double>());
     k_scalars<<<k, RB, 0, s>>>(3, w.part1.as<double>(), w.part2.as<double>(), nb, st, eps,
                                max_iters);
     k_update_d<<<ugrid, 256, 0, s>>>(n, k, st, D, S);
@@ -520,11 +539,11 @@ void ServeBatch::step(FactorEngine& M, const DevCsr& A, int kact, double eps, in
   A.spmv(D, n, Q, n, kact, RUN, st);
   k_dot_masked<<<dim3(nb, kact), RB, 0, st>>>(n, D, n, Q, n, part1.as<double>(), RUN);
   k_scalars<<<kact, RB, 0, st>>>(2, part1.as<double>(), nullptr, nb, c, eps, max_iters);
-  k_update_xr<<<dim3(ublocks, kact), 256, 0, st>>>(n, kact, c, X, n, R, D, Q);
-  // INIT and RUN columns: s = M^{-1} r, r.s, r.r
+  // (with the partials of r.r for INIT and RUN columns, as the solo loop)
+  k_update_xr_rr<<<dim3(nb, kact), RB, 0, st>>>(n, c, X, n, R, D, Q, part2.as<double>(), BUSY);
+  // INIT and RUN columns: s = M^{-1} r, r.s
   M.apply(kact, R, n, S, n, st, anyb.as<int>(), 1);
   k_dot_masked<<<dim3(nb, kact), RB, 0, st>>>(n, R, n, S, n, part1.as<double>(), BUSY);
-  k_dot_masked<<<dim3(nb, kact), RB, 0, st>>>(n, R, n, R, n, part2.as<double>(), BUSY);
   k_serve_scalars<<<kact, RB, 0, st>>>(part1.as<double>(), part2.as<double>(), nb, c,
                                        state.as<int>(), eps, max_iters);
   k_serve_update_d<<<dim3(ublocks, kact), 256, 0, st>>>(n, c, state.as<int>(), D, S);

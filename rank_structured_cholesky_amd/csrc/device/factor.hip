@@ -122,15 +122,24 @@ class Engine final : public FactorEngine {
  public:
   Engine(const Plan& plan, const Csc& A, hipStream_t st)
       : plan_(plan), A_(A), st_(st) {
-    start_rng();
-    setup_static();
+    try {
+      start_rng();
+      setup_static();
+    } catch (...) {
+      // the helper threads already started are joined before the members
+      // go (a joinable std::thread would end the process)
+      shutdown();
+      throw;
+    }
   }
-  ~Engine() override {
+  ~Engine() override { shutdown(); }
+  void shutdown() {
     if (prog_thread_.joinable()) prog_thread_.join();
     (void)hipStreamSynchronize(st_);
     if (rng_thread_.joinable()) rng_thread_.join();
     if (df_thread_.joinable()) df_thread_.join();
     if (rng_host_) (void)hipHostFree(rng_host_);
+    rng_host_ = nullptr;
     drop_graph();
   }
 

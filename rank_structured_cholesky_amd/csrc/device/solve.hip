@@ -1385,12 +1385,68 @@ __device__ inline void ms_batched(int total, Index index, Load load, Use use) {
   }
 }
 
+// A processed supernode of a multi-right-hand-side subtree task as its
+// parent needs it.  A task runs its supernodes in postorder, so when a
+// supernode starts, its children are the top entries of a stack of these,
+// in increasing order (checked on the host when the tasks are built).
+struct MsChild {
+  int64_t fvr;  // its update vector (fv_off + ncols), first right-hand side
+  int64_t rm;   // its relative map
+  int R, pad;
+};
+// Descriptors of up to kMsRing consecutive supernodes of a task, loaded
+// together (two dependent memory round trips per kMsRing supernodes instead
+// of two per supernode), with the original indices of the columns of those
+// of at most kMsMaxCols columns.
+constexpr int kMsRing = 8;
+struct MsNodeDesc {
+  int64_t fv_off, rows_off, relmap_off, panel;
+  int first, ncols, R, nchild, root, pad;
+};
+struct MsRing {
+  MsNodeDesc d[kMsRing];
+  int perm[kMsRing][kMsMaxCols];
+};
+
+// Every thread of the workgroup: descriptors of the task's supernodes at
+// list positions q .. q + nr into the ring (lane l reads position q + l),
+// then the original indices of the small ones' columns (with_perm).
+__device__ void ms_ring_load(const SubtreeArgs& a, int q, int nr, bool with_perm, MsRing& rg) {
+  const int tid = body_tid();
+  __syncthreads();  // earlier readers of the ring are done
+  if (tid < nr) {
+    const int jr = a.nodes[q + tid];
+    const bool root = jr < 0;
+    const int j = root ? ~jr : jr;
+    const VecNode v = a.vn[j];
+    MsNodeDesc& d = rg.d[tid];
+    d.fv_off = v.fv_off;
+    d.rows_off = v.rows_off;
+    d.relmap_off = v.relmap_off;
+    d.panel = a.panel[j];
+    d.first = v.first;
+    d.ncols = v.ncols;
+    d.R = v.R;
+    d.nchild = v.nchild;
+    d.root = root ? 1 : 0;
+  }
+  __syncthreads();
+  if (with_perm) {
+    for (int idx = tid; idx < nr * kMsMaxCols; idx += 256) {
+      const int l = idx / kMsMaxCols, i = idx - l * kMsMaxCols;
+      const int c = rg.d[l].ncols;
+      if (i < c && c <= kMsMaxCols) rg.perm[l][i] = a.perm[rg.d[l].first + i];
+    }
+    __syncthreads();
+  }
+}
+
 // Forward step of a larger supernode of a multi subtree (fv_R in device
 // memory, plain accesses except the subtree root's final values).
 template <typename T>
 __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
                                  const SubtreeArgs& a, const VecBufs& b, int c0, int kb,
-                                 bool root, double* lds) {
+                                 bool root, const MsChild* ch, double* lds) {
   const int tid = body_tid();
   const int c = S.ncols, R = S.R, ld = c + R;
   double* Fc = lds;                      // [16][kMlFld] fv_c
@@ -1410,10 +1466,9 @@ __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
   }
   __syncthreads();
   for (int q = 0; q < S.nchild; ++q) {
-    const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
-    const int* rm = a.relmap + C.relmap_off;
-    const double* fc = b.fv + C.fv_off + C.ncols + (int64_t)c0 * b.fvs;
-    const int CR = C.R;
+    const int* rm = a.relmap + uni64(ch[q].rm);
+    const double* fc = b.fv + uni64(ch[q].fvr) + (int64_t)c0 * b.fvs;
+    const int CR = uni(ch[q].R);
     // a child's entries land on distinct positions: every load of a round
     // (its values, and the old fv_R entries they are added to) goes first
     for (int base = tid; base < kb * CR; base += 256 * 4) {
@@ -1534,64 +1589,91 @@ __device__ void ms_backward_large(const T* __restrict__ Zp, const VecNode& S,
 template <typename T>
 __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                            const VecBufs& b, int k, double* lds) {
+  __shared__ MsChild s_stk[kMsStack];
+  __shared__ MsRing s_ring;
   double* Fr = lds;                       // [16][kMsFld] front vector
   double* Yr = lds + kMfmaRhs * kMsFld;   // [16][kMsYld] y
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
+    int sp = 0;  // stack depth (the same in every thread)
     for (int qi = q0; qi < q1; ++qi) {
+      if ((qi - q0) % kMsRing == 0) ms_ring_load(a, qi, min(kMsRing, q1 - qi), true, s_ring);
       const int tid = body_tid();  // per supernode (not hoisted out of the loop)
-      // a root of the task (~j): its update vector goes to other workgroups
-      const int jr = uni(a.nodes[qi]);
-      const bool root = jr < 0;
-      const int j = root ? ~jr : jr;
-      const VecNode S = uniform_node(a.vn[j]);
-      const int64_t P = uni64(a.panel[j]);
+      const int l = (qi - q0) % kMsRing;
+      const MsNodeDesc& D = s_ring.d[l];
+      VecNode S{};
+      S.first = uni(D.first);
+      S.ncols = uni(D.ncols);
+      S.R = uni(D.R);
+      S.nchild = uni(D.nchild);
+      S.fv_off = uni64(D.fv_off);
+      S.rows_off = uni64(D.rows_off);
+      S.relmap_off = uni64(D.relmap_off);
+      const bool root = uni(D.root) != 0;  // its update vector goes to other workgroups
+      const int64_t P = uni64(D.panel);
       const int c = S.ncols, R = S.R, ld = c + R;
       const T* __restrict__ Lt = F + P;
+      const MsChild* ch = s_stk + sp - S.nchild;  // the children, in order
       if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
-        ms_forward_large<T>(Lt, S, a, b, c0, kb, root, lds);
-        continue;
-      }
-      for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
-        const int r = idx / ld, p = idx - r * ld;
-        if (r >= kb || p >= c) Fr[r * kMsFld + p] = 0.0;
-      }
-      ms_batched<2>(
-          kb * c, [&](int i) { return a.perm[S.first + i % c]; },
-          [&](int i, int o) { return a.B[o + (int64_t)(c0 + i / c) * a.ldb]; },
-          [&](int i, int, double v) { Fr[(i / c) * kMsFld + i % c] = v; });
-      __syncthreads();
-      for (int q = 0; q < S.nchild; ++q) {
-        const VecNode C = uniform_node(a.vn[uni(a.children[S.child_off + q])]);
-        const int* rm = a.relmap + C.relmap_off;
-        const double* fc = b.fv + C.fv_off + C.ncols;
-        const int CR = C.R;
-        ms_batched<4>(
-            kb * CR, [&](int i) { return rm[i % CR]; },
-            [&](int i, int) { return fc[(int64_t)(c0 + i / CR) * b.fvs + i % CR]; },
-            [&](int i, int p, double v) { Fr[(i / CR) * kMsFld + p] += v; });
+        ms_forward_large<T>(Lt, S, a, b, c0, kb, root, ch, lds);
+      } else {
+        for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
+          const int r = idx / ld, p = idx - r * ld;
+          if (r >= kb || p >= c) Fr[r * kMsFld + p] = 0.0;
+        }
+        // fv_c = b[perm[cols]] (original indices staged with the
+        // descriptors): every load of the node at once
+        {
+          const int* pm = s_ring.perm[l];
+          double v[2];
+          int ii[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int i = tid + 256 * u;
+            ii[u] = i;
+            v[u] = i < kb * c ? a.B[pm[i % c] + (int64_t)(c0 + i / c) * a.ldb] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+            if (ii[u] < kb * c) Fr[(ii[u] / c) * kMsFld + ii[u] % c] = v[u];
+        }
         __syncthreads();
+        for (int q = 0; q < S.nchild; ++q) {
+          const int* rm = a.relmap + uni64(ch[q].rm);
+          const double* fc = b.fv + uni64(ch[q].fvr);
+          const int CR = uni(ch[q].R);
+          ms_batched<4>(
+              kb * CR, [&](int i) { return rm[i % CR]; },
+              [&](int i, int) { return fc[(int64_t)(c0 + i / CR) * b.fvs + i % CR]; },
+              [&](int i, int p, double v) { Fr[(i / CR) * kMsFld + p] += v; });
+          __syncthreads();
+        }
+        // y = Lc^{-1} fv_c (y_i = sum_j Lt[j, i] fv_c[j]) and fv_R -= Zp fv_c
+        ms_product<T, true>(Lt, ld, c, c, Fr, kMsFld, kb, ms_none,
+                            [&](int i, int r, double, double v) { Yr[r * kMsYld + i] = v; });
+        ms_product<T, false>(Lt + c, ld, R, c, Fr, kMsFld, kb, ms_none,
+                             [&](int i, int r, double, double v) { Fr[r * kMsFld + c + i] -= v; });
+        __syncthreads();
+        // z = Lc^{-T} y
+        double* zb = b.z + (int64_t)c0 * b.n + S.first;
+        ms_product<T, false>(Lt, ld, c, c, Yr, kMsYld, kb, ms_none,
+                             [&](int i, int r, double, double v) {
+                               vst<true>(zb + (int64_t)r * b.n + i, v);
+                             });
+        double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
+        for (int idx = tid; idx < kb * R; idx += 256) {
+          const int r = idx / R, t = idx - r * R;
+          const double v = Fr[r * kMsFld + c + t];
+          if (root) vst<true>(fvR + (int64_t)r * b.fvs + t, v);
+          else fvR[(int64_t)r * b.fvs + t] = v;
+        }
       }
-      // y = Lc^{-1} fv_c (y_i = sum_j Lt[j, i] fv_c[j]) and fv_R -= Zp fv_c
-      ms_product<T, true>(Lt, ld, c, c, Fr, kMsFld, kb, ms_none,
-                          [&](int i, int r, double, double v) { Yr[r * kMsYld + i] = v; });
-      ms_product<T, false>(Lt + c, ld, R, c, Fr, kMsFld, kb, ms_none,
-                           [&](int i, int r, double, double v) { Fr[r * kMsFld + c + i] -= v; });
-      __syncthreads();
-      // z = Lc^{-T} y
-      double* zb = b.z + (int64_t)c0 * b.n + S.first;
-      ms_product<T, false>(Lt, ld, c, c, Yr, kMsYld, kb, ms_none,
-                           [&](int i, int r, double, double v) {
-                             vst<true>(zb + (int64_t)r * b.n + i, v);
-                           });
-      double* fvR = b.fv + S.fv_off + c + (int64_t)c0 * b.fvs;
-      for (int idx = tid; idx < kb * R; idx += 256) {
-        const int r = idx / R, t = idx - r * R;
-        const double v = Fr[r * kMsFld + c + t];
-        if (root) vst<true>(fvR + (int64_t)r * b.fvs + t, v);
-        else fvR[(int64_t)r * b.fvs + t] = v;
-      }
+      // the children are replaced by this node on the stack (its entries
+      // are read after the barrier below, by its parent)
+      sp -= S.nchild;
+      if (tid == 0) s_stk[sp] = MsChild{S.fv_off + c, S.relmap_off, R, 0};
+      ++sp;
       __syncthreads();
     }
   }
@@ -1600,16 +1682,30 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
 template <typename T>
 __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                             const VecBufs& b, int k, double* lds) {
+  __shared__ MsRing s_ring;
   double* Fr = lds;  // [16][kMsFld]: x_R at positions c .. c + R
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
+    int rbase = q1;  // list position of the ring's first descriptor
     for (int qi = q1 - 1; qi >= q0; --qi) {
+      if (qi < rbase) {
+        // descriptors of the next (at most) kMsRing supernodes going down
+        const int nr = min(kMsRing, qi - q0 + 1);
+        rbase = qi - nr + 1;
+        ms_ring_load(a, rbase, nr, false, s_ring);
+      }
       const int tid = body_tid();  // per supernode (not hoisted out of the loop)
-      const int jr = uni(a.nodes[qi]);
-      const int j = jr < 0 ? ~jr : jr;  // task roots are listed as ~j
-      const VecNode S = uniform_node(a.vn[j]);
-      const int64_t P = uni64(a.panel[j]);
+      const MsNodeDesc& D = s_ring.d[qi - rbase];
+      VecNode S{};
+      S.first = uni(D.first);
+      S.ncols = uni(D.ncols);
+      S.R = uni(D.R);
+      S.nchild = uni(D.nchild);
+      S.fv_off = uni64(D.fv_off);
+      S.rows_off = uni64(D.rows_off);
+      S.relmap_off = uni64(D.relmap_off);
+      const int64_t P = uni64(D.panel);
       const int c = S.ncols, R = S.R, ld = c + R;
       if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
         ms_backward_large<T>(F + P + c, S, a, b, c0, kb, lds);

@@ -45,21 +45,28 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi,
     double work = 0;  // supernodes plus front entries / 150 (relative cost)
   };
   std::vector<Tree> trees;
-  std::vector<int> stack;
   for (int j = 0; j < ns; ++j) {
     const int par = P.nodes[j].parent;
     if (!ok[j] || (par >= 0 && ok[par])) continue;
     Tree t;
     t.roots.push_back(j);
-    stack.assign(1, j);
-    while (!stack.empty()) {
-      const int q = stack.back();
-      stack.pop_back();
+    // depth-first postorder, children in increasing order: every supernode
+    // follows its whole subtree, so the processed supernodes of a task form
+    // a stack whose top entries are the next supernode's children
+    std::vector<std::pair<int, int>> dfs{{j, 0}};  // (node, next child)
+    while (!dfs.empty()) {
+      auto& top = dfs.back();
+      const std::vector<int>& ch = P.nodes[top.first].children;
+      if (top.second < (int)ch.size()) {
+        const int c = ch[top.second++];
+        dfs.push_back({c, 0});
+        continue;
+      }
+      const int q = top.first;
+      dfs.pop_back();
       t.nodes.push_back(q);
       t.work += 1.0 + (P.nodes[q].ncols + P.nodes[q].R()) / 150.0;
-      for (int ch : P.nodes[q].children) stack.push_back(ch);
     }
-    std::sort(t.nodes.begin(), t.nodes.end());
     trees.push_back(std::move(t));
   }
   if (max_tasks > 0 && (int)trees.size() > max_tasks) {
@@ -91,6 +98,29 @@ void build_subtrees(const Plan& P, int64_t max_elems, Subtrees& S, bool multi,
     // tasks in the order of their first root
     std::sort(trees.begin(), trees.end(),
               [](const Tree& u, const Tree& v) { return u.roots[0] < v.roots[0]; });
+  }
+  if (multi) {
+    // the device runs a task's supernodes in list order with the processed
+    // ones on a stack: each supernode's children must be the top entries,
+    // in increasing order, and the stack must fit; otherwise no subtree
+    // tasks (every supernode runs as ordinary tasks)
+    size_t depth = 0;
+    std::vector<int> stk;
+    for (const Tree& t : trees) {
+      stk.clear();
+      for (int q : t.nodes) {
+        const std::vector<int>& ch = P.nodes[q].children;
+        if (stk.size() < ch.size() || !std::equal(ch.begin(), ch.end(), stk.end() - ch.size()))
+          throw std::logic_error("subtree task: children are not the stack's top entries");
+        stk.resize(stk.size() - ch.size());
+        stk.push_back(q);
+        depth = std::max(depth, stk.size());
+      }
+    }
+    if (depth > (size_t)kMsStack) {
+      S.of.assign(ns, -1);
+      return;
+    }
   }
   for (const Tree& t : trees) {
     const int id = (int)S.root.size();

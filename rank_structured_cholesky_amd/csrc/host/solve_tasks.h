@@ -100,6 +100,9 @@ constexpr int kMsMaxFront = 224;
 // Larger supernodes of such subtrees keep fv_c and y in LDS and fv_R in
 // device memory (any R): at most kMlMaxCols columns.
 constexpr int kMlMaxCols = 128;
+// A multi-right-hand-side subtree task keeps its processed supernodes on a
+// stack of this many entries (a supernode's children are its top entries).
+constexpr int kMsStack = 64;
 // A supernode of a multi-right-hand-side subtree task: uncompressed, and
 // either small (front in LDS) or at most kMlMaxCols columns.
 inline bool multi_subtree_node(int ncols, int R) {
@@ -109,8 +112,8 @@ inline bool multi_subtree_node(int ncols, int R) {
 // Subtrees of uncompressed supernodes handled by one task each.
 struct Subtrees {
   std::vector<int> of;          // per supernode: subtree task id, -1 if none
-  std::vector<int> ptr, nodes;  // nodes of task s (each tree's children before
-                                // its parents)
+  std::vector<int> ptr, nodes;  // nodes of task s: each tree in depth-first
+                                // postorder (a node follows its subtree)
   std::vector<int> root;        // first root supernode of task s
   std::vector<int> rptr, roots;  // every root of task s: roots[rptr[s] .. rptr[s + 1])
   std::vector<int> cols;        // per task: first column, column count

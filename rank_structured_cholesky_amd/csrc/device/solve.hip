@@ -1597,6 +1597,11 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
     int sp = 0;  // stack depth (the same in every thread)
+    // the right-hand-side entries of the next supernode, loaded during the
+    // step of the one before it when both are small and the next
+    // descriptor is in the ring
+    bool pre = false;
+    double nbv[2] = {0.0, 0.0};
     for (int qi = q0; qi < q1; ++qi) {
       if ((qi - q0) % kMsRing == 0) ms_ring_load(a, qi, min(kMsRing, q1 - qi), true, s_ring);
       const int tid = body_tid();  // per supernode (not hoisted out of the loop)
@@ -1617,13 +1622,14 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
       const MsChild* ch = s_stk + sp - S.nchild;  // the children, in order
       if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
         ms_forward_large<T>(Lt, S, a, b, c0, kb, root, ch, lds);
+        pre = false;
       } else {
         for (int idx = tid; idx < kMfmaRhs * ld; idx += 256) {
           const int r = idx / ld, p = idx - r * ld;
           if (r >= kb || p >= c) Fr[r * kMsFld + p] = 0.0;
         }
         // fv_c = b[perm[cols]] (original indices staged with the
-        // descriptors): every load of the node at once
+        // descriptors), loaded during the previous supernode's step or now
         {
           const int* pm = s_ring.perm[l];
           double v[2];
@@ -1632,13 +1638,27 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
           for (int u = 0; u < 2; ++u) {
             const int i = tid + 256 * u;
             ii[u] = i;
-            v[u] = i < kb * c ? a.B[pm[i % c] + (int64_t)(c0 + i / c) * a.ldb] : 0.0;
+            v[u] = pre ? nbv[u]
+                       : (i < kb * c ? a.B[pm[i % c] + (int64_t)(c0 + i / c) * a.ldb] : 0.0);
           }
 #pragma unroll
           for (int u = 0; u < 2; ++u)
             if (ii[u] < kb * c) Fr[(ii[u] / c) * kMsFld + ii[u] % c] = v[u];
         }
         __syncthreads();
+        pre = false;
+        if (l + 1 < kMsRing && qi + 1 < q1) {
+          const int cn = uni(s_ring.d[l + 1].ncols), Rn = uni(s_ring.d[l + 1].R);
+          if (cn <= kMsMaxCols && cn + Rn <= kMsMaxFront) {
+            pre = true;
+            const int* pn = s_ring.perm[l + 1];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const int i = tid + 256 * u;
+              nbv[u] = i < kb * cn ? a.B[pn[i % cn] + (int64_t)(c0 + i / cn) * a.ldb] : 0.0;
+            }
+          }
+        }
         for (int q = 0; q < S.nchild; ++q) {
           const int* rm = a.relmap + uni64(ch[q].rm);
           const double* fc = b.fv + uni64(ch[q].fvr);
@@ -1916,6 +1936,10 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
     __syncthreads();
     const int ticket = uni(s_task);
     if (ticket < 0) break;
+    // the completion step's groups and counts, read with the descriptor
+    const DfTask& tk = a.tasks[ticket];
+    const int g1 = uni(tk.group), g2 = uni(tk.group2);
+    const unsigned n1 = (unsigned)uni((int)tk.need1), n2 = (unsigned)uni((int)tk.need2);
     df_run_task<T, MULTI>(F, a, b, k, ticket, red, lds);
     // every thread: its device-scope stores have completed before the count
     __builtin_amdgcn_s_waitcnt(0);
@@ -1925,9 +1949,6 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
       // both completion counts go out together; only the task completing a
       // group does more (its successors' pending counts, then push slots
       // for those that became ready)
-      const DfTask& tk = a.tasks[ticket];
-      const int g1 = tk.group, g2 = tk.group2;
-      const unsigned n1 = a.q_need[g1], n2 = g2 >= 0 ? a.q_need[g2] : 0u;
       const unsigned o1 =
           __hip_atomic_fetch_add(a.cnt + g1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned o2 =

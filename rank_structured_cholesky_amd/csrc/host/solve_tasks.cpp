@@ -939,6 +939,8 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     t.wend = gwoff[t.group + 1];
     t.wg0 = t.wend > t.wbeg ? gw[t.wbeg] : -1;
     t.wn0 = t.wg0 >= 0 ? need[t.wg0] : 0u;
+    t.need1 = need[t.group];
+    t.need2 = t.group2 >= 0 ? need[t.group2] : 0u;
   }
   {
     // per task for timing traces: the cost model's estimate and, for

@@ -73,6 +73,8 @@ struct DfTask {
   int tag;     // (supernode + 1) << 8 | stage, for traces (supernode -1: scatter;
                // stage 0 assembly, 1 gather, 2 scatter, 3 / 4 subtree forward /
                // backward, 10+k chain A, 20+k chain B, 30+k chain C)
+  unsigned need1, need2;  // task counts completing group and group2 (0: none),
+                          // read with the descriptor for the completion step
 };
 
 // One entry of the wait list: a group and the task count that completes it.

@@ -545,6 +545,8 @@ int main(int argc, char** argv) {
           if (cnt[D.gw[q]].load(std::memory_order_acquire) != D.need[D.gw[q]])
             fail("queued task run before its waits completed");
         run_df_task(S, D, t);
+        if (t.need1 != D.need[t.group] || t.need2 != (t.group2 >= 0 ? D.need[t.group2] : 0u))
+          fail("task descriptor holds wrong completion counts");
         const int gs[2] = {t.group, t.group2};
         for (int g : gs) {
           if (g < 0) continue;

@@ -107,7 +107,7 @@ struct DfList {
   DBuf tasks, gv, gw, cnt;
   // ready-queue dispatch: queue slots, pop/push counters, pending waits per
   // group and their initial values, tasks per group, successor groups
-  DBuf q_queue, q_ctr, q_pend, q_pend0, q_need, q_gptr, q_gtasks, q_sptr, q_succ, q_init;
+  DBuf q_queue, q_ctr, q_pend, q_pend0, q_need, q_gptr, q_gtasks, q_ginfo, q_sptr, q_succ, q_init;
   int q_n0 = 0;
   int ntasks = 0, ngroups = 0, grid = 1;
   // kept for the optional timing trace
@@ -1443,6 +1443,7 @@ void Engine<T>::finish_solve_setup() {
       D.q_need.upload(DT.need, st_);
       D.q_gptr.upload(DT.q_gptr, st_);
       D.q_gtasks.upload(DT.q_gtasks, st_);
+      D.q_ginfo.upload(DT.q_ginfo.empty() ? std::vector<int>(4, 0) : DT.q_ginfo, st_);
       D.q_sptr.upload(DT.q_sptr, st_);
       D.q_succ.upload(DT.q_succ.empty() ? std::vector<int>{0} : DT.q_succ, st_);
       D.q_init.upload(DT.q_init.empty() ? std::vector<int>{0} : DT.q_init, st_);
@@ -2090,6 +2091,7 @@ void Engine<T>::run_apply(int k, const double* B, int ldb, double* X, int ldx,
       a.q_need = D.q_need.as<unsigned>();
       a.q_gptr = D.q_gptr.as<int>();
       a.q_gtasks = D.q_gtasks.as<int>();
+      a.q_ginfo = D.q_ginfo.as<int4>();
       a.q_sptr = D.q_sptr.as<int>();
       a.q_succ = D.q_succ.as<int>();
       dev::launch_df_reset_queue(D.cnt.as<unsigned>(), D.q_pend.as<int>(), D.q_pend0.as<int>(),

@@ -93,6 +93,7 @@ struct DfArgs {
   const unsigned* q_need;  // per group: tasks signalling it
   const int* q_gptr;     // tasks of group g: q_gtasks[q_gptr[g] .. q_gptr[g + 1])
   const int* q_gtasks;
+  const int4* q_ginfo;   // per group: (q_gptr[g], its task count, its first task, 0)
   const int* q_sptr;     // groups waiting on g: q_succ[q_sptr[g] .. q_sptr[g + 1])
   const int* q_succ;
 };

@@ -1910,7 +1910,7 @@ template <typename T, bool MULTI>
 __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const VecBufs& b,
                               int k, double (*red)[64], typename DfLds<MULTI>::type& lds) {
   __shared__ int s_task, s_next, s_np;
-  __shared__ int s_pg[kQPush], s_pb[kQPush], s_pf[kQPush];
+  __shared__ int s_pb[kQPush], s_pf[kQPush], s_pc[kQPush], s_pn[kQPush];
   __shared__ unsigned long long s_tgrab, s_tready;
   const int tid = body_tid();
   if (tid == 0) s_next = -1;
@@ -1973,21 +1973,23 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
                                : 0;
           for (int u = 0; u < 8; ++u) {
             if (hs[u] < 0 || ps[u] != 1) continue;
-            const int h = hs[u];
-            const int c0 = a.q_gptr[h], cn = a.q_gptr[h + 1] - c0;
+            // the group's tasks (range, count, first task) in one record
+            const int4 gi = a.q_ginfo[hs[u]];
+            const int c0 = gi.x, cn = gi.y;
             if (cn <= 0) continue;  // a group signalled by other groups' tasks
             const int base = (int)__hip_atomic_fetch_add(a.q_ctr + 1, (unsigned)cn,
                                                          __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT);
             int first = 0;
             if (next < 0) {
-              next = a.q_gtasks[c0];
+              next = gi.z;
               first = 1;
             }
             if (np < kQPush) {
-              s_pg[np] = h;
               s_pb[np] = base;
               s_pf[np] = first;
+              s_pc[np] = c0;
+              s_pn[np] = cn;
               ++np;
             } else {
               for (int i = 0; i < cn; ++i)
@@ -2011,8 +2013,8 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
     __syncthreads();
     const int np = uni(s_np);
     for (int r = 0; r < np; ++r) {
-      const int h = uni(s_pg[r]), base = uni(s_pb[r]), first = uni(s_pf[r]);
-      const int c0 = uni(a.q_gptr[h]), cn = uni(a.q_gptr[h + 1]) - c0;
+      const int base = uni(s_pb[r]), first = uni(s_pf[r]);
+      const int c0 = uni(s_pc[r]), cn = uni(s_pn[r]);
       for (int i = tid; i < cn; i += 256)
         __hip_atomic_store(a.q_queue + base + i, i < first ? kQSkip : a.q_gtasks[c0 + i],
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -1025,6 +1025,13 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     for (int g = 0; g < ng; ++g)
       std::stable_sort(out.q_succ.begin() + out.q_sptr[g], out.q_succ.begin() + out.q_sptr[g + 1],
                        [&](int x, int y) { return bl[x] > bl[y]; });
+    out.q_ginfo.assign(4 * (size_t)ng, 0);
+    for (int g = 0; g < ng; ++g) {
+      const int c0 = out.q_gptr[g], cn = out.q_gptr[g + 1] - c0;
+      out.q_ginfo[4 * g] = c0;
+      out.q_ginfo[4 * g + 1] = cn;
+      out.q_ginfo[4 * g + 2] = cn > 0 ? out.q_gtasks[c0] : -1;
+    }
     out.q_init.clear();
     for (size_t i = 0; i < tasks.size(); ++i)
       if (out.q_pend0[tasks[i].group] == 0) out.q_init.push_back((int)i);

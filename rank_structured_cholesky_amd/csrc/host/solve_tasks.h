@@ -175,6 +175,8 @@ struct DataflowTasks {
   // Ready-queue form of the same graph: a group's tasks are pushed to the
   // queue once every group it waits on has completed.
   std::vector<int> q_gptr, q_gtasks;  // tasks of group g: q_gtasks[q_gptr[g] ..)
+  std::vector<int> q_ginfo;           // per group, one 16-byte record: q_gptr[g],
+                                      // its task count, its first task (-1: none), 0
   std::vector<int> q_sptr, q_succ;    // groups waiting on group g: q_succ[q_sptr[g] ..)
   std::vector<int> q_pend0;           // groups each group waits on
   std::vector<int> q_init;            // tasks ready at the start, in list order

@@ -517,8 +517,14 @@ int main(int argc, char** argv) {
     std::atomic<int> head{0}, tail{(int)D.q_init.size()};
     std::atomic<bool> stuck{false};
     if ((int)D.q_gptr.size() != ng + 1 || (int)D.q_sptr.size() != ng + 1 ||
-        (int)D.q_gtasks.size() != nt)
+        (int)D.q_gtasks.size() != nt || (int)D.q_ginfo.size() != 4 * ng)
       fail("ready-queue arrays have the wrong sizes");
+    for (int g = 0; g < ng; ++g) {
+      const int c0 = D.q_gptr[g], cn = D.q_gptr[g + 1] - c0;
+      if (D.q_ginfo[4 * g] != c0 || D.q_ginfo[4 * g + 1] != cn ||
+          D.q_ginfo[4 * g + 2] != (cn > 0 ? D.q_gtasks[c0] : -1))
+        fail("a group's ready-queue record disagrees with its task list");
+    }
     constexpr int kSkip = -3;
     auto worker = [&]() {
       int next = -1;  // the task this worker made ready and runs itself

@@ -1940,6 +1940,7 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
     const DfTask& tk = a.tasks[ticket];
     const int g1 = uni(tk.group), g2 = uni(tk.group2);
     const unsigned n1 = (unsigned)uni((int)tk.need1), n2 = (unsigned)uni((int)tk.need2);
+    const int sp[4] = {uni(tk.succ1[0]), uni(tk.succ1[1]), uni(tk.succ2[0]), uni(tk.succ2[1])};
     df_run_task<T, MULTI>(F, a, b, k, ticket, red, lds);
     // every thread: its device-scope stores have completed before the count
     __builtin_amdgcn_s_waitcnt(0);
@@ -1960,7 +1961,7 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
       for (int e = 0; e < 2; ++e) {
         const int g = gs[e];
         if (g < 0) continue;
-        const int q0 = a.q_sptr[g], q1 = a.q_sptr[g + 1];
+        const int q0 = sp[2 * e], q1 = sp[2 * e + 1];
         for (int qb = q0; qb < q1; qb += 8) {
           // up to eight successors' pending counts at once
           int hs[8], ps[8];

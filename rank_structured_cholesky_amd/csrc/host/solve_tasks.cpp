@@ -1025,6 +1025,12 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     for (int g = 0; g < ng; ++g)
       std::stable_sort(out.q_succ.begin() + out.q_sptr[g], out.q_succ.begin() + out.q_sptr[g + 1],
                        [&](int x, int y) { return bl[x] > bl[y]; });
+    for (auto& t : tasks) {
+      t.succ1[0] = out.q_sptr[t.group];
+      t.succ1[1] = out.q_sptr[t.group + 1];
+      t.succ2[0] = t.group2 >= 0 ? out.q_sptr[t.group2] : 0;
+      t.succ2[1] = t.group2 >= 0 ? out.q_sptr[t.group2 + 1] : 0;
+    }
     out.q_ginfo.assign(4 * (size_t)ng, 0);
     for (int g = 0; g < ng; ++g) {
       const int c0 = out.q_gptr[g], cn = out.q_gptr[g + 1] - c0;

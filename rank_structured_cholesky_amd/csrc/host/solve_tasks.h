@@ -75,6 +75,8 @@ struct DfTask {
                // backward, 10+k chain A, 20+k chain B, 30+k chain C)
   unsigned need1, need2;  // task counts completing group and group2 (0: none),
                           // read with the descriptor for the completion step
+  int succ1[2], succ2[2];  // groups waiting on group / group2: ready-queue
+                           // successor list ranges [begin, end)
 };
 
 // One entry of the wait list: a group and the task count that completes it.

@@ -525,6 +525,11 @@ int main(int argc, char** argv) {
           D.q_ginfo[4 * g + 2] != (cn > 0 ? D.q_gtasks[c0] : -1))
         fail("a group's ready-queue record disagrees with its task list");
     }
+    for (const auto& t : D.tasks)
+      if (t.succ1[0] != D.q_sptr[t.group] || t.succ1[1] != D.q_sptr[t.group + 1] ||
+          (t.group2 >= 0 &&
+           (t.succ2[0] != D.q_sptr[t.group2] || t.succ2[1] != D.q_sptr[t.group2 + 1])))
+        fail("a task descriptor's successor ranges disagree with the queue lists");
     constexpr int kSkip = -3;
     auto worker = [&]() {
       int next = -1;  // the task this worker made ready and runs itself

@@ -1941,6 +1941,7 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
     const int g1 = uni(tk.group), g2 = uni(tk.group2);
     const unsigned n1 = (unsigned)uni((int)tk.need1), n2 = (unsigned)uni((int)tk.need2);
     const int sp[4] = {uni(tk.succ1[0]), uni(tk.succ1[1]), uni(tk.succ2[0]), uni(tk.succ2[1])};
+    const int sf[2] = {uni(tk.sfirst[0]), uni(tk.sfirst[1])};
     df_run_task<T, MULTI>(F, a, b, k, ticket, red, lds);
     // every thread: its device-scope stores have completed before the count
     __builtin_amdgcn_s_waitcnt(0);
@@ -1963,19 +1964,24 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
         if (g < 0) continue;
         const int q0 = sp[2 * e], q1 = sp[2 * e + 1];
         for (int qb = q0; qb < q1; qb += 8) {
-          // up to eight successors' pending counts at once
+          // up to eight successors' pending counts at once (the first
+          // successor comes with the descriptor), and with them the
+          // successors' task records (range, count, first task)
           int hs[8], ps[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) hs[u] = qb + u < q1 ? a.q_succ[qb + u] : -1;
+          int4 gr[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u)
+            hs[u] = qb + u < q1 ? (qb + u == q0 ? sf[e] : a.q_succ[qb + u]) : -1;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
             ps[u] = hs[u] >= 0 ? __hip_atomic_fetch_add(a.q_pend + hs[u], -1, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT)
                                : 0;
+            gr[u] = hs[u] >= 0 ? a.q_ginfo[hs[u]] : make_int4(0, 0, -1, 0);
+          }
           for (int u = 0; u < 8; ++u) {
             if (hs[u] < 0 || ps[u] != 1) continue;
-            // the group's tasks (range, count, first task) in one record
-            const int4 gi = a.q_ginfo[hs[u]];
+            const int4 gi = gr[u];
             const int c0 = gi.x, cn = gi.y;
             if (cn <= 0) continue;  // a group signalled by other groups' tasks
             const int base = (int)__hip_atomic_fetch_add(a.q_ctr + 1, (unsigned)cn,

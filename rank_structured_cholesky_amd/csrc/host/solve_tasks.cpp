@@ -1030,6 +1030,8 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
       t.succ1[1] = out.q_sptr[t.group + 1];
       t.succ2[0] = t.group2 >= 0 ? out.q_sptr[t.group2] : 0;
       t.succ2[1] = t.group2 >= 0 ? out.q_sptr[t.group2 + 1] : 0;
+      t.sfirst[0] = t.succ1[1] > t.succ1[0] ? out.q_succ[t.succ1[0]] : -1;
+      t.sfirst[1] = t.succ2[1] > t.succ2[0] ? out.q_succ[t.succ2[0]] : -1;
     }
     out.q_ginfo.assign(4 * (size_t)ng, 0);
     for (int g = 0; g < ng; ++g) {

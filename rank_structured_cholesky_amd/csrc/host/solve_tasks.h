@@ -77,6 +77,7 @@ struct DfTask {
                           // read with the descriptor for the completion step
   int succ1[2], succ2[2];  // groups waiting on group / group2: ready-queue
                            // successor list ranges [begin, end)
+  int sfirst[2];           // the first entry of each of those lists (-1: empty)
 };
 
 // One entry of the wait list: a group and the task count that completes it.

@@ -530,6 +530,10 @@ int main(int argc, char** argv) {
           (t.group2 >= 0 &&
            (t.succ2[0] != D.q_sptr[t.group2] || t.succ2[1] != D.q_sptr[t.group2 + 1])))
         fail("a task descriptor's successor ranges disagree with the queue lists");
+    for (const auto& t : D.tasks)
+      if (t.sfirst[0] != (t.succ1[1] > t.succ1[0] ? D.q_succ[t.succ1[0]] : -1) ||
+          (t.group2 >= 0 && t.sfirst[1] != (t.succ2[1] > t.succ2[0] ? D.q_succ[t.succ2[0]] : -1)))
+        fail("a task descriptor's first successors disagree with the queue lists");
     constexpr int kSkip = -3;
     auto worker = [&]() {
       int next = -1;  // the task this worker made ready and runs itself

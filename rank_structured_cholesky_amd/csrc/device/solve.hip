@@ -1296,9 +1296,10 @@ static_assert(kMlMaxCols <= 128, "at most two 16-row blocks per wave");
 // four in increasing order for every right-hand side (so a column's bits do
 // not depend on kb).
 //
-// The epilogue reads the four old values a lane needs (get(row, r), e.g.
-// from device memory) before writing any result (put(row, r, old, v)), so
-// a read-modify-write epilogue costs one round trip per row block.
+// The four old values a lane's epilogue needs (get(row, r), e.g. from
+// device memory) are read before the pass's matrix loads, and every result
+// (put(row, r, old, v)) is written after them, so a read-modify-write
+// epilogue adds no round trip of its own.
 template <typename T, bool TRANS, typename Get, typename Put>
 __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
                            const double* xs, int xld, int kb, Get get, Put put) {
@@ -1308,6 +1309,15 @@ __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
   // of 32: the sixteen matrix loads of a chunk go out together
   for (int i0 = wave * 16; i0 < rows; i0 += 128) {
     const bool two = i0 + 64 < rows;
+    // the old values go out first: they arrive with the matrix loads
+    double old[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = i0 + 64 * h + kq + 4 * v;
+        old[h][v] = (row < rows && li < kb) ? get(row, li) : 0.0;
+      }
     d4v acc[2];
     acc[0] = d4v{0, 0, 0, 0};
     acc[1] = d4v{0, 0, 0, 0};
@@ -1335,15 +1345,6 @@ __device__ void ms_product(const T* __restrict__ M, int ld, int rows, int cols,
         if (two) acc[1] = mfma64((double)m[1][u], xv, acc[1]);
       }
     }
-    // every old value of both blocks is read before any result is written
-    double old[2][4];
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = i0 + 64 * h + kq + 4 * v;
-        old[h][v] = (row < rows && li < kb) ? get(row, li) : 0.0;
-      }
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -1470,7 +1471,10 @@ __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
     const double* fc = b.fv + uni64(ch[q].fvr) + (int64_t)c0 * b.fvs;
     const int CR = uni(ch[q].R);
     // a child's entries land on distinct positions: every load of a round
-    // (its values, and the old fv_R entries they are added to) goes first
+    // (its values, and the old fv_R entries they are added to) goes first;
+    // the first child adds to the zeros stored above, so it stores its
+    // values without loading them
+    const bool fresh = q == 0;
     for (int base = tid; base < kb * CR; base += 256 * 4) {
       int pp[4];
       double v[4], o[4];
@@ -1483,7 +1487,8 @@ __device__ void ms_forward_large(const T* __restrict__ Lt, const VecNode& S,
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int i = base + 256 * u;
-        o[u] = (i < kb * CR && pp[u] >= c) ? fvR[(int64_t)(i / CR) * b.fvs + pp[u] - c] : 0.0;
+        o[u] = (!fresh && i < kb * CR && pp[u] >= c) ? fvR[(int64_t)(i / CR) * b.fvs + pp[u] - c]
+                                                     : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {

@@ -1944,9 +1944,9 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
     // the completion step's groups and counts, read with the descriptor
     const DfTask& tk = a.tasks[ticket];
     const int g1 = uni(tk.group), g2 = uni(tk.group2);
-    const unsigned n1 = (unsigned)uni((int)tk.need1), n2 = (unsigned)uni((int)tk.need2);
     const int sp[4] = {uni(tk.succ1[0]), uni(tk.succ1[1]), uni(tk.succ2[0]), uni(tk.succ2[1])};
     const int sf[2] = {uni(tk.sfirst[0]), uni(tk.sfirst[1])};
+    const unsigned n1 = (unsigned)uni((int)tk.need1), n2 = (unsigned)uni((int)tk.need2);
     df_run_task<T, MULTI>(F, a, b, k, ticket, red, lds);
     // every thread: its device-scope stores have completed before the count
     __builtin_amdgcn_s_waitcnt(0);
@@ -1956,14 +1956,21 @@ __device__ void df_queue_loop(const T* __restrict__ F, const DfArgs& a, const Ve
       // both completion counts go out together; only the task completing a
       // group does more (its successors' pending counts, then push slots
       // for those that became ready)
-      const unsigned o1 =
-          __hip_atomic_fetch_add(a.cnt + g1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned o2 =
-          g2 >= 0 ? __hip_atomic_fetch_add(a.cnt + g2, 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT)
-                  : 0u;
+      // a task of a small group counts down the pending counts of the groups
+      // waiting on it directly; in a larger group only the task completing
+      // it does (both completion counts go out together); the task bringing
+      // a pending count to zero pushes that group's tasks
+      const bool d1 = n1 <= kQDirectMax, d2 = g2 >= 0 && n2 <= kQDirectMax;
+      const unsigned o1 = d1 ? 0u
+                             : __hip_atomic_fetch_add(a.cnt + g1, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned o2 = (g2 < 0 || d2) ? 0u
+                                         : __hip_atomic_fetch_add(a.cnt + g2, 1u,
+                                                                  __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
       int np = 0, next = -1;
-      const int gs[2] = {o1 + 1u == n1 ? g1 : -1, (g2 >= 0 && o2 + 1u == n2) ? g2 : -1};
+      const int gs[2] = {(d1 || o1 + 1u == n1) ? g1 : -1,
+                         (g2 >= 0 && (d2 || o2 + 1u == n2)) ? g2 : -1};
       for (int e = 0; e < 2; ++e) {
         const int g = gs[e];
         if (g < 0) continue;

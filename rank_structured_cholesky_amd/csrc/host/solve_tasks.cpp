@@ -998,7 +998,9 @@ void build_dataflow_tasks(const Plan& P, const SolveLayout& L, DataflowTasks& ou
     out.q_sptr.assign(ng + 1, 0);
     out.q_pend0.assign(ng, 0);
     for (int h = 0; h < ng; ++h) {
-      out.q_pend0[h] = gwoff[h + 1] - gwoff[h];
+      // a small group's tasks each count down h once, a larger group once
+      for (int q = gwoff[h]; q < gwoff[h + 1]; ++q)
+        out.q_pend0[h] += need[gw[q]] <= dev::kQDirectMax ? (int)need[gw[q]] : 1;
       for (int q = gwoff[h]; q < gwoff[h + 1]; ++q) ++out.q_sptr[gw[q] + 1];
     }
     for (int g = 0; g < ng; ++g) out.q_sptr[g + 1] += out.q_sptr[g];

@@ -80,6 +80,12 @@ struct DfTask {
   int sfirst[2];           // the first entry of each of those lists (-1: empty)
 };
 
+// Ready-queue completion: a task of a group of at most kQDirectMax tasks
+// counts down the pending counts of the groups waiting on it directly; in
+// larger groups only the task completing the group does (one count-down per
+// group instead of one per task).
+constexpr unsigned kQDirectMax = 4;
+
 // One entry of the wait list: a group and the task count that completes it.
 struct DfWait {
   int group;
@@ -181,7 +187,8 @@ struct DataflowTasks {
   std::vector<int> q_ginfo;           // per group, one 16-byte record: q_gptr[g],
                                       // its task count, its first task (-1: none), 0
   std::vector<int> q_sptr, q_succ;    // groups waiting on group g: q_succ[q_sptr[g] ..)
-  std::vector<int> q_pend0;           // groups each group waits on
+  std::vector<int> q_pend0;           // per group: count-downs it waits for (each group
+                                      // waited on: its task count up to kQDirectMax, else 1)
   std::vector<int> q_init;            // tasks ready at the start, in list order
 };
 

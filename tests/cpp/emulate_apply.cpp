@@ -498,8 +498,9 @@ int main(int argc, char** argv) {
 
   // (4) the ready-queue form on threads, as the device runs it: pop a slot,
   // wait until it is written, run the task (every wait must be complete),
-  // signal its groups; the task completing a group's last wait pushes the
-  // tasks of that group
+  // count down the pending counts of the groups waiting on its groups (a
+  // small group's tasks each, a larger group's last task once); the task
+  // bringing a group's count to zero pushes the tasks of that group
   S.reset_work();
   {
     const int nt = (int)D.tasks.size();
@@ -565,7 +566,10 @@ int main(int argc, char** argv) {
         const int gs[2] = {t.group, t.group2};
         for (int g : gs) {
           if (g < 0) continue;
-          if (cnt[g].fetch_add(1u, std::memory_order_acq_rel) + 1u != D.need[g]) continue;
+          // a task of a small group counts down the groups waiting on g
+          // directly, in a larger group only the task completing it
+          const unsigned o = cnt[g].fetch_add(1u, std::memory_order_acq_rel);
+          if (D.need[g] > dev::kQDirectMax && o + 1u != D.need[g]) continue;
           for (int q = D.q_sptr[g]; q < D.q_sptr[g + 1]; ++q) {
             const int h = D.q_succ[q];
             if (pend[h].fetch_sub(1, std::memory_order_acq_rel) != 1) continue;

@@ -1708,10 +1708,12 @@ template <typename T>
 __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int sb,
                             const VecBufs& b, int k, double* lds) {
   __shared__ MsRing s_ring;
+  __shared__ int s_rw[kMsMaxFront];  // row indices of the current small supernode
   double* Fr = lds;  // [16][kMsFld]: x_R at positions c .. c + R
   const int q0 = uni(a.ptr[sb]), q1 = uni(a.ptr[sb + 1]);
   for (int c0 = 0; c0 < k; c0 += kMfmaRhs) {
     const int kb = min(kMfmaRhs, k - c0);
+    bool pre = false;  // s_rw holds supernode qi's rows (loaded during the step before)
     int rbase = q1;  // list position of the ring's first descriptor
     for (int qi = q1 - 1; qi >= q0; --qi) {
       if (qi < rbase) {
@@ -1734,12 +1736,30 @@ __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int s
       const int c = S.ncols, R = S.R, ld = c + R;
       if (!(c <= kMsMaxCols && ld <= kMsMaxFront)) {
         ms_backward_large<T>(F + P + c, S, a, b, c0, kb, lds);
+        pre = false;
         continue;
       }
-      const int* rw = a.rows + S.rows_off;
+      if (!pre) {
+        const int* rw = a.rows + S.rows_off;
+        for (int t = tid; t < R; t += 256) s_rw[t] = rw[t];
+        __syncthreads();
+      }
+      // the next supernode's rows (going down), when it is small and its
+      // descriptor is in the ring: loaded now, staged after the gather
+      bool pre_next = false;
+      int nrw = 0, Rn = 0;
+      if (qi - 1 >= rbase) {
+        const MsNodeDesc& Dn = s_ring.d[qi - 1 - rbase];
+        const int cn = uni(Dn.ncols);
+        Rn = uni(Dn.R);
+        if (cn <= kMsMaxCols && cn + Rn <= kMsMaxFront) {
+          pre_next = true;
+          if (tid < Rn) nrw = a.rows[uni64(Dn.rows_off) + tid];
+        }
+      }
       const double* wb = b.w + (int64_t)c0 * b.n;
       ms_batched<4>(
-          kb * R, [&](int i) { return rw[i % R]; },
+          kb * R, [&](int i) { return s_rw[i % R]; },
           [&](int i, int row) { return vld<true>(wb + (int64_t)(i / R) * b.n + row); },
           [&](int i, int, double v) { Fr[(i / R) * kMsFld + c + i % R] = v; });
       __syncthreads();
@@ -1750,6 +1770,9 @@ __device__ void ms_backward(const T* __restrict__ F, const SubtreeArgs& a, int s
           F + P + c, ld, c, R, Fr + c, kMsFld, kb,
           [&](int i, int r) { return vld<true>(zb + (int64_t)r * b.n + i); },
           [&](int i, int r, double z, double v) { vst<true>(xb + (int64_t)r * b.n + i, z - v); });
+      // s_rw was last read by the gather, before the barrier above
+      if (pre_next && tid < Rn) s_rw[tid] = nrw;
+      pre = pre_next;
       __syncthreads();
     }
   }

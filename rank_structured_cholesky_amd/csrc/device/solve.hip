@@ -1664,7 +1664,41 @@ __device__ void ms_forward(const T* __restrict__ F, const SubtreeArgs& a, int sb
             }
           }
         }
-        for (int q = 0; q < S.nchild; ++q) {
+        int q = 0;
+        if (S.nchild >= 2 && kb * uni(ch[0].R) <= 1024 && kb * uni(ch[1].R) <= 1024) {
+          // two children of one round each: both children's loads first,
+          // then their adds in order
+          const int* rm0 = a.relmap + uni64(ch[0].rm);
+          const int* rm1 = a.relmap + uni64(ch[1].rm);
+          const double* fc0 = b.fv + uni64(ch[0].fvr);
+          const double* fc1 = b.fv + uni64(ch[1].fvr);
+          const int CR0 = uni(ch[0].R), CR1 = uni(ch[1].R);
+          int p0[4], p1[4];
+          double v0[4], v1[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = tid + 256 * u;
+            const bool k0 = i < kb * CR0, k1 = i < kb * CR1;
+            p0[u] = k0 ? rm0[i % CR0] : 0;
+            v0[u] = k0 ? fc0[(int64_t)(c0 + i / CR0) * b.fvs + i % CR0] : 0.0;
+            p1[u] = k1 ? rm1[i % CR1] : 0;
+            v1[u] = k1 ? fc1[(int64_t)(c0 + i / CR1) * b.fvs + i % CR1] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = tid + 256 * u;
+            if (i < kb * CR0) Fr[(i / CR0) * kMsFld + p0[u]] += v0[u];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = tid + 256 * u;
+            if (i < kb * CR1) Fr[(i / CR1) * kMsFld + p1[u]] += v1[u];
+          }
+          __syncthreads();
+          q = 2;
+        }
+        for (; q < S.nchild; ++q) {
           const int* rm = a.relmap + uni64(ch[q].rm);
           const double* fc = b.fv + uni64(ch[q].fvr);
           const int CR = uni(ch[q].R);
